@@ -1,0 +1,172 @@
+"""Benchmark: SVO ray cast throughput on one MI355X.
+
+Workload (default): large_1080p - depth-11 octree (2048^3; 512 spheres of radius
+16-128 voxels united with a 5-octave fBm heightfield, ~32M surface leaves),
+1920x1080 primary rays plus one shadow ray per lit hit toward a fixed
+directional light, 32 orbit frames. One step = one frame (primary + shadow
+rays, shading, all per-pixel outputs written to HBM).
+
+Prints one JSON line. `value` = (primary + shadow rays traced) / wall time of
+the timed steps. `roofline` covers the frame kernel: algorithmic bytes per
+frame = 8 B per node record read (exact visit counts from the kernel) + 4 B per
+hit leaf payload + 25 B of outputs per pixel, divided by the kernel's average
+duration from HIP events on its stream.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+import __graft_entry__ as entry
+
+BASELINE_METRIC = "Mrays/s (1080p, depth-11 SVO, primary+shadow) + %HBM peak; fwd+bwd Mrays/s"
+HBM_PEAK_GBS = 8000.0
+OUT_BYTES_PER_PIXEL = 4 + 12 + 4 + 4 + 1  # hit_id, voxel, t, rgba, shadow
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--workload", default="large_1080p", choices=["large_1080p", "plain_512"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-frames", type=int, default=4, help="frames in the CPU baseline sample")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: every box here has one MI355X and no multi-GPU run is "
+              f"made (asked for --gpus {args.gpus}).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    yx = entry.load_package()
+    wl = yx.workloads
+    if args.workload == "large_1080p":
+        desc, w, h, nframes, mode = wl.LARGE_1080P, 1920, 1080, 32, yx.YX_LIGHT_SUN_SHADOW
+    else:
+        desc, w, h, nframes, mode = wl.PLAIN_512, 512, 512, 8, yx.YX_LIGHT_HEADLIGHT
+    t0 = time.time()
+    scene = yx.SVOData.build_synthetic(**desc)
+    info = scene.info()
+    build_s = time.time() - t0
+    print(f"[bench] scene {args.workload}: {info['num_nodes']} nodes, {info['num_leaves']} leaves, "
+          f"built in {build_s:.1f} s", file=sys.stderr, flush=True)
+    cams = wl.orbit_cameras(nframes, seed=0)
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    r = yx.SVORenderer()
+    r.SetScene(scene)
+    r.SetFOV(60.0)
+    r.SetResolution(w, h)
+    r.SetLight(wl.SUN, mode)
+    outs = {"hit_id": torch.empty((h, w), dtype=torch.int32, device=dev),
+            "voxel": torch.empty((h, w, 3), dtype=torch.int32, device=dev),
+            "t": torch.empty((h, w), dtype=torch.float32, device=dev),
+            "rgba": torch.empty((h, w), dtype=torch.int32, device=dev),
+            "shadow": torch.empty((h, w), dtype=torch.uint8, device=dev)}
+    visits = torch.empty((h, w), dtype=torch.int32, device=dev)
+
+    # untimed counting pass: rays and algorithmic bytes of every frame
+    rays_f, bytes_f = [], []
+    for pos, d in cams:
+        r.SetViewPos(pos)
+        r.SetViewDir(d)
+        r.Render(visits=visits, **outs)
+        torch.cuda.synchronize()
+        sh = outs["shadow"]
+        nshadow = int((sh < 2).sum().item())
+        nhit = int((outs["hit_id"] != -1).sum().item())
+        nvis = int(visits.to(torch.int64).sum().item())
+        rays_f.append(w * h + nshadow)
+        bytes_f.append(8 * nvis + 4 * nhit + OUT_BYTES_PER_PIXEL * w * h)
+    print(f"[bench] counting pass done: mean rays/frame {np.mean(rays_f):.0f}", file=sys.stderr, flush=True)
+
+    stream = torch.cuda.current_stream()
+
+    def step(k):
+        pos, d = cams[k % nframes]
+        r.SetViewPos(pos)
+        r.SetViewDir(d)
+        r.Render(stream=stream, **outs)
+
+    for k in range(args.warmup):
+        step(k)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        step(k)
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    t_end = time.perf_counter()
+    elapsed = t_end - t_start
+    kern_ms = [a.elapsed_time(b) for a, b in ev]
+    total_rays = sum(rays_f[k % nframes] for k in range(args.steps))
+    total_bytes = sum(bytes_f[k % nframes] for k in range(args.steps))
+    value = total_rays / elapsed / 1e6
+    avg_kernel_s = float(np.mean(kern_ms)) / 1e3
+    achieved = total_bytes / args.steps / avg_kernel_s / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(scene, cams, w, h, mode, wl.SUN, args.cpu_frames)
+
+    line = {
+        "metric": BASELINE_METRIC,
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "depth": desc["depth"], "resolution": [w, h],
+                   "frames": nframes, "rays": "primary+shadow" if mode == 2 else "primary",
+                   "nodes": info["num_nodes"], "leaves": info["num_leaves"],
+                   "mean_rays_per_frame": round(float(np.mean(rays_f)), 1)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "frame_kernel", "kernel_ms": round(avg_kernel_s * 1e3, 4),
+                     "algorithmic_bytes_per_launch": int(np.mean(bytes_f))},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(scene, cams, w, h, mode, sun, nframes):
+    """The oracle (line-faithful CPU restatement) on a bounded sample: `nframes`
+    of the orbit frames, row bands over the host threads (TreadedRenderer's
+    split, cell/ppu_renderer.cpp:121-144)."""
+    orc = entry.load_oracle().Oracle()
+    nodes, root = scene.nodes(), scene.GetRoot()
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    idx = np.linspace(0, len(cams) - 1, nframes).astype(int)
+    rays = 0
+    t0 = time.perf_counter()
+    for i in idx:
+        pos, d = cams[i]
+        o = orc.render(nodes, root, pos, d, fov=60.0, w=w, h=h, light_mode=mode, light=sun, threads=threads)
+        rays += o["rays"]
+    dt = time.perf_counter() - t0
+    return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"{len(idx)} of the {len(cams)} orbit frames (frames {idx.tolist()}), full "
+                      f"{w}x{h}, {rays} rays, {dt:.1f} s wall on {threads} threads"}
+
+
+if __name__ == "__main__":
+    main()

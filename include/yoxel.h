@@ -1,0 +1,154 @@
+/*
+ * yoxel.h - C ABI of the MI355X sparse-voxel-octree ray caster.
+ *
+ * Every entry point returns an int status (YX_OK == 0, negative on error) unless
+ * stated otherwise; yx_last_error() gives a message for the calling thread's last
+ * failure. Handles are opaque. No C++ exceptions cross this boundary. Device
+ * pointers are plain HIP device addresses; `stream` arguments are hipStream_t
+ * values passed as void* (NULL = the default stream).
+ *
+ * Reference interfaces replaced (paths are inside znah/yoxel-voxel):
+ *   ISVORenderer            cell/svorenderer.h:5-31
+ *   RendererBase            cell/renderer_base.h:7-62
+ *   SVOData                 cell/svodata.h:22-55
+ *   SVORenderer (GPU)       demo/SVORenderer.h:8-64, demo/SVORenderer.cpp:95-149
+ *   RunTrace (C ABI)        ntree/ntree_trace.cuh:22-28
+ *   VoxNode format          reaction/report/main.tex:36-56
+ */
+#ifndef YOXEL_H
+#define YOXEL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define YX_OK              0
+#define YX_ERR_INVALID_ARG (-1) /* NULL handle, zero resolution, bad config   */
+#define YX_ERR_OOM         (-2) /* host or device allocation failed           */
+#define YX_ERR_IO          (-3) /* missing / short / unwritable .vox file     */
+#define YX_ERR_FORMAT      (-4) /* child id out of range, cycle, bad header   */
+#define YX_ERR_HIP         (-5) /* a HIP runtime call failed                   */
+#define YX_ERR_NO_SCENE    (-6) /* render without a scene (reference: NULL)   */
+#define YX_ERR_RANGE       (-7) /* size beyond a documented limit             */
+
+/* ---- node format (reaction/report/main.tex:36-56) ------------------------ */
+/* flags: bits 0..7 leafFlags, bits 8..15 nullFlags, bit 19 emptyFlag.
+ * child[i] is a node id, a leaf payload (VoxData) when leafFlags bit i is set,
+ * or a special id >= 0x80000000 (EmptyNode / FullNode). Child index bit 0 is x,
+ * bit 1 is y, bit 2 is z. Leaf payloads of synthetic scenes are RGBA8 (r in the
+ * low byte). */
+#define YX_EMPTY_NODE 0x80000000u
+#define YX_FULL_NODE  0x80000001u
+#define YX_MISS       0xFFFFFFFFu /* hit_id of a ray that hits nothing */
+
+typedef struct yx_vox_node {
+    uint32_t flags;
+    uint32_t data;
+    uint32_t child[8];
+} yx_vox_node; /* 40 bytes, the reference's VoxNode */
+
+typedef struct yx_scene yx_scene;       /* host node array + root */
+typedef struct yx_renderer yx_renderer; /* camera state + device scene copy */
+
+/* ---- scenes (SVOData, cell/svodata.h:22-55) ------------------------------ */
+
+/* Synthetic scene recipe: spheres (centres uniform in [0,1)^3, radii uniform in
+ * [radius_min, radius_max] voxels) united with an optional fBm-Perlin
+ * heightfield; only surface voxels (inside, with at least one 6-neighbour
+ * outside; outside the domain counts as outside) become leaves. */
+typedef struct yx_scene_desc {
+    int32_t depth;          /* leaf grid is 2^depth per axis, 1..16            */
+    int32_t num_spheres;
+    float radius_min;       /* in voxels */
+    float radius_max;       /* in voxels */
+    int32_t heightfield;    /* 0 or 1 */
+    int32_t hf_octaves;     /* fBm octaves */
+    float hf_amplitude;     /* fraction of the extent */
+    float hf_base;          /* base height, fraction of the extent */
+    uint64_t seed;
+} yx_scene_desc;
+
+typedef struct yx_scene_info {
+    uint32_t root;
+    uint32_t num_nodes;
+    uint64_t num_leaves;
+    int32_t depth;          /* deepest leaf level (root children are level 1) */
+} yx_scene_info;
+
+/* SVOData::Load (cell/svodata.h:31-50): header root:u32, 2 x u32 (ignored),
+ * size:u32, then size x 40-byte VoxNode, little-endian. */
+int yx_scene_load_vox(const char* path, yx_scene** out);
+int yx_scene_save_vox(const yx_scene* scene, const char* path);
+/* Copies `num_nodes` VoxNode records (10 u32 each). */
+int yx_scene_from_nodes(const uint32_t* nodes, uint32_t num_nodes, uint32_t root,
+                        yx_scene** out);
+int yx_scene_build_synthetic(const yx_scene_desc* desc, yx_scene** out);
+int yx_scene_get_info(const yx_scene* scene, yx_scene_info* info);
+/* SVOData::GetRoot / operator[]: borrowed view of the node array (valid until
+ * yx_scene_free). */
+int yx_scene_nodes(const yx_scene* scene, const uint32_t** nodes, uint32_t* num_nodes,
+                   uint32_t* root);
+void yx_scene_free(yx_scene* scene);
+
+/* ---- renderer (ISVORenderer cell/svorenderer.h:5-31, SVORenderer) -------- */
+
+/* Light modes. The reference CPU shader puts the light at the eye
+ * (RendererBase::SetViewPos, cell/renderer_base.h:30-35): YX_LIGHT_HEADLIGHT.
+ * YX_LIGHT_SUN_SHADOW adds one shadow ray per lit hit toward a fixed
+ * directional light. */
+#define YX_LIGHT_HEADLIGHT  0
+#define YX_LIGHT_SUN        1
+#define YX_LIGHT_SUN_SHADOW 2
+
+/* Per-pixel outputs; any pointer may be NULL to skip that output. Device
+ * pointers for yx_render, row-major (y * width + x). */
+typedef struct yx_outputs {
+    uint32_t* hit_id; /* canonical parent VoxNode id * 8 + child slot, YX_MISS  */
+    int32_t* voxel;   /* 3 x int32 per pixel: integer cell coords at hit level,
+                         (-1,-1,-1) on a miss                                    */
+    float* t;         /* hit distance along the unit ray, -1 on a miss          */
+    uint32_t* rgba;   /* shaded RGBA8 (r low byte); (0,0,0,0) on a miss         */
+    uint8_t* shadow;  /* 0 lit, 1 shadowed, 2 no shadow ray cast                */
+    uint32_t* visits; /* node records read (primary + shadow), for byte counts  */
+} yx_outputs;
+
+int yx_renderer_create(yx_renderer** out);
+void yx_renderer_destroy(yx_renderer* r);
+/* SetScene: uploads the scene to HBM (re-laid out). The scene may be freed
+ * afterwards; the renderer keeps its own device copy. */
+int yx_renderer_set_scene(yx_renderer* r, const yx_scene* scene);
+int yx_renderer_set_view_pos(yx_renderer* r, const float pos[3]);
+int yx_renderer_set_view_dir(yx_renderer* r, const float dir[3]);
+int yx_renderer_set_view_up(yx_renderer* r, const float up[3]);
+int yx_renderer_set_resolution(yx_renderer* r, int32_t width, int32_t height);
+int yx_renderer_get_resolution(const yx_renderer* r, int32_t* width, int32_t* height);
+int yx_renderer_set_fov(yx_renderer* r, float fov_deg); /* horizontal, default 70 */
+int yx_renderer_set_light(yx_renderer* r, const float dir[3], int32_t mode);
+/* Ray directions of the current camera (RendererBase::InitRayDir,
+ * cell/renderer_base.h:50-61): dir0, du, dv as 9 floats. */
+int yx_renderer_ray_basis(const yx_renderer* r, float basis[9]);
+
+/* Render into caller-owned device buffers on `stream` (asynchronous). The GPU
+ * variant of RenderFrame; demo/SVORenderer.cpp:159-213 Render(void*). */
+int yx_render(yx_renderer* r, const yx_outputs* out, void* stream);
+/* RenderFrame (cell/ppu_renderer.cpp:76-85): synchronous; returns a pointer to
+ * a renderer-owned host RGBA8 buffer valid until the next call or resolution
+ * change. Fails with YX_ERR_NO_SCENE when no scene is set (the reference
+ * returns NULL). */
+int yx_render_frame(yx_renderer* r, const uint32_t** colors);
+
+/* Device memory used by the uploaded scene, in bytes. */
+int yx_renderer_scene_bytes(const yx_renderer* r, uint64_t* bytes);
+
+const char* yx_last_error(void);
+const char* yx_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* YOXEL_H */

@@ -1,0 +1,136 @@
+"""ctypes wrapper of the CPU restatement (liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker / baseline. Never a product path.
+"""
+import ctypes
+import pathlib
+import subprocess
+import threading
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+def build(plant=0, out=None):
+    """Compile the restatement with gcc (PLANT != 0 builds a planted-mistake
+    variant for the pin tests). Returns the library path."""
+    out = pathlib.Path(out) if out else LIB
+    subprocess.run(["make", "-s", "-C", str(HERE), f"OUT={out}", f"PLANT={plant}", "-B"], check=True)
+    return out
+
+
+class Oracle:
+    def __init__(self, path=None):
+        path = pathlib.Path(path) if path else LIB
+        if not path.exists():
+            build(out=path)
+        l = ctypes.CDLL(str(path))
+        l.oracle_init_ray_dir.argtypes = [_f32p, _f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int, _f32p]
+        l.oracle_init_ray_dir.restype = None
+        l.oracle_render_rows.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
+                                         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _f32p, _u32p, _i32p, _f32p, _u32p, _u8p, _u32p]
+        l.oracle_render_rows.restype = ctypes.c_longlong
+        l.oracle_trace_rays.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _f32p,
+                                        _f32p, ctypes.c_int, _u32p, _i32p, _f32p, _i32p, _f32p, _u32p]
+        l.oracle_trace_rays.restype = ctypes.c_int
+        l.oracle_pixel_dirs.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _f32p]
+        l.oracle_pixel_dirs.restype = None
+        l.oracle_plant.restype = ctypes.c_int
+        self.lib = l
+
+    @staticmethod
+    def _p(a, t):
+        return a.ctypes.data_as(t) if a is not None else None
+
+    def init_ray_dir(self, direction, up, fov, w, h):
+        d = np.asarray(direction, np.float32)
+        u = np.asarray(up, np.float32)
+        out = np.zeros(9, np.float32)
+        self.lib.oracle_init_ray_dir(self._p(d, _f32p), self._p(u, _f32p), float(fov), int(w), int(h),
+                                     self._p(out, _f32p))
+        return out
+
+    def pixel_dirs(self, basis, w, h):
+        b = np.asarray(basis, np.float32)
+        out = np.zeros((h, w, 3), np.float32)
+        self.lib.oracle_pixel_dirs(self._p(b, _f32p), w, h, self._p(out, _f32p))
+        return out
+
+    def render(self, nodes, root, pos, direction, up=(0, 0, 1), fov=70.0, w=64, h=64, light_mode=0,
+               light=(0, 0, 1), rows=None, threads=1):
+        """RenderRect over rows (default all) into full-frame arrays. `threads`
+        splits the rows into bands, as TreadedRenderer (cell/ppu_renderer.cpp:
+        121-144) does. Returns a dict of outputs plus 'rays' traced."""
+        nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
+        nn = nodes.size // 10
+        basis = self.init_ray_dir(direction, up, fov, w, h)
+        p = np.asarray(pos, np.float32)
+        lt = np.asarray(light, np.float32)
+        out = {"hit_id": np.full((h, w), 0xFFFFFFFF, np.uint32),
+               "voxel": np.full((h, w, 3), -1, np.int32),
+               "t": np.full((h, w), -1.0, np.float32),
+               "rgba": np.zeros((h, w), np.uint32),
+               "shadow": np.full((h, w), 2, np.uint8),
+               "visits": np.zeros((h, w), np.uint32)}
+        y0, y1 = (0, h) if rows is None else rows
+        bands = []
+        step = -(-(y1 - y0) // threads)
+        for i in range(threads):
+            a, b = y0 + i * step, min(y1, y0 + (i + 1) * step)
+            if a < b:
+                bands.append((a, b))
+        rays = [0] * len(bands)
+
+        def run(k):
+            a, b = bands[k]
+            rays[k] = self.lib.oracle_render_rows(
+                self._p(nodes, _u32p), nn, int(root), self._p(p, _f32p), self._p(basis, _f32p), w, h, a, b,
+                int(light_mode), self._p(lt, _f32p), self._p(out["hit_id"], _u32p),
+                self._p(out["voxel"], _i32p), self._p(out["t"], _f32p), self._p(out["rgba"], _u32p),
+                self._p(out["shadow"], _u8p), self._p(out["visits"], _u32p))
+
+        if len(bands) == 1:
+            run(0)
+        else:
+            ts = [threading.Thread(target=run, args=(k,)) for k in range(len(bands))]
+            for t in ts:
+                t.start()
+            for t in ts:
+                t.join()
+        if any(r < 0 for r in rays):
+            raise ValueError("oracle followed a child id out of range")
+        out["rays"] = int(sum(rays))
+        return out
+
+    def trace_rays(self, nodes, root, pos, dirs, occlusion=False):
+        nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = dirs.shape[0]
+        if pos.shape[0] == 1 and n > 1:
+            pos = np.ascontiguousarray(np.repeat(pos, n, axis=0))
+        hit = np.zeros(n, np.uint32)
+        vox = np.zeros((n, 3), np.int32)
+        t = np.zeros(n, np.float32)
+        lvl = np.zeros(n, np.int32)
+        t1 = np.zeros((n, 3), np.float32)
+        vis = np.zeros(n, np.uint32)
+        rc = self.lib.oracle_trace_rays(self._p(nodes, _u32p), nodes.size // 10, int(root), n,
+                                        self._p(pos, _f32p), self._p(dirs, _f32p), 1 if occlusion else 0,
+                                        self._p(hit, _u32p), self._p(vox, _i32p), self._p(t, _f32p),
+                                        self._p(lvl, _i32p), self._p(t1, _f32p), self._p(vis, _u32p))
+        if rc < 0:
+            raise ValueError("oracle followed a child id out of range")
+        return {"hit_id": hit, "voxel": vox, "t": t, "level": lvl, "t1": t1, "visits": vis}
+
+    def plant(self):
+        return self.lib.oracle_plant()

@@ -1,0 +1,258 @@
+"""Independent helpers for the tests: a VoxNode tree writer from leaf boxes (any
+node order), a tree walker, a float64 brute-force first-hit search, and a
+numpy restatement of the synthetic voxelizer's definition."""
+import numpy as np
+
+EMPTY = 0x80000000
+MISS = 0xFFFFFFFF
+MASK64 = (1 << 64) - 1
+
+
+# ---- trees from leaf boxes -------------------------------------------------
+
+def tree_from_leaves(leaves, order="bfs", seed=0):
+    """leaves: iterable of (level, x, y, z, payload) with level >= 1 and coords
+    in [0, 2^level). Returns (nodes[N,10] uint32, root). order: 'bfs' or
+    'shuffle' (random node ids, root anywhere)."""
+    nodes = {}  # (level, x, y, z) -> {"leaf": {slot: payload}, "child": set(slot)}
+
+    def node(key):
+        if key not in nodes:
+            nodes[key] = {"leaf": {}, "child": set()}
+        return nodes[key]
+
+    node((0, 0, 0, 0))
+    for (lv, x, y, z, payload) in leaves:
+        assert lv >= 1
+        slot = (x & 1) | ((y & 1) << 1) | ((z & 1) << 2)
+        pkey = (lv - 1, x >> 1, y >> 1, z >> 1)
+        node(pkey)["leaf"][slot] = int(payload) & 0xFFFFFFFF
+        # ancestors
+        cl, cx, cy, cz = pkey
+        while cl > 0:
+            s = (cx & 1) | ((cy & 1) << 1) | ((cz & 1) << 2)
+            par = (cl - 1, cx >> 1, cy >> 1, cz >> 1)
+            node(par)["child"].add(s)
+            cl, cx, cy, cz = par
+    for key, n in nodes.items():
+        assert not (set(n["leaf"]) & n["child"]), f"leaf overlaps a subtree at {key}"
+    keys = sorted(nodes, key=lambda k: (k[0], k[1], k[2], k[3]))
+    if order == "shuffle":
+        rng = np.random.default_rng(seed)
+        perm = rng.permutation(len(keys))
+        ids = {k: int(perm[i]) for i, k in enumerate(keys)}
+    else:
+        ids = {k: i for i, k in enumerate(keys)}
+    out = np.zeros((len(keys), 10), np.uint32)
+    for k, n in nodes.items():
+        i = ids[k]
+        flags = 0
+        lv, x, y, z = k
+        for s in range(8):
+            if s in n["leaf"]:
+                flags |= 1 << s
+                out[i, 2 + s] = n["leaf"][s]
+            elif s in n["child"]:
+                ck = (lv + 1, 2 * x + (s & 1), 2 * y + ((s >> 1) & 1), 2 * z + ((s >> 2) & 1))
+                out[i, 2 + s] = ids[ck]
+            else:
+                flags |= 1 << (8 + s)
+                out[i, 2 + s] = EMPTY
+        out[i, 0] = flags
+    return out, ids[(0, 0, 0, 0)]
+
+
+def walk_leaves(nodes, root):
+    """All leaves of a VoxNode tree: dict (level, x, y, z) -> (payload, parent id, slot)."""
+    res = {}
+    stack = [(int(root), 0, 0, 0, 0)]
+    while stack:
+        nid, lv, x, y, z = stack.pop()
+        n = nodes[nid]
+        for s in range(8):
+            cx, cy, cz = 2 * x + (s & 1), 2 * y + ((s >> 1) & 1), 2 * z + ((s >> 2) & 1)
+            if (int(n[0]) >> s) & 1:
+                res[(lv + 1, cx, cy, cz)] = (int(n[2 + s]), nid, s)
+            elif not (int(n[2 + s]) & 0x80000000):
+                stack.append((int(n[2 + s]), lv + 1, cx, cy, cz))
+    return res
+
+
+def dense_leaves(occ, rng=None):
+    """Leaves at the finest level of a dense boolean grid occ[x, y, z]."""
+    n = occ.shape[0]
+    lv = int(np.log2(n))
+    xs, ys, zs = np.nonzero(occ)
+    pay = (rng.integers(0, 2**32, size=xs.size, dtype=np.uint64) if rng is not None
+           else np.arange(xs.size, dtype=np.uint64) * 2654435761)
+    return [(lv, int(a), int(b), int(c), int(p) & 0xFFFFFFFF) for a, b, c, p in zip(xs, ys, zs, pay)]
+
+
+# ---- brute force ------------------------------------------------------------
+
+def adjust(d):
+    d = np.array(d, np.float32)
+    small = np.abs(d) < np.float32(1e-6)
+    d[small] = np.where(d[small] < 0, np.float32(-1e-6), np.float32(1e-6))
+    return d
+
+
+def brute_first_hit(leaf_keys, pos, dirs, eps=1e-5):
+    """First leaf box along each ray by exhaustive float64 slab tests.
+    Returns (idx, tin, ambiguous) with idx -1 for a miss. A ray is ambiguous
+    when the winner is not separated by eps from another candidate or from a
+    grazing box."""
+    keys = np.asarray(leaf_keys, np.float64)  # (L, 4): level, x, y, z
+    scale = 2.0 ** (-keys[:, 0])
+    lo = keys[:, 1:4] * scale[:, None]
+    hi = (keys[:, 1:4] + 1.0) * scale[:, None]
+    pos = np.asarray(pos, np.float64).reshape(-1, 3)
+    dirs = np.asarray(dirs, np.float64).reshape(-1, 3)
+    if pos.shape[0] == 1:
+        pos = np.repeat(pos, dirs.shape[0], axis=0)
+    n = dirs.shape[0]
+    idx = np.full(n, -1, np.int64)
+    tin_best = np.full(n, np.inf)
+    amb = np.zeros(n, bool)
+    for s in range(0, n, 256):
+        p = pos[s:s + 256, None, :]
+        d = dirs[s:s + 256, None, :]
+        ta = (lo[None] - p) / d
+        tb = (hi[None] - p) / d
+        tin = np.minimum(ta, tb).max(axis=2)
+        tout = np.maximum(ta, tb).min(axis=2)
+        solid = tout - tin > eps
+        graze = (tout - tin > -eps) & ~solid
+        tin_s = np.where(solid, tin, np.inf)
+        best = tin_s.argmin(axis=1)
+        bt = tin_s[np.arange(best.size), best]
+        tin_s2 = tin_s.copy()
+        tin_s2[np.arange(best.size), best] = np.inf
+        second = tin_s2.min(axis=1)
+        tg = np.where(graze, tin, np.inf).min(axis=1)
+        hit = np.isfinite(bt)
+        idx[s:s + 256] = np.where(hit, best, -1)
+        tin_best[s:s + 256] = bt
+        with np.errstate(invalid="ignore"):
+            amb[s:s + 256] = (second - bt < eps) | (tg < bt + eps)
+    return idx, tin_best, amb
+
+
+# ---- numpy restatement of the synthetic voxelizer ----------------------------
+
+def _mix64(z):
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9 & MASK64
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EB & MASK64
+    return z ^ (z >> 31)
+
+
+class SplitMix64:
+    def __init__(self, seed):
+        self.s = seed & MASK64
+
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & MASK64
+        return _mix64(self.s)
+
+    def uniform(self):
+        return float(self.next() >> 11) * 2.0 ** -53
+
+
+def perlin_perm(seed):
+    rng = SplitMix64(seed)
+    p = list(range(256))
+    for i in range(255, 0, -1):
+        j = rng.next() % (i + 1)
+        p[i], p[j] = p[j], p[i]
+    return np.array(p + p, np.int64)
+
+
+def _fade(t):
+    return t * t * t * (t * (t * 6.0 - 15.0) + 10.0)
+
+
+def _lerp(t, a, b):
+    return a + t * (b - a)
+
+
+def _grad2(h, x, y):
+    h = h & 7
+    out = np.empty_like(x)
+    table = [x + y, -x + y, x - y, -x - y, x, -x, y, -y]
+    for k in range(8):
+        m = h == k
+        out[m] = table[k][m]
+    return out
+
+
+def noise2(perm, x, y):
+    fx, fy = np.floor(x), np.floor(y)
+    xi = fx.astype(np.int64) & 255
+    yi = fy.astype(np.int64) & 255
+    xf, yf = x - fx, y - fy
+    u, v = _fade(xf), _fade(yf)
+    aa = perm[perm[xi] + yi]
+    ab = perm[perm[xi] + yi + 1]
+    ba = perm[perm[xi + 1] + yi]
+    bb = perm[perm[xi + 1] + yi + 1]
+    x1 = _lerp(u, _grad2(aa, xf, yf), _grad2(ba, xf - 1.0, yf))
+    x2 = _lerp(u, _grad2(ab, xf, yf - 1.0), _grad2(bb, xf - 1.0, yf - 1.0))
+    return _lerp(v, x1, x2)
+
+
+def fbm2(perm, x, y, octaves):
+    s = np.zeros_like(x)
+    amp, freq, norm = 1.0, 1.0, 0.0
+    for _ in range(octaves):
+        s = s + amp * noise2(perm, x * freq, y * freq)
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return s / norm
+
+
+def synthetic_inside(depth, num_spheres, radius_min, radius_max, heightfield, hf_octaves, hf_amplitude,
+                     hf_base, seed):
+    """Dense inside grid [x, y, z] of the synthetic scene definition."""
+    n = 1 << depth
+    rng = SplitMix64(seed)
+    c = np.arange(n, dtype=np.float64) + 0.5
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    inside = np.zeros((n, n, n), bool)
+    rmin = float(np.float32(radius_min))
+    rmax = float(np.float32(radius_max))
+    for _ in range(num_spheres):
+        cx, cy, cz = rng.uniform(), rng.uniform(), rng.uniform()
+        r = rmin + (rmax - rmin) * rng.uniform()
+        dx, dy, dz = X - cx * n, Y - cy * n, Z - cz * n
+        inside |= dx * dx + dy * dy + dz * dz <= r * r
+    if heightfield:
+        perm = perlin_perm(seed ^ 0xD1B54A32D192ED03)
+        u = c / float(n)
+        U, V = np.meshgrid(u, u, indexing="ij")
+        f = fbm2(perm, U * 4.0, V * 4.0, hf_octaves)
+        h = float(n) * (float(np.float32(hf_base)) + float(np.float32(hf_amplitude)) * f)
+        inside |= Z < h[:, :, None]
+    return inside
+
+
+def surface(inside):
+    pad = np.pad(inside, 1, constant_values=False)
+    interior = inside.copy()
+    for ax in range(3):
+        for sh in (-1, 1):
+            interior &= np.roll(pad, sh, axis=ax)[1:-1, 1:-1, 1:-1]
+    return inside & ~interior
+
+
+def morton3(x, y, z):
+    code = np.zeros(np.shape(x), np.uint64)
+    for b in range(21):
+        code |= ((np.asarray(x, np.uint64) >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b)
+        code |= ((np.asarray(y, np.uint64) >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + 1)
+        code |= ((np.asarray(z, np.uint64) >> np.uint64(b)) & np.uint64(1)) << np.uint64(3 * b + 2)
+    return code
+
+
+def leaf_color(code, seed):
+    return _mix64(int(code) ^ ((seed * 0x9E3779B97F4A7C15) & MASK64)) & 0xFFFFFFFF

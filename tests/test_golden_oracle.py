@@ -1,0 +1,23 @@
+"""The committed fixture (tests/golden/plain_512.json) pins both the synthetic
+scene builder (node-array digest) and the oracle's outputs on the plain_512
+workload; a change to either shows up here."""
+import golden.make_golden as mg
+
+
+def test_builder_matches_fixture(yx):
+    gold = mg.load()
+    s = yx.SVOData.build_synthetic(**yx.workloads.PLAIN_512)
+    nodes = s.nodes()
+    assert nodes.shape[0] == gold["num_nodes"]
+    assert mg.node_digest(nodes) == gold["nodes_digest"]
+
+
+def test_oracle_matches_fixture(yx, orc):
+    gold = mg.load()
+    s = yx.SVOData.build_synthetic(**yx.workloads.PLAIN_512)
+    nodes, root = s.nodes(), s.GetRoot()
+    for ent in gold["views"]:
+        o = orc.render(nodes, root, ent["pos"], ent["dir"], fov=gold["fov"], w=gold["w"], h=gold["h"],
+                       light_mode=ent["light_mode"], light=gold["light"], threads=4)
+        assert mg.digest(o) == ent["digest"], f"view {ent['view']} mode {ent['light_mode']}"
+        assert o["rays"] == ent["rays"]

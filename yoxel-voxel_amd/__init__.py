@@ -1,0 +1,13 @@
+"""MI355X-native sparse-voxel-octree ray caster (libyoxel over a C ABI).
+
+The directory name is not a Python identifier; load it with load_package()
+from __graft_entry__ or import it after registering it as `yoxel_voxel_amd`.
+"""
+from ._lib import (YX_EMPTY_NODE, YX_FULL_NODE, YX_LIGHT_HEADLIGHT, YX_LIGHT_SUN,
+                   YX_LIGHT_SUN_SHADOW, YX_MISS, LIB_PATH, YoxelError, lib)
+from .renderer import SVOData, SVORenderer
+from . import workloads
+
+__all__ = ["SVOData", "SVORenderer", "YoxelError", "lib", "workloads", "LIB_PATH",
+           "YX_EMPTY_NODE", "YX_FULL_NODE", "YX_MISS", "YX_LIGHT_HEADLIGHT", "YX_LIGHT_SUN",
+           "YX_LIGHT_SUN_SHADOW"]

@@ -1,0 +1,357 @@
+// SVO ray-cast kernels for gfx950 and the renderer side of the C ABI.
+//
+// One frame = primary rays for every pixel (ray generation of
+// RendererBase::InitRayDir, cell/renderer_base.h:50-61, and
+// cell/ppu_renderer.cpp:56), exact octree traversal (RecTrace,
+// cell/ppu_renderer.cpp:18-41), an optional shadow ray per lit hit, and the
+// Lambert shade, fused in one kernel (the reference GPU pipeline runs
+// InitEyeRays -> Trace -> ShadeSimple as three launches, trace_cuda.py:101-108).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "internal.h"
+#include "trace_device.h"
+
+namespace yx {
+
+struct FrameArgs {
+    const uint2* recs;
+    float pos[3];
+    float basis[9];
+    float light[3];  // unit direction toward the light, as given
+    int light_mode;
+    int w, h;
+    yx_outputs out;
+};
+
+// Lambert over the RGBA8 payload; DEFINED HERE (the reference shader source is
+// missing): k = 0.1 + 0.9 * max(0, n.L) * visible, n = entry-face normal.
+__device__ __forceinline__ uint32_t shade_rgba(uint32_t leaf, float k) {
+    uint32_t out = 0xff000000u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float v = (float)((leaf >> (8 * c)) & 0xffu) * k + 0.5f;
+        out |= ((uint32_t)v & 0xffu) << (8 * c);
+    }
+    return out;
+}
+
+template <int MAXD>
+__global__ __launch_bounds__(256) void frame_kernel(FrameArgs a) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= a.w || y >= a.h) return;
+    const size_t offs = (size_t)y * (size_t)a.w + (size_t)x;
+
+    Ray3 d = pixel_dir(a.basis, x, y);
+    d.x = adjust_coord(d.x);
+    d.y = adjust_coord(d.y);
+    d.z = adjust_coord(d.z);
+
+    uint32_t hit_id = YX_MISS, rgba = 0, visits = 0;
+    int vx = -1, vy = -1, vz = -1;
+    float t = -1.0f;
+    uint8_t shadow = 2;
+    HitInfo h;
+    if (a.recs && trace_ray<MAXD>(a.recs, a.pos, d, false, h, visits)) {
+        const uint2 leaf = a.recs[h.leaf_rec];
+        hit_id = leaf.y * 8u + (uint32_t)h.child;
+        vx = h.vx;
+        vy = h.vy;
+        vz = h.vz;
+        t = max3(h.cell.t1x, h.cell.t1y, h.cell.t1z);
+        const int e = entry_axis(h.cell);
+        const float de = e == 0 ? d.x : (e == 1 ? d.y : d.z);
+        float lam;
+        if (a.light_mode == YX_LIGHT_HEADLIGHT) {
+            lam = fabsf(de);
+        } else {
+            const float le = e == 0 ? a.light[0] : (e == 1 ? a.light[1] : a.light[2]);
+            lam = (de < 0) ? le : -le;
+            lam = dmax(lam, 0.0f);
+        }
+        bool lit = true;
+        if (a.light_mode == YX_LIGHT_SUN_SHADOW && lam > 0.0f) {
+            float p[3];
+            p[0] = a.pos[0] + d.x * t;
+            p[1] = a.pos[1] + d.y * t;
+            p[2] = a.pos[2] + d.z * t;
+            const float half = __builtin_bit_cast(float, (uint32_t)(127 - (h.level + 1)) << 23);
+            if (de < 0)
+                p[e] = p[e] + half;
+            else
+                p[e] = p[e] - half;
+            Ray3 l;
+            l.x = adjust_coord(a.light[0]);
+            l.y = adjust_coord(a.light[1]);
+            l.z = adjust_coord(a.light[2]);
+            HitInfo hs;
+            lit = !trace_ray<MAXD>(a.recs, p, l, true, hs, visits);
+            shadow = lit ? 0 : 1;
+        }
+        const float k = lit ? 0.1f + 0.9f * lam : 0.1f;
+        rgba = shade_rgba(leaf.x, k);
+    }
+    if (a.out.hit_id) a.out.hit_id[offs] = hit_id;
+    if (a.out.voxel) {
+        a.out.voxel[3 * offs + 0] = vx;
+        a.out.voxel[3 * offs + 1] = vy;
+        a.out.voxel[3 * offs + 2] = vz;
+    }
+    if (a.out.t) a.out.t[offs] = t;
+    if (a.out.rgba) a.out.rgba[offs] = rgba;
+    if (a.out.shadow) a.out.shadow[offs] = shadow;
+    if (a.out.visits) a.out.visits[offs] = visits;
+}
+
+}  // namespace yx
+
+using namespace yx;
+
+// ---- renderer object ------------------------------------------------------
+
+struct yx_renderer {
+    float pos[3] = {0.0f, 0.0f, 0.0f};
+    float dir[3] = {1.0f, 0.0f, 0.0f};
+    float up[3] = {0.0f, 0.0f, 1.0f};  // RendererBase default m_up(0, 0, 1)
+    float fov = 70.0f;                 // RendererBase default m_fov(70.0)
+    int w = 640, h = 480;              // RendererBase::SetResolution(640, 480)
+    float light[3] = {0.0f, 0.0f, 1.0f};
+    int light_mode = YX_LIGHT_HEADLIGHT;
+    bool has_scene = false;
+    uint2* d_recs = nullptr;
+    size_t n_recs = 0;
+    int depth = 0;
+    uint32_t* d_frame = nullptr;  // RenderFrame's device colour buffer
+    std::vector<uint32_t> h_frame;
+};
+
+#define YX_HIP(call)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess) return fail(YX_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+// RendererBase::InitRayDir (cell/renderer_base.h:50-61); integer factors are
+// converted to float first, as the cg point operators do.
+static void init_ray_dir(const yx_renderer* r, float* out) {
+    auto norm = [](float* v) {
+        float n2 = 0.0f;
+        n2 += v[0] * v[0];
+        n2 += v[1] * v[1];
+        n2 += v[2] * v[2];
+        float n = std::sqrt(n2);
+        v[0] = v[0] / n;
+        v[1] = v[1] / n;
+        v[2] = v[2] / n;
+    };
+    auto cross = [](const float* a, const float* b, float* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    float fwd[3] = {r->dir[0], r->dir[1], r->dir[2]};
+    norm(fwd);
+    float right[3], upv[3];
+    cross(fwd, r->up, right);
+    norm(right);
+    cross(right, fwd, upv);
+    const float deg2rad = (float)(3.14159265358979323846 / (double)180.0f);
+    const float da = tanf((r->fov / 2.0f) * deg2rad) / (float)r->w;
+    const float wf = (float)r->w, hf = (float)r->h;
+    for (int i = 0; i < 3; ++i) {
+        const float du = (right[i] * 2.0f) * da;
+        const float dv = (upv[i] * -2.0f) * da;
+        out[3 + i] = du;
+        out[6 + i] = dv;
+        out[i] = (fwd[i] - (du * wf) / 2.0f) - (dv * hf) / 2.0f;
+    }
+}
+
+static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t stream) {
+    FrameArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.recs = r->d_recs;
+    for (int i = 0; i < 3; ++i) {
+        a.pos[i] = r->pos[i];
+        a.light[i] = r->light[i];
+    }
+    init_ray_dir(r, a.basis);
+    a.light_mode = r->light_mode;
+    a.w = r->w;
+    a.h = r->h;
+    a.out = out;
+    dim3 grid((r->w + 15) / 16, (r->h + 15) / 16);
+    if (r->depth <= 8)
+        hipLaunchKernelGGL(frame_kernel<8>, grid, dim3(256), 0, stream, a);
+    else if (r->depth <= 12)
+        hipLaunchKernelGGL(frame_kernel<12>, grid, dim3(256), 0, stream, a);
+    else
+        hipLaunchKernelGGL(frame_kernel<kMaxDepth>, grid, dim3(256), 0, stream, a);
+    YX_HIP(hipGetLastError());
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_create(yx_renderer** out) {
+    clear_error();
+    if (!out) return fail(YX_ERR_INVALID_ARG, "null argument");
+    yx_renderer* r = new (std::nothrow) yx_renderer();
+    if (!r) return fail(YX_ERR_OOM, "out of host memory");
+    *out = r;
+    return YX_OK;
+}
+
+extern "C" void yx_renderer_destroy(yx_renderer* r) {
+    if (!r) return;
+    if (r->d_recs) (void)hipFree(r->d_recs);
+    if (r->d_frame) (void)hipFree(r->d_frame);
+    delete r;
+}
+
+extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
+    clear_error();
+    if (!r || !s) return fail(YX_ERR_INVALID_ARG, "null argument");
+    GpuTree t;
+    int rc;
+    try {
+        rc = build_gpu_tree(*s, t);
+    } catch (const std::bad_alloc&) {
+        return fail(YX_ERR_OOM, "out of host memory in the re-layout");
+    }
+    if (rc != YX_OK) return rc;
+    if (r->d_recs) {
+        YX_HIP(hipFree(r->d_recs));
+        r->d_recs = nullptr;
+    }
+    r->n_recs = 0;
+    r->depth = 0;
+    if (!t.empty) {
+        const size_t bytes = t.recs.size() * sizeof(uint32_t);
+        hipError_t e = hipMalloc((void**)&r->d_recs, bytes);
+        if (e != hipSuccess) {
+            r->d_recs = nullptr;
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) for the scene: %s", bytes, hipGetErrorString(e));
+        }
+        YX_HIP(hipMemcpy(r->d_recs, t.recs.data(), bytes, hipMemcpyHostToDevice));
+        r->n_recs = t.recs.size() / 2;
+        r->depth = t.depth;
+    }
+    r->has_scene = true;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_view_pos(yx_renderer* r, const float pos[3]) {
+    clear_error();
+    if (!r || !pos) return fail(YX_ERR_INVALID_ARG, "null argument");
+    for (int i = 0; i < 3; ++i) r->pos[i] = pos[i];
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_view_dir(yx_renderer* r, const float dir[3]) {
+    clear_error();
+    if (!r || !dir) return fail(YX_ERR_INVALID_ARG, "null argument");
+    for (int i = 0; i < 3; ++i) r->dir[i] = dir[i];
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_view_up(yx_renderer* r, const float up[3]) {
+    clear_error();
+    if (!r || !up) return fail(YX_ERR_INVALID_ARG, "null argument");
+    for (int i = 0; i < 3; ++i) r->up[i] = up[i];
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_resolution(yx_renderer* r, int32_t w, int32_t h) {
+    clear_error();
+    if (!r) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (w <= 0 || h <= 0) return fail(YX_ERR_INVALID_ARG, "resolution %dx%d must be positive", w, h);
+    if ((int64_t)w * (int64_t)h > (int64_t)1 << 28) return fail(YX_ERR_RANGE, "resolution %dx%d too large", w, h);
+    if (w != r->w || h != r->h) {
+        if (r->d_frame) {
+            YX_HIP(hipFree(r->d_frame));
+            r->d_frame = nullptr;
+        }
+        r->h_frame.clear();
+    }
+    r->w = w;
+    r->h = h;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_get_resolution(const yx_renderer* r, int32_t* w, int32_t* h) {
+    clear_error();
+    if (!r || !w || !h) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *w = r->w;
+    *h = r->h;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_fov(yx_renderer* r, float fov) {
+    clear_error();
+    if (!r) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (!(fov > 0.0f && fov < 180.0f)) return fail(YX_ERR_INVALID_ARG, "fov %g outside (0, 180)", fov);
+    r->fov = fov;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_light(yx_renderer* r, const float dir[3], int32_t mode) {
+    clear_error();
+    if (!r || !dir) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (mode < YX_LIGHT_HEADLIGHT || mode > YX_LIGHT_SUN_SHADOW)
+        return fail(YX_ERR_INVALID_ARG, "unknown light mode %d", mode);
+    for (int i = 0; i < 3; ++i) r->light[i] = dir[i];
+    r->light_mode = mode;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_ray_basis(const yx_renderer* r, float basis[9]) {
+    clear_error();
+    if (!r || !basis) return fail(YX_ERR_INVALID_ARG, "null argument");
+    init_ray_dir(r, basis);
+    return YX_OK;
+}
+
+extern "C" int yx_render(yx_renderer* r, const yx_outputs* out, void* stream) {
+    clear_error();
+    if (!r || !out) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (!r->has_scene) return fail(YX_ERR_NO_SCENE, "no scene set");
+    return launch_frame(r, *out, (hipStream_t)stream);
+}
+
+extern "C" int yx_render_frame(yx_renderer* r, const uint32_t** colors) {
+    clear_error();
+    if (!r || !colors) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *colors = nullptr;
+    if (!r->has_scene) return fail(YX_ERR_NO_SCENE, "no scene set");
+    const size_t n = (size_t)r->w * (size_t)r->h;
+    if (!r->d_frame) {
+        hipError_t e = hipMalloc((void**)&r->d_frame, n * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            r->d_frame = nullptr;
+            return fail(YX_ERR_OOM, "hipMalloc frame: %s", hipGetErrorString(e));
+        }
+    }
+    try {
+        r->h_frame.resize(n);
+    } catch (const std::bad_alloc&) {
+        return fail(YX_ERR_OOM, "out of host memory");
+    }
+    yx_outputs o;
+    std::memset(&o, 0, sizeof(o));
+    o.rgba = r->d_frame;
+    int rc = launch_frame(r, o, nullptr);
+    if (rc != YX_OK) return rc;
+    YX_HIP(hipMemcpy(r->h_frame.data(), r->d_frame, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    *colors = r->h_frame.data();
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_scene_bytes(const yx_renderer* r, uint64_t* bytes) {
+    clear_error();
+    if (!r || !bytes) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *bytes = (uint64_t)r->n_recs * 8u;
+    return YX_OK;
+}

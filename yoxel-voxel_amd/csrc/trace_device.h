@@ -1,0 +1,193 @@
+// Device-side SVO traversal for gfx950 (HIP). Included by the trace kernels.
+//
+// The traversal is the iterative form of the reference's recursive RecTrace
+// (cell/ppu_renderer.cpp:18-41) with FindFirstChild / GoNext as in the SPE
+// twins (cell/spu/trace_spu.cpp:48-93), SetupTrace as in
+// cell/spu/trace_spu.c_:79-99 and AdjustDir as ntree/ntree_trace.cu:38-50.
+// Every float operation is the same, in the same order, as the CPU restatement
+// the tests compare against, so hit ids and voxel coordinates are bit-exact;
+// the library is built with -ffp-contract=off and correctly rounded fp32
+// division and square root.
+//
+// Node records (8 bytes, uint2) are the unified breadth-first layout:
+//   node: x = first child record, y = child_mask | leaf_mask << 8
+//   leaf: x = RGBA8 payload,      y = canonical parent id
+// A present child c lives at x + popcount((child_mask|leaf_mask) & ((1<<c)-1)).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace yx {
+
+struct Ray3 {
+    float x, y, z;
+};
+
+__device__ __forceinline__ float dmax(float a, float b) { return (a < b) ? b : a; }
+__device__ __forceinline__ float dmin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float max3(float x, float y, float z) { return dmax(x, dmax(y, z)); }
+__device__ __forceinline__ float min3(float x, float y, float z) { return dmin(x, dmin(y, z)); }
+
+__device__ __forceinline__ float adjust_coord(float x) {
+    const float eps = 1e-6f;
+    if (fabsf(x) < eps) x = (x < 0) ? -eps : eps;
+    return x;
+}
+
+// normalized(dir0 + du*x + dv*y), cell/ppu_renderer.cpp:56
+__device__ __forceinline__ Ray3 pixel_dir(const float* b, int x, int y) {
+    float xf = (float)x, yf = (float)y;
+    float dx = b[0] + b[3] * xf, dy = b[1] + b[4] * xf, dz = b[2] + b[5] * xf;
+    dx = dx + b[6] * yf;
+    dy = dy + b[7] * yf;
+    dz = dz + b[8] * yf;
+    float n2 = 0.0f;
+    n2 += dx * dx;
+    n2 += dy * dy;
+    n2 += dz * dz;
+    float n = __builtin_sqrtf(n2);
+    Ray3 r;
+    r.x = dx / n;
+    r.y = dy / n;
+    r.z = dz / n;
+    return r;
+}
+
+// Exact per-ray traversal state of one level.
+struct Cell {
+    float t1x, t1y, t1z, t2x, t2y, t2z;
+};
+
+__device__ __forceinline__ bool setup_trace(const float p[3], const Ray3& d, Cell& c, uint32_t& flags) {
+    flags = 0;
+    if (d.x > 0) { c.t1x = (-p[0]) / d.x; c.t2x = (1.0f - p[0]) / d.x; }
+    else         { c.t2x = (-p[0]) / d.x; c.t1x = (1.0f - p[0]) / d.x; }
+    if (d.y > 0) { c.t1y = (-p[1]) / d.y; c.t2y = (1.0f - p[1]) / d.y; }
+    else         { c.t2y = (-p[1]) / d.y; c.t1y = (1.0f - p[1]) / d.y; }
+    if (d.z > 0) { c.t1z = (-p[2]) / d.z; c.t2z = (1.0f - p[2]) / d.z; }
+    else         { c.t2z = (-p[2]) / d.z; c.t1z = (1.0f - p[2]) / d.z; }
+    if (d.x < 0) flags |= 1u;
+    if (d.y < 0) flags |= 2u;
+    if (d.z < 0) flags |= 4u;
+    return max3(c.t1x, c.t1y, c.t1z) < min3(c.t2x, c.t2y, c.t2z);
+}
+
+// FindFirstChild, cell/spu/trace_spu.cpp:48-68
+__device__ __forceinline__ int find_first_child(Cell& c) {
+    float mx = 0.5f * (c.t1x + c.t2x);
+    float my = 0.5f * (c.t1y + c.t2y);
+    float mz = 0.5f * (c.t1z + c.t2z);
+    float te = max3(c.t1x, c.t1y, c.t1z);
+    bool bx = te > mx, by = te > my, bz = te > mz;
+    c.t1x = bx ? mx : c.t1x;  c.t2x = bx ? c.t2x : mx;
+    c.t1y = by ? my : c.t1y;  c.t2y = by ? c.t2y : my;
+    c.t1z = bz ? mz : c.t1z;  c.t2z = bz ? c.t2z : mz;
+    return (bx ? 1 : 0) | (by ? 2 : 0) | (bz ? 4 : 0);
+}
+
+// GoNext, cell/spu/trace_spu.cpp:70-93 (argmin tie order of lines 75-78)
+__device__ __forceinline__ bool go_next(int& ch, Cell& c) {
+    int e;
+    if (c.t2x > c.t2y)
+        e = (c.t2y < c.t2z) ? 1 : 2;
+    else
+        e = (c.t2x < c.t2z) ? 0 : 2;
+    const int mask = 1 << e;
+    if (ch & mask) return false;
+    ch ^= mask;
+    if (e == 0) { float dt = c.t2x - c.t1x; c.t1x = c.t2x; c.t2x += dt; }
+    else if (e == 1) { float dt = c.t2y - c.t1y; c.t1y = c.t2y; c.t2y += dt; }
+    else { float dt = c.t2z - c.t1z; c.t1z = c.t2z; c.t2z += dt; }
+    return true;
+}
+
+// Entry-face axis of the hit cell: argmax(t1) with the tie order of
+// bittree/trace.cu:100-106.
+__device__ __forceinline__ int entry_axis(const Cell& c) {
+    if (c.t1x < c.t1y) return (c.t1y > c.t1z) ? 1 : 2;
+    return (c.t1x > c.t1z) ? 0 : 2;
+}
+
+struct HitInfo {
+    uint32_t leaf_rec;  // record index of the hit leaf
+    int child;          // real child slot
+    int level;          // leaf level (root children = 1)
+    int vx, vy, vz;
+    Cell cell;          // hit cell (t1 = entry times)
+};
+
+// Per-lane traversal stack. Level l holds the parent state pushed when
+// descending from level l to l+1.
+template <int MAXD>
+struct PrivStack {
+    float t[MAXD][6];
+    uint2 node[MAXD];
+    int ch[MAXD];
+    __device__ __forceinline__ void push(int l, const Cell& c, uint2 n, int h) {
+        t[l][0] = c.t1x; t[l][1] = c.t1y; t[l][2] = c.t1z;
+        t[l][3] = c.t2x; t[l][4] = c.t2y; t[l][5] = c.t2z;
+        node[l] = n;
+        ch[l] = h;
+    }
+    __device__ __forceinline__ void pop(int l, Cell& c, uint2& n, int& h) const {
+        c.t1x = t[l][0]; c.t1y = t[l][1]; c.t1z = t[l][2];
+        c.t2x = t[l][3]; c.t2y = t[l][4]; c.t2z = t[l][5];
+        n = node[l];
+        h = ch[l];
+    }
+};
+
+// One ray from p along d (already AdjustDir'ed). occlusion: leaf cells behind
+// the origin are skipped (shadow rays). Returns true on a hit.
+template <int MAXD>
+__device__ bool trace_ray(const uint2* __restrict__ recs, const float p[3], const Ray3& d,
+                          bool occlusion, HitInfo& hit, uint32_t& visits) {
+    Cell c;
+    uint32_t flags;
+    if (!setup_trace(p, d, c, flags)) return false;
+    if (min3(c.t2x, c.t2y, c.t2z) <= 0) return false;
+    PrivStack<MAXD> stk;
+    ++visits;
+    uint2 node = recs[0];
+    int level = 0;
+    int px = 0, py = 0, pz = 0;
+    int ch = find_first_child(c);
+    for (;;) {
+        const int real = ch ^ (int)flags;
+        const uint32_t cm = node.y & 0xffu, lm = (node.y >> 8) & 0xffu;
+        const uint32_t below = (cm | lm) & ((1u << real) - 1u);
+        if ((lm >> real) & 1u) {
+            if (!occlusion || min3(c.t2x, c.t2y, c.t2z) > 0) {
+                hit.leaf_rec = node.x + (uint32_t)__builtin_popcount(below);
+                hit.child = real;
+                hit.level = level + 1;
+                hit.vx = 2 * px + (real & 1);
+                hit.vy = 2 * py + ((real >> 1) & 1);
+                hit.vz = 2 * pz + ((real >> 2) & 1);
+                hit.cell = c;
+                return true;
+            }
+        } else if (((cm >> real) & 1u) && min3(c.t2x, c.t2y, c.t2z) > 0) {
+            ++visits;
+            stk.push(level, c, node, ch);
+            node = recs[node.x + (uint32_t)__builtin_popcount(below)];
+            px = 2 * px + (real & 1);
+            py = 2 * py + ((real >> 1) & 1);
+            pz = 2 * pz + ((real >> 2) & 1);
+            ++level;
+            ch = find_first_child(c);
+            continue;
+        }
+        while (!go_next(ch, c)) {
+            if (level == 0) return false;
+            --level;
+            stk.pop(level, c, node, ch);
+            px >>= 1;
+            py >>= 1;
+            pz >>= 1;
+        }
+    }
+}
+
+}  // namespace yx

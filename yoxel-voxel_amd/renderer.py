@@ -1,0 +1,199 @@
+"""Host-side mirror of the reference renderer API over the C ABI.
+
+SVOData      - cell/svodata.h:22-55 (Load, GetRoot, operator[])
+SVORenderer  - ISVORenderer (cell/svorenderer.h:5-31): SetScene, SetViewPos,
+               SetViewDir, SetViewUp, SetResolution, GetResolution, SetFOV,
+               RenderFrame; plus the GPU renderer's Render(dst)
+               (demo/SVORenderer.h:8-39) and the light setter.
+Error behaviour follows the reference: RenderFrame returns None when no scene
+is set (cell/ppu_renderer.cpp:78-79); other failures raise YoxelError.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import YoxelError, check, f3, lib
+
+
+class SVOData:
+    """Octree in the reference VoxNode layout (10 u32 per node)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().yx_scene_free(self._h)
+            self._h = ctypes.c_void_p()
+
+    def _replace(self, h):
+        if self._h.value:
+            lib().yx_scene_free(self._h)
+        self._h = h
+
+    # -- constructors ------------------------------------------------------
+    def Load(self, fn):
+        h = ctypes.c_void_p()
+        check(lib().yx_scene_load_vox(str(fn).encode(), ctypes.byref(h)))
+        self._replace(h)
+        return self
+
+    def Save(self, fn):
+        check(lib().yx_scene_save_vox(self._h, str(fn).encode()))
+
+    @classmethod
+    def from_nodes(cls, nodes, root):
+        a = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 10)
+        s = cls()
+        h = ctypes.c_void_p()
+        check(lib().yx_scene_from_nodes(a.ctypes.data_as(ctypes.c_void_p), a.shape[0], int(root),
+                                        ctypes.byref(h)))
+        s._h = h
+        return s
+
+    @classmethod
+    def build_synthetic(cls, depth, num_spheres=0, radius_min=0.0, radius_max=0.0, heightfield=False,
+                        hf_octaves=5, hf_amplitude=0.2, hf_base=0.3, seed=0):
+        d = _lib.SceneDesc(depth, num_spheres, radius_min, radius_max, 1 if heightfield else 0,
+                           hf_octaves, hf_amplitude, hf_base, seed)
+        s = cls()
+        h = ctypes.c_void_p()
+        check(lib().yx_scene_build_synthetic(ctypes.byref(d), ctypes.byref(h)))
+        s._h = h
+        return s
+
+    # -- accessors ---------------------------------------------------------
+    def info(self):
+        i = _lib.SceneInfo()
+        check(lib().yx_scene_get_info(self._h, ctypes.byref(i)))
+        return {"root": i.root, "num_nodes": i.num_nodes, "num_leaves": i.num_leaves,
+                "depth": i.depth}
+
+    def GetRoot(self):
+        return self.info()["root"]
+
+    def nodes(self):
+        """Copy of the node array, shape (num_nodes, 10) uint32."""
+        p = ctypes.c_void_p()
+        n = ctypes.c_uint32()
+        r = ctypes.c_uint32()
+        check(lib().yx_scene_nodes(self._h, ctypes.byref(p), ctypes.byref(n), ctypes.byref(r)))
+        if n.value == 0:
+            return np.zeros((0, 10), np.uint32)
+        buf = (ctypes.c_uint32 * (n.value * 10)).from_address(p.value)
+        return np.frombuffer(buf, dtype=np.uint32).reshape(n.value, 10).copy()
+
+    def __getitem__(self, node_id):
+        return self.nodes()[node_id]
+
+    @property
+    def handle(self):
+        return self._h
+
+
+class SVORenderer:
+    """ISVORenderer / SVORenderer over libyoxel."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        check(lib().yx_renderer_create(ctypes.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().yx_renderer_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def SetScene(self, svo):
+        check(lib().yx_renderer_set_scene(self._h, svo.handle))
+
+    def SetViewPos(self, pos):
+        check(lib().yx_renderer_set_view_pos(self._h, f3(pos)))
+
+    def SetViewDir(self, d):
+        check(lib().yx_renderer_set_view_dir(self._h, f3(d)))
+
+    def SetViewUp(self, up):
+        check(lib().yx_renderer_set_view_up(self._h, f3(up)))
+
+    def SetResolution(self, width, height):
+        check(lib().yx_renderer_set_resolution(self._h, int(width), int(height)))
+
+    SetViewSize = SetResolution  # demo/SVORenderer.h:20
+
+    def GetResolution(self):
+        w = ctypes.c_int32()
+        h = ctypes.c_int32()
+        check(lib().yx_renderer_get_resolution(self._h, ctypes.byref(w), ctypes.byref(h)))
+        return (w.value, h.value)
+
+    def SetFOV(self, fov):
+        check(lib().yx_renderer_set_fov(self._h, float(fov)))
+
+    def SetLight(self, direction, mode=_lib.YX_LIGHT_SUN_SHADOW):
+        check(lib().yx_renderer_set_light(self._h, f3(direction), int(mode)))
+
+    def ray_basis(self):
+        b = (ctypes.c_float * 9)()
+        check(lib().yx_renderer_ray_basis(self._h, b))
+        return np.array(b[:], dtype=np.float32)
+
+    def RenderFrame(self):
+        """Shaded RGBA8 frame as (h, w) uint32, or None without a scene."""
+        p = ctypes.c_void_p()
+        rc = lib().yx_render_frame(self._h, ctypes.byref(p))
+        if rc == _lib.YX_ERR_NO_SCENE:
+            return None
+        check(rc)
+        w, h = self.GetResolution()
+        buf = (ctypes.c_uint32 * (w * h)).from_address(p.value)
+        return np.frombuffer(buf, dtype=np.uint32).reshape(h, w).copy()
+
+    def Render(self, hit_id=None, voxel=None, t=None, rgba=None, shadow=None, visits=None, stream=None):
+        """Render into caller-owned device buffers (torch CUDA tensors or raw
+        device addresses), asynchronously on `stream` (a torch stream, a raw
+        hipStream_t address, or None for the current torch stream)."""
+        o = _lib.Outputs(_addr(hit_id), _addr(voxel), _addr(t), _addr(rgba), _addr(shadow),
+                         _addr(visits))
+        check(lib().yx_render(self._h, ctypes.byref(o), ctypes.c_void_p(_stream(stream))))
+
+    def render_tensors(self, outputs=("hit_id", "voxel", "t", "rgba", "shadow"), stream=None):
+        """Allocate torch output tensors on the current device, render, return them."""
+        import torch
+        w, h = self.GetResolution()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        spec = {"hit_id": ((h, w), torch.int32), "voxel": ((h, w, 3), torch.int32),
+                "t": ((h, w), torch.float32), "rgba": ((h, w), torch.int32),
+                "shadow": ((h, w), torch.uint8), "visits": ((h, w), torch.int32)}
+        out = {k: torch.empty(spec[k][0], dtype=spec[k][1], device=dev) for k in outputs}
+        self.Render(stream=stream, **out)
+        return out
+
+    def scene_bytes(self):
+        b = ctypes.c_uint64()
+        check(lib().yx_renderer_scene_bytes(self._h, ctypes.byref(b)))
+        return b.value
+
+    @property
+    def handle(self):
+        return self._h
+
+
+def _addr(x):
+    if x is None:
+        return None
+    if isinstance(x, int):
+        return x
+    return x.data_ptr()
+
+
+def _stream(s):
+    if s is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(s, int):
+        return s
+    return s.cuda_stream
+
+
+__all__ = ["SVOData", "SVORenderer", "YoxelError"]

@@ -1,0 +1,43 @@
+"""Synthetic workloads: scene recipes and orbit cameras.
+
+plain_512: depth-8 octree (256^3) from 64 spheres, radii 8-32 voxels, 512x512
+           primary rays, 60 deg FOV, 8 seeded orbit angles at radius 1.5.
+large_1080p: depth-11 octree (2048^3) from 512 spheres, radii 16-128 voxels,
+           united with a 5-octave fBm-Perlin heightfield of amplitude 0.2;
+           1920x1080 primary rays plus one shadow ray per lit hit toward a fixed
+           directional light; 32 orbit frames.
+"""
+import math
+
+import numpy as np
+
+CENTER = (0.5, 0.5, 0.5)
+ORBIT_RADIUS = 1.5
+ELEVATION_DEG = 30.0
+UP = (0.0, 0.0, 1.0)
+
+PLAIN_512 = dict(depth=8, num_spheres=64, radius_min=8.0, radius_max=32.0, heightfield=False,
+                 hf_octaves=5, hf_amplitude=0.2, hf_base=0.3, seed=1)
+LARGE_1080P = dict(depth=11, num_spheres=512, radius_min=16.0, radius_max=128.0, heightfield=True,
+                   hf_octaves=5, hf_amplitude=0.2, hf_base=0.3, seed=11)
+
+# fixed directional light of the large scene: unit vector toward the light
+SUN = tuple(np.float32(v) for v in (np.array([0.4, 0.3, 0.85]) / np.linalg.norm([0.4, 0.3, 0.85])))
+
+
+def orbit_cameras(n, seed=0, radius=ORBIT_RADIUS, elevation_deg=ELEVATION_DEG):
+    """n cameras on a circle around the cube centre looking at it: azimuth
+    2*pi*k/n plus a seeded offset in [0, 2*pi/n). Returns [(pos, dir)] as
+    float32 triples."""
+    rng = np.random.default_rng(seed)
+    off = rng.uniform(0.0, 2.0 * math.pi / n)
+    el = math.radians(elevation_deg)
+    cams = []
+    for k in range(n):
+        az = 2.0 * math.pi * k / n + off
+        p = np.array([CENTER[0] + radius * math.cos(el) * math.cos(az),
+                      CENTER[1] + radius * math.cos(el) * math.sin(az),
+                      CENTER[2] + radius * math.sin(el)])
+        d = np.array(CENTER) - p
+        cams.append((tuple(np.float32(p)), tuple(np.float32(d))))
+    return cams
