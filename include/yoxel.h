@@ -144,6 +144,25 @@ int yx_render_frame(yx_renderer* r, const uint32_t** colors);
 /* Device memory used by the uploaded scene, in bytes. */
 int yx_renderer_scene_bytes(const yx_renderer* r, uint64_t* bytes);
 
+/* Launch shape of the ray-cast kernels. No reference counterpart (the
+ * reference fixes 16x16 / 16x32 blocks, trace_cuda.py:26); set by the
+ * autotuner. Every setting renders identical outputs. */
+#define YX_KERNEL_PER_PIXEL  0 /* one thread per pixel, 16x16 tiles           */
+#define YX_KERNEL_PERSISTENT 1 /* persistent waves on atomic ray queues       */
+typedef struct yx_tuning {
+    int32_t kernel;       /* YX_KERNEL_*                                        */
+    int32_t block;        /* threads per workgroup of the persistent kernel:
+                             256 or 512                                         */
+    int32_t tile_w;       /* ray packet (queue tile) width in pixels, power of 2 */
+    int32_t tile_h;       /* ray packet height, power of 2; tile_w*tile_h <= 4096 */
+    int32_t refill_min;   /* idle lanes before a wave takes new rays, 1..64     */
+    int32_t queues;       /* ray queues, 1..64 (8 = one per XCD)                */
+    int32_t window;       /* ancestor cells kept in registers: 0, 1 or 2        */
+    int32_t waves_per_cu; /* persistent launch size; 0 = occupancy limit        */
+} yx_tuning;
+int yx_renderer_set_tuning(yx_renderer* r, const yx_tuning* tuning);
+int yx_renderer_get_tuning(const yx_renderer* r, yx_tuning* tuning);
+
 const char* yx_last_error(void);
 const char* yx_version(void);
 

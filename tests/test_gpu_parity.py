@@ -13,8 +13,21 @@ pytestmark = pytest.mark.gpu
 KEYS = ("hit_id", "voxel", "t", "rgba", "shadow", "visits")
 
 
-def gpu_render(yx, torch, scene, pos, d, w, h, fov=60.0, light_mode=0, light=(0, 0, 1), keys=KEYS):
-    r = yx.SVORenderer()
+TUNINGS = [
+    dict(kernel=0),
+    dict(kernel=1),  # library default
+    dict(kernel=1, window=0, block=512, tile_w=16, tile_h=4, refill_min=1, queues=1),
+    dict(kernel=1, window=1, block=256, tile_w=64, tile_h=1, refill_min=64, queues=13),
+    dict(kernel=1, window=2, block=512, tile_w=32, tile_h=32, refill_min=32, queues=64, waves_per_cu=4),
+    dict(kernel=1, window=0, block=256, tile_w=1, tile_h=1, refill_min=8, queues=8, waves_per_cu=1),
+]
+
+
+def gpu_render(yx, torch, scene, pos, d, w, h, fov=60.0, light_mode=0, light=(0, 0, 1), keys=KEYS,
+               tuning=None, renderer=None):
+    r = renderer or yx.SVORenderer()
+    if tuning:
+        r.set_tuning(**tuning)
     r.SetScene(scene)
     r.SetViewPos(pos)
     r.SetViewDir(d)
@@ -48,6 +61,27 @@ def test_plain_512_eight_views(yx, orc, cuda):
         o = orc.render(nodes, root, pos, d, fov=60.0, w=512, h=512, threads=8)
         assert_same(g, o, what=f"view {i}")
         assert (g["hit_id"] != H.MISS).sum() > 10000
+
+
+def test_tunings_render_identically(yx, orc, cuda):
+    """Every launch shape of both kernels gives the oracle's outputs, on the
+    plain_512 scene (light at the eye and sun + shadow) and on one full
+    large_1080p frame; the queues reset themselves between launches."""
+    small = yx.SVOData.build_synthetic(**yx.workloads.PLAIN_512)
+    large = yx.SVOData.build_synthetic(**yx.workloads.LARGE_1080P)
+    cams = yx.workloads.orbit_cameras(8, seed=0)
+    cases = [(small, cams[1], 512, 512, 0), (small, cams[6], 512, 512, 2),
+             (large, yx.workloads.orbit_cameras(32, seed=0)[13], 1920, 1080, 2)]
+    for s, (pos, d), w, h, mode in cases:
+        o = orc.render(s.nodes(), s.GetRoot(), pos, d, fov=60.0, w=w, h=h, light_mode=mode,
+                       light=yx.workloads.SUN, threads=16)
+        r = yx.SVORenderer()
+        for tu in TUNINGS:
+            print(f"tuning {tu} on {w}x{h} mode {mode}", flush=True)
+            for rep in range(2):
+                g = gpu_render(yx, cuda, s, pos, d, w, h, light_mode=mode, light=yx.workloads.SUN, tuning=tu,
+                               renderer=r)
+                assert_same(g, o, what=f"tuning {tu} rep {rep}")
 
 
 def test_plain_512_sun_shadow(yx, orc, cuda):
@@ -93,9 +127,11 @@ def test_edge_cases(yx, orc, cuda):
     ]
     for pos, d, w, h in cases:
         for mode in (0, 2):
-            g = gpu_render(yx, cuda, s, pos, d, w, h, fov=70.0, light_mode=mode, light=yx.workloads.SUN)
             o = orc.render(nodes, root, pos, d, fov=70.0, w=w, h=h, light_mode=mode, light=yx.workloads.SUN)
-            assert_same(g, o, what=f"edge {pos} {w}x{h} mode {mode}")
+            for tu in (TUNINGS[0], TUNINGS[1], TUNINGS[4]):
+                g = gpu_render(yx, cuda, s, pos, d, w, h, fov=70.0, light_mode=mode, light=yx.workloads.SUN,
+                               tuning=tu)
+                assert_same(g, o, what=f"edge {pos} {w}x{h} mode {mode} tuning {tu}")
 
 
 def test_deep_and_tiny_trees(yx, orc, cuda):
@@ -107,10 +143,36 @@ def test_deep_and_tiny_trees(yx, orc, cuda):
         c = (np.array([x, y, z]) + 0.5) / 2.0 ** lv
         pos = (float(c[0]) - 1.0, float(c[1]) - 0.3, float(c[2]) + 0.2)
         d = (1.0, 0.3, -0.2)
-        g = gpu_render(yx, cuda, s, pos, d, 64, 64, fov=0.01 if lv == 16 else 40.0)
         o = orc.render(nodes, root, pos, d, fov=0.01 if lv == 16 else 40.0, w=64, h=64)
-        assert_same(g, o, what=f"tree level {lv}")
-        assert (g["hit_id"] != H.MISS).any()
+        for tu in (TUNINGS[0], TUNINGS[1], TUNINGS[2]):
+            g = gpu_render(yx, cuda, s, pos, d, 64, 64, fov=0.01 if lv == 16 else 40.0, tuning=tu)
+            assert_same(g, o, what=f"tree level {lv} tuning {tu}")
+            assert (g["hit_id"] != H.MISS).any()
+
+
+def test_empty_node_chain_below_leaves(yx, orc, cuda):
+    """A node chain with no leaves reaching below the deepest leaf level (legal
+    in a .vox file) must not overrun the per-level stack rows."""
+    leaves = [(2, 3, 3, 3, 5)]  # under root slot 7; slot 0 stays empty
+    nodes, root = H.tree_from_leaves(leaves)
+    # hang an 8-deep chain of empty nodes off slot 0 of the root
+    extra = np.zeros((8, 10), np.uint32)
+    extra[:, 2:] = H.EMPTY
+    extra[:, 0] = 0xFF00
+    base = nodes.shape[0]
+    for k in range(7):
+        extra[k, 2] = base + k + 1
+        extra[k, 0] = 0xFE00
+    nodes = np.concatenate([nodes, extra])
+    assert nodes[root, 2] == H.EMPTY
+    nodes[root, 2] = base
+    nodes[root, 0] &= ~np.uint32(1 << 8)
+    s = yx.SVOData.from_nodes(nodes, root)
+    pos, d = (-0.5, -0.4, -0.3), (1.0, 0.9, 0.8)
+    o = orc.render(nodes, root, pos, d, fov=50.0, w=48, h=48)
+    for tu in TUNINGS:
+        g = gpu_render(yx, cuda, s, pos, d, 48, 48, fov=50.0, tuning=tu)
+        assert_same(g, o, what=f"empty chain tuning {tu}")
 
 
 def test_empty_scene_all_miss(yx, cuda):

@@ -77,6 +77,25 @@ class Outputs(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("kernel", ctypes.c_int32),
+        ("block", ctypes.c_int32),
+        ("tile_w", ctypes.c_int32),
+        ("tile_h", ctypes.c_int32),
+        ("refill_min", ctypes.c_int32),
+        ("queues", ctypes.c_int32),
+        ("window", ctypes.c_int32),
+        ("waves_per_cu", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+YX_KERNEL_PER_PIXEL = 0
+YX_KERNEL_PERSISTENT = 1
+
 _P = ctypes.c_void_p
 _F3 = ctypes.POINTER(ctypes.c_float)
 
@@ -105,6 +124,8 @@ SIGNATURES = {
     "yx_render": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), _P]),
     "yx_render_frame": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "yx_renderer_scene_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "yx_renderer_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "yx_renderer_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "yx_last_error": (ctypes.c_char_p, []),
     "yx_version": (ctypes.c_char_p, []),
 }

@@ -64,6 +64,7 @@ struct Perlin {
 struct GpuTree {
     std::vector<uint32_t> recs;  // 2 u32 per record
     int depth = 0;               // deepest leaf level
+    int node_levels = 0;         // deepest node level + 1 (root only: 1)
     bool empty = true;
 };
 int build_gpu_tree(const yx_scene& s, GpuTree& out);

@@ -499,13 +499,14 @@ int build_gpu_tree(const yx_scene& s, GpuTree& out) {
     q_rec.push_back(0);
     q_lvl.push_back(0);
     seen[s.root] = 1;
-    int depth = 0;
+    int depth = 0, max_node_level = 0;
     for (size_t h = 0; h < q_id.size(); ++h) {
         const uint32_t id = q_id[h];
         const uint32_t* n = &s.nodes[(size_t)id * 10];
         const uint32_t leaf_mask = n[0] & 0xffu;
         uint32_t child_mask = 0;
         const int lvl = q_lvl[h];
+        max_node_level = std::max(max_node_level, lvl);
         for (int c = 0; c < 8; ++c) {
             if (leaf_mask & (1u << c)) continue;
             uint32_t ch = n[2 + c];
@@ -543,6 +544,7 @@ int build_gpu_tree(const yx_scene& s, GpuTree& out) {
         }
     }
     out.depth = depth;
+    out.node_levels = max_node_level + 1;
     out.empty = (depth == 0);
     return YX_OK;
 }

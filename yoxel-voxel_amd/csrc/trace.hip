@@ -8,12 +8,14 @@
 // InitEyeRays -> Trace -> ShadeSimple as three launches, trace_cuda.py:101-108).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
 
 #include "internal.h"
 #include "trace_device.h"
+#include "trace_persist.h"
 
 namespace yx {
 
@@ -26,18 +28,6 @@ struct FrameArgs {
     int w, h;
     yx_outputs out;
 };
-
-// Lambert over the RGBA8 payload; DEFINED HERE (the reference shader source is
-// missing): k = 0.1 + 0.9 * max(0, n.L) * visible, n = entry-face normal.
-__device__ __forceinline__ uint32_t shade_rgba(uint32_t leaf, float k) {
-    uint32_t out = 0xff000000u;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        float v = (float)((leaf >> (8 * c)) & 0xffu) * k + 0.5f;
-        out |= ((uint32_t)v & 0xffu) << (8 * c);
-    }
-    return out;
-}
 
 template <int MAXD>
 __global__ __launch_bounds__(256) void frame_kernel(FrameArgs a) {
@@ -113,6 +103,8 @@ using namespace yx;
 
 // ---- renderer object ------------------------------------------------------
 
+static constexpr int kMaxQueues = 64;
+
 struct yx_renderer {
     float pos[3] = {0.0f, 0.0f, 0.0f};
     float dir[3] = {1.0f, 0.0f, 0.0f};
@@ -124,9 +116,14 @@ struct yx_renderer {
     bool has_scene = false;
     uint2* d_recs = nullptr;
     size_t n_recs = 0;
-    int depth = 0;
+    int depth = 0;        // deepest leaf level
+    int node_levels = 0;  // deepest node level + 1
     uint32_t* d_frame = nullptr;  // RenderFrame's device colour buffer
     std::vector<uint32_t> h_frame;
+    uint32_t* d_counters = nullptr;  // ray queue heads + retired-wave count
+    int device = -1;
+    int num_cus = 0;
+    yx_tuning tune = {YX_KERNEL_PERSISTENT, 256, 8, 8, 16, 8, 2, 0};
 };
 
 #define YX_HIP(call)                                                                        \
@@ -171,28 +168,103 @@ static void init_ray_dir(const yx_renderer* r, float* out) {
     }
 }
 
+static int ensure_device(yx_renderer* r) {
+    if (r->device >= 0) return YX_OK;
+    int dev = 0;
+    YX_HIP(hipGetDevice(&dev));
+    int cus = 0;
+    YX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    hipError_t e = hipMalloc((void**)&r->d_counters, (kMaxQueues + 1) * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        r->d_counters = nullptr;
+        return fail(YX_ERR_OOM, "hipMalloc counters: %s", hipGetErrorString(e));
+    }
+    YX_HIP(hipMemset(r->d_counters, 0, (kMaxQueues + 1) * sizeof(uint32_t)));
+    r->device = dev;
+    r->num_cus = cus;
+    return YX_OK;
+}
+
+template <int W, int BLOCK>
+static int launch_persist_t(yx_renderer* r, PersistArgs& a, size_t lds, hipStream_t stream) {
+    const void* fn = (const void*)persist_kernel<W, BLOCK>;
+    int per_cu = r->tune.waves_per_cu > 0 ? std::max(1, r->tune.waves_per_cu / (BLOCK / 64)) : 0;
+    if (per_cu == 0) {
+        int occ = 0;
+        YX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BLOCK, lds));
+        per_cu = std::max(1, occ);
+    }
+    const uint32_t grid = (uint32_t)(per_cu * r->num_cus);
+    a.n_waves = grid * (BLOCK / 64);
+    hipLaunchKernelGGL((persist_kernel<W, BLOCK>), dim3(grid), dim3(BLOCK), lds, stream, a);
+    YX_HIP(hipGetLastError());
+    return YX_OK;
+}
+
 static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t stream) {
-    FrameArgs a;
+    float basis[9];
+    init_ray_dir(r, basis);
+    if (r->tune.kernel == YX_KERNEL_PER_PIXEL || !r->d_recs) {
+        FrameArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.recs = r->d_recs;
+        for (int i = 0; i < 3; ++i) {
+            a.pos[i] = r->pos[i];
+            a.light[i] = r->light[i];
+        }
+        std::memcpy(a.basis, basis, sizeof(basis));
+        a.light_mode = r->light_mode;
+        a.w = r->w;
+        a.h = r->h;
+        a.out = out;
+        dim3 grid((r->w + 15) / 16, (r->h + 15) / 16);
+        // the per-lane stack holds node levels 0 .. node_levels - 2
+        if (r->node_levels <= 9)
+            hipLaunchKernelGGL(frame_kernel<8>, grid, dim3(256), 0, stream, a);
+        else if (r->node_levels <= 13)
+            hipLaunchKernelGGL(frame_kernel<12>, grid, dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL(frame_kernel<kMaxDepth>, grid, dim3(256), 0, stream, a);
+        YX_HIP(hipGetLastError());
+        return YX_OK;
+    }
+    int rc = ensure_device(r);
+    if (rc != YX_OK) return rc;
+    PersistArgs a;
     std::memset(&a, 0, sizeof(a));
     a.recs = r->d_recs;
     for (int i = 0; i < 3; ++i) {
         a.pos[i] = r->pos[i];
         a.light[i] = r->light[i];
     }
-    init_ray_dir(r, a.basis);
+    std::memcpy(a.basis, basis, sizeof(basis));
     a.light_mode = r->light_mode;
     a.w = r->w;
     a.h = r->h;
     a.out = out;
-    dim3 grid((r->w + 15) / 16, (r->h + 15) / 16);
-    if (r->depth <= 8)
-        hipLaunchKernelGGL(frame_kernel<8>, grid, dim3(256), 0, stream, a);
-    else if (r->depth <= 12)
-        hipLaunchKernelGGL(frame_kernel<12>, grid, dim3(256), 0, stream, a);
-    else
-        hipLaunchKernelGGL(frame_kernel<kMaxDepth>, grid, dim3(256), 0, stream, a);
-    YX_HIP(hipGetLastError());
-    return YX_OK;
+    a.counters = r->d_counters;
+    const yx_tuning& t = r->tune;
+    a.tile_wl = __builtin_ctz((unsigned)t.tile_w);
+    a.tile_hl = __builtin_ctz((unsigned)t.tile_h);
+    a.tiles_x = (r->w + t.tile_w - 1) / t.tile_w;
+    const int tiles_y = (r->h + t.tile_h - 1) / t.tile_h;
+    const uint32_t tile = (uint32_t)(t.tile_w * t.tile_h);
+    a.total = (uint32_t)a.tiles_x * (uint32_t)tiles_y * tile;
+    a.nq = t.queues;
+    const uint32_t tiles_per_q = ((uint32_t)a.tiles_x * (uint32_t)tiles_y + t.queues - 1) / t.queues;
+    a.per_q = tiles_per_q * tile;
+    a.refill_min = t.refill_min;
+    a.levels = std::max(1, r->node_levels);
+    const int block = t.block;
+    const size_t lds = (size_t)a.levels * (size_t)block * sizeof(uint2);
+    if (block == 256) {
+        if (t.window == 0) return launch_persist_t<0, 256>(r, a, lds, stream);
+        if (t.window == 1) return launch_persist_t<1, 256>(r, a, lds, stream);
+        return launch_persist_t<2, 256>(r, a, lds, stream);
+    }
+    if (t.window == 0) return launch_persist_t<0, 512>(r, a, lds, stream);
+    if (t.window == 1) return launch_persist_t<1, 512>(r, a, lds, stream);
+    return launch_persist_t<2, 512>(r, a, lds, stream);
 }
 
 extern "C" int yx_renderer_create(yx_renderer** out) {
@@ -208,7 +280,32 @@ extern "C" void yx_renderer_destroy(yx_renderer* r) {
     if (!r) return;
     if (r->d_recs) (void)hipFree(r->d_recs);
     if (r->d_frame) (void)hipFree(r->d_frame);
+    if (r->d_counters) (void)hipFree(r->d_counters);
     delete r;
+}
+
+extern "C" int yx_renderer_set_tuning(yx_renderer* r, const yx_tuning* t) {
+    clear_error();
+    if (!r || !t) return fail(YX_ERR_INVALID_ARG, "null argument");
+    auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+    if (t->kernel != YX_KERNEL_PER_PIXEL && t->kernel != YX_KERNEL_PERSISTENT)
+        return fail(YX_ERR_INVALID_ARG, "unknown kernel %d", t->kernel);
+    if (t->block != 256 && t->block != 512) return fail(YX_ERR_INVALID_ARG, "block must be 256 or 512");
+    if (!pow2(t->tile_w) || !pow2(t->tile_h) || t->tile_w * t->tile_h > 4096)
+        return fail(YX_ERR_INVALID_ARG, "tile %dx%d: powers of two, at most 4096 pixels", t->tile_w, t->tile_h);
+    if (t->refill_min < 1 || t->refill_min > 64) return fail(YX_ERR_INVALID_ARG, "refill_min must be 1..64");
+    if (t->queues < 1 || t->queues > kMaxQueues) return fail(YX_ERR_INVALID_ARG, "queues must be 1..%d", kMaxQueues);
+    if (t->window < 0 || t->window > 2) return fail(YX_ERR_INVALID_ARG, "window must be 0, 1 or 2");
+    if (t->waves_per_cu < 0 || t->waves_per_cu > 32) return fail(YX_ERR_INVALID_ARG, "waves_per_cu must be 0..32");
+    r->tune = *t;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_get_tuning(const yx_renderer* r, yx_tuning* t) {
+    clear_error();
+    if (!r || !t) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *t = r->tune;
+    return YX_OK;
 }
 
 extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
@@ -228,6 +325,7 @@ extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
     }
     r->n_recs = 0;
     r->depth = 0;
+    r->node_levels = 0;
     if (!t.empty) {
         const size_t bytes = t.recs.size() * sizeof(uint32_t);
         hipError_t e = hipMalloc((void**)&r->d_recs, bytes);
@@ -238,6 +336,7 @@ extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
         YX_HIP(hipMemcpy(r->d_recs, t.recs.data(), bytes, hipMemcpyHostToDevice));
         r->n_recs = t.recs.size() / 2;
         r->depth = t.depth;
+        r->node_levels = t.node_levels;
     }
     r->has_scene = true;
     return YX_OK;

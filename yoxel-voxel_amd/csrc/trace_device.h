@@ -61,12 +61,16 @@ struct Cell {
 
 __device__ __forceinline__ bool setup_trace(const float p[3], const Ray3& d, Cell& c, uint32_t& flags) {
     flags = 0;
-    if (d.x > 0) { c.t1x = (-p[0]) / d.x; c.t2x = (1.0f - p[0]) / d.x; }
-    else         { c.t2x = (-p[0]) / d.x; c.t1x = (1.0f - p[0]) / d.x; }
-    if (d.y > 0) { c.t1y = (-p[1]) / d.y; c.t2y = (1.0f - p[1]) / d.y; }
-    else         { c.t2y = (-p[1]) / d.y; c.t1y = (1.0f - p[1]) / d.y; }
-    if (d.z > 0) { c.t1z = (-p[2]) / d.z; c.t2z = (1.0f - p[2]) / d.z; }
-    else         { c.t2z = (-p[2]) / d.z; c.t1z = (1.0f - p[2]) / d.z; }
+    // plane x = 0 at a, plane x = 1 at b; entry is the nearer one
+    const float ax = (-p[0]) / d.x, bx = (1.0f - p[0]) / d.x;
+    const float ay = (-p[1]) / d.y, by = (1.0f - p[1]) / d.y;
+    const float az = (-p[2]) / d.z, bz = (1.0f - p[2]) / d.z;
+    c.t1x = (d.x > 0) ? ax : bx;
+    c.t2x = (d.x > 0) ? bx : ax;
+    c.t1y = (d.y > 0) ? ay : by;
+    c.t2y = (d.y > 0) ? by : ay;
+    c.t1z = (d.z > 0) ? az : bz;
+    c.t2z = (d.z > 0) ? bz : az;
     if (d.x < 0) flags |= 1u;
     if (d.y < 0) flags |= 2u;
     if (d.z < 0) flags |= 4u;
@@ -107,6 +111,19 @@ __device__ __forceinline__ bool go_next(int& ch, Cell& c) {
 __device__ __forceinline__ int entry_axis(const Cell& c) {
     if (c.t1x < c.t1y) return (c.t1y > c.t1z) ? 1 : 2;
     return (c.t1x > c.t1z) ? 0 : 2;
+}
+
+// Lambert over the RGBA8 payload; DEFINED HERE (the reference shader source is
+// missing): k = 0.1 + 0.9 * max(0, n.L) * visible, n = entry-face normal;
+// channel = (uint8)(c * k + 0.5), alpha 255.
+__device__ __forceinline__ uint32_t shade_rgba(uint32_t leaf, float k) {
+    uint32_t out = 0xff000000u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float v = (float)((leaf >> (8 * c)) & 0xffu) * k + 0.5f;
+        out |= ((uint32_t)v & 0xffu) << (8 * c);
+    }
+    return out;
 }
 
 struct HitInfo {

@@ -133,6 +133,21 @@ class SVORenderer:
     def SetLight(self, direction, mode=_lib.YX_LIGHT_SUN_SHADOW):
         check(lib().yx_renderer_set_light(self._h, f3(direction), int(mode)))
 
+    def get_tuning(self):
+        t = _lib.Tuning()
+        check(lib().yx_renderer_get_tuning(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, **kw):
+        """Update launch-shape fields (see yx_tuning in include/yoxel.h)."""
+        t = _lib.Tuning()
+        check(lib().yx_renderer_get_tuning(self._h, ctypes.byref(t)))
+        for k, v in kw.items():
+            if not hasattr(t, k):
+                raise KeyError(k)
+            setattr(t, k, int(v))
+        check(lib().yx_renderer_set_tuning(self._h, ctypes.byref(t)))
+
     def ray_basis(self):
         b = (ctypes.c_float * 9)()
         check(lib().yx_renderer_ray_basis(self._h, b))
