@@ -123,6 +123,7 @@ struct yx_renderer {
     uint32_t* d_counters = nullptr;  // ray queue heads + retired-wave count
     int device = -1;
     int num_cus = 0;
+    size_t max_lds = 0;  // dynamic LDS bytes a workgroup may use
     yx_tuning tune = {YX_KERNEL_PERSISTENT, 256, 8, 8, 16, 8, 2, 0};
 };
 
@@ -174,6 +175,9 @@ static int ensure_device(yx_renderer* r) {
     YX_HIP(hipGetDevice(&dev));
     int cus = 0;
     YX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    int lds_max = 0;
+    YX_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    r->max_lds = (size_t)lds_max;
     hipError_t e = hipMalloc((void**)&r->d_counters, (kMaxQueues + 1) * sizeof(uint32_t));
     if (e != hipSuccess) {
         r->d_counters = nullptr;
@@ -255,8 +259,15 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
     a.per_q = tiles_per_q * tile;
     a.refill_min = t.refill_min;
     a.levels = std::max(1, r->node_levels);
-    const int block = t.block;
-    const size_t lds = (size_t)a.levels * (size_t)block * sizeof(uint2);
+    int block = t.block;
+    size_t lds = (size_t)a.levels * (size_t)block * sizeof(uint2);
+    if (lds > r->max_lds && block == 512) {  // deep trees: same results with smaller workgroups
+        block = 256;
+        lds = (size_t)a.levels * (size_t)block * sizeof(uint2);
+    }
+    if (lds > r->max_lds)
+        return fail(YX_ERR_RANGE, "per-level LDS rows need %zu bytes per workgroup, device allows %zu", lds,
+                    r->max_lds);
     if (block == 256) {
         if (t.window == 0) return launch_persist_t<0, 256>(r, a, lds, stream);
         if (t.window == 1) return launch_persist_t<1, 256>(r, a, lds, stream);
