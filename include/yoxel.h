@@ -148,11 +148,14 @@ int yx_renderer_scene_bytes(const yx_renderer* r, uint64_t* bytes);
  * reference fixes 16x16 / 16x32 blocks, trace_cuda.py:26); set by the
  * autotuner. Every setting renders identical outputs. */
 #define YX_KERNEL_PER_PIXEL  0 /* one thread per pixel, 16x16 tiles           */
-#define YX_KERNEL_PERSISTENT 1 /* persistent waves on atomic ray queues       */
+#define YX_KERNEL_PERSISTENT 1 /* persistent waves on atomic ray queues, per-lane
+                                  traversal stack in LDS                       */
+#define YX_KERNEL_REPLAY     2 /* persistent waves, no t stack: ancestors are
+                                  rebuilt exactly by replay (deep trees)       */
 typedef struct yx_tuning {
     int32_t kernel;       /* YX_KERNEL_*                                        */
-    int32_t block;        /* threads per workgroup of the persistent kernel:
-                             256 or 512                                         */
+    int32_t block;        /* threads per workgroup: 64, 128, 256 (stack kernel)
+                             or 256, 512 (replay kernel)                        */
     int32_t tile_w;       /* ray packet (queue tile) width in pixels, power of 2 */
     int32_t tile_h;       /* ray packet height, power of 2; tile_w*tile_h <= 4096 */
     int32_t refill_min;   /* idle lanes before a wave takes new rays, 1..64     */

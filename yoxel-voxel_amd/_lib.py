@@ -95,6 +95,7 @@ class Tuning(ctypes.Structure):
 
 YX_KERNEL_PER_PIXEL = 0
 YX_KERNEL_PERSISTENT = 1
+YX_KERNEL_REPLAY = 2
 
 _P = ctypes.c_void_p
 _F3 = ctypes.POINTER(ctypes.c_float)

@@ -16,6 +16,7 @@
 #include "internal.h"
 #include "trace_device.h"
 #include "trace_persist.h"
+#include "trace_stack.h"
 
 namespace yx {
 
@@ -124,7 +125,7 @@ struct yx_renderer {
     int device = -1;
     int num_cus = 0;
     size_t max_lds = 0;  // dynamic LDS bytes a workgroup may use
-    yx_tuning tune = {YX_KERNEL_PERSISTENT, 256, 8, 8, 16, 8, 2, 0};
+    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 16, 1, 2, 0};
 };
 
 #define YX_HIP(call)                                                                        \
@@ -189,21 +190,25 @@ static int ensure_device(yx_renderer* r) {
     return YX_OK;
 }
 
-template <int W, int BLOCK>
-static int launch_persist_t(yx_renderer* r, PersistArgs& a, size_t lds, hipStream_t stream) {
-    const void* fn = (const void*)persist_kernel<W, BLOCK>;
-    int per_cu = r->tune.waves_per_cu > 0 ? std::max(1, r->tune.waves_per_cu / (BLOCK / 64)) : 0;
+// Persistent launch: as many workgroups as fit on the device at once (or the
+// tuned waves per CU); every wave loops over the ray queues until they drain.
+static int launch_persistent(yx_renderer* r, const void* fn, void (*kern)(PersistArgs), int block,
+                             PersistArgs& a, size_t lds, hipStream_t stream) {
+    int per_cu = r->tune.waves_per_cu > 0 ? std::max(1, r->tune.waves_per_cu / (block / 64)) : 0;
     if (per_cu == 0) {
         int occ = 0;
-        YX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, BLOCK, lds));
+        YX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, lds));
         per_cu = std::max(1, occ);
     }
     const uint32_t grid = (uint32_t)(per_cu * r->num_cus);
-    a.n_waves = grid * (BLOCK / 64);
-    hipLaunchKernelGGL((persist_kernel<W, BLOCK>), dim3(grid), dim3(BLOCK), lds, stream, a);
+    a.n_waves = grid * (uint32_t)(block / 64);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, a);
     YX_HIP(hipGetLastError());
     return YX_OK;
 }
+
+#define YX_LAUNCH(KERNEL, BLOCK) \
+    launch_persistent(r, (const void*)KERNEL, KERNEL, BLOCK, a, lds, stream)
 
 static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t stream) {
     float basis[9];
@@ -258,8 +263,28 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
     const uint32_t tiles_per_q = ((uint32_t)a.tiles_x * (uint32_t)tiles_y + t.queues - 1) / t.queues;
     a.per_q = tiles_per_q * tile;
     a.refill_min = t.refill_min;
-    a.levels = std::max(1, r->node_levels);
+    // stack rows: levels 0 .. node_levels - 2 are pushed
+    a.levels = std::max(1, r->node_levels - 1);
+    int kernel = t.kernel;
     int block = t.block;
+    if (kernel == YX_KERNEL_PERSISTENT) {
+        // 32 bytes per level and lane; shrink the workgroup, then fall back to
+        // the replay kernel (same results) when a row set does not fit
+        size_t lds = (size_t)a.levels * 32u * (size_t)block;
+        while (lds > r->max_lds && block > 64) {
+            block /= 2;
+            lds = (size_t)a.levels * 32u * (size_t)block;
+        }
+        if (lds <= r->max_lds) {
+            if (block == 64) return YX_LAUNCH(stack_kernel<64>, 64);
+            if (block == 128) return YX_LAUNCH(stack_kernel<128>, 128);
+            return YX_LAUNCH(stack_kernel<256>, 256);
+        }
+        kernel = YX_KERNEL_REPLAY;
+        block = 256;
+    }
+    a.levels = std::max(1, r->node_levels);
+    block = block >= 512 ? 512 : 256;
     size_t lds = (size_t)a.levels * (size_t)block * sizeof(uint2);
     if (lds > r->max_lds && block == 512) {  // deep trees: same results with smaller workgroups
         block = 256;
@@ -269,13 +294,13 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
         return fail(YX_ERR_RANGE, "per-level LDS rows need %zu bytes per workgroup, device allows %zu", lds,
                     r->max_lds);
     if (block == 256) {
-        if (t.window == 0) return launch_persist_t<0, 256>(r, a, lds, stream);
-        if (t.window == 1) return launch_persist_t<1, 256>(r, a, lds, stream);
-        return launch_persist_t<2, 256>(r, a, lds, stream);
+        if (t.window == 0) return YX_LAUNCH((persist_kernel<0, 256>), 256);
+        if (t.window == 1) return YX_LAUNCH((persist_kernel<1, 256>), 256);
+        return YX_LAUNCH((persist_kernel<2, 256>), 256);
     }
-    if (t.window == 0) return launch_persist_t<0, 512>(r, a, lds, stream);
-    if (t.window == 1) return launch_persist_t<1, 512>(r, a, lds, stream);
-    return launch_persist_t<2, 512>(r, a, lds, stream);
+    if (t.window == 0) return YX_LAUNCH((persist_kernel<0, 512>), 512);
+    if (t.window == 1) return YX_LAUNCH((persist_kernel<1, 512>), 512);
+    return YX_LAUNCH((persist_kernel<2, 512>), 512);
 }
 
 extern "C" int yx_renderer_create(yx_renderer** out) {
@@ -299,9 +324,10 @@ extern "C" int yx_renderer_set_tuning(yx_renderer* r, const yx_tuning* t) {
     clear_error();
     if (!r || !t) return fail(YX_ERR_INVALID_ARG, "null argument");
     auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
-    if (t->kernel != YX_KERNEL_PER_PIXEL && t->kernel != YX_KERNEL_PERSISTENT)
+    if (t->kernel < YX_KERNEL_PER_PIXEL || t->kernel > YX_KERNEL_REPLAY)
         return fail(YX_ERR_INVALID_ARG, "unknown kernel %d", t->kernel);
-    if (t->block != 256 && t->block != 512) return fail(YX_ERR_INVALID_ARG, "block must be 256 or 512");
+    if (t->block != 64 && t->block != 128 && t->block != 256 && t->block != 512)
+        return fail(YX_ERR_INVALID_ARG, "block must be 64, 128, 256 or 512");
     if (!pow2(t->tile_w) || !pow2(t->tile_h) || t->tile_w * t->tile_h > 4096)
         return fail(YX_ERR_INVALID_ARG, "tile %dx%d: powers of two, at most 4096 pixels", t->tile_w, t->tile_h);
     if (t->refill_min < 1 || t->refill_min > 64) return fail(YX_ERR_INVALID_ARG, "refill_min must be 1..64");

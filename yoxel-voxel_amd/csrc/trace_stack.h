@@ -1,0 +1,258 @@
+// Persistent SVO ray-cast kernel with a per-lane traversal stack in LDS.
+//
+// Same work distribution as persist_kernel (atomic ray queues, ballot refill of
+// idle lanes, shadow rays continued in the lane that found the hit). The
+// traversal keeps, per lane, the exact state the reference recursion keeps on
+// its call stack (RecTrace, cell/ppu_renderer.cpp:18-41): one 32-byte row per
+// level holding the cell (t1, t2) and the node record, two 16-byte LDS
+// accesses per push or pop. Each iteration of the wave loop performs exactly
+// one step per lane - test the current child and descend, or GoNext, or pop
+// one level - so the wave never waits on a long per-lane inner loop.
+#pragma once
+
+#include "trace_persist.h"
+
+namespace yx {
+
+// Wave-cooperative refill: idle lanes of the wave take consecutive ray ids
+// from queue `q` with one atomic. Returns true in a lane that got a pixel.
+struct QueueCursor {
+    int q;
+    int left;  // queues not yet found drained (wave-uniform)
+};
+
+__device__ __forceinline__ bool take_ray(const PersistArgs& a, QueueCursor& qc, uint64_t need, bool idle,
+                                         uint32_t lane, int& x, int& y) {
+    const uint32_t cnt = (uint32_t)__popcll(need);
+    const int leader = __ffsll((unsigned long long)need) - 1;
+    uint32_t got = 0;
+    if ((int)lane == leader) got = atomicAdd(&a.counters[qc.q], cnt);
+    got = __shfl(got, leader);
+    const uint32_t qbeg = (uint32_t)qc.q * a.per_q;
+    const uint32_t qend = min(a.total, qbeg + a.per_q);
+    if (qbeg + got + cnt >= qend) {
+        qc.q = (qc.q + 1 == a.nq) ? 0 : qc.q + 1;
+        --qc.left;
+    }
+    if (!idle || qbeg + got >= qend) return false;
+    const uint32_t my = qbeg + got + lanes_below(need);
+    if (my >= qend) return false;
+    const int tl = a.tile_wl + a.tile_hl;
+    const uint32_t tile = my >> tl, within = my & ((1u << tl) - 1u);
+    const int tx = (int)(tile % (uint32_t)a.tiles_x);
+    const int ty = (int)(tile / (uint32_t)a.tiles_x);
+    x = (tx << a.tile_wl) + (int)(within & ((1u << a.tile_wl) - 1u));
+    y = (ty << a.tile_hl) + (int)(within >> a.tile_wl);
+    return x < a.w && y < a.h;
+}
+
+__device__ __forceinline__ void write_miss(const yx_outputs& o, uint32_t pix, uint32_t visits) {
+    if (o.hit_id) o.hit_id[pix] = YX_MISS;
+    if (o.voxel) {
+        o.voxel[3 * (size_t)pix] = -1;
+        o.voxel[3 * (size_t)pix + 1] = -1;
+        o.voxel[3 * (size_t)pix + 2] = -1;
+    }
+    if (o.t) o.t[pix] = -1.0f;
+    if (o.rgba) o.rgba[pix] = 0u;
+    if (o.shadow) o.shadow[pix] = 2;
+    if (o.visits) o.visits[pix] = visits;
+}
+
+__device__ __forceinline__ void write_final(const yx_outputs& o, uint32_t pix, uint32_t color, float k,
+                                            uint8_t shadow, uint32_t visits) {
+    if (o.rgba) o.rgba[pix] = shade_rgba(color, k);
+    if (o.shadow) o.shadow[pix] = shadow;
+    if (o.visits) o.visits[pix] = visits;
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
+    extern __shared__ uint4 sh_stk[];  // [levels][2][BLOCK]
+    const int tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint2* __restrict__ recs = a.recs;
+
+    QueueCursor qc;
+    qc.q = (int)(blockIdx.x % (uint32_t)a.nq);
+    qc.left = a.nq;
+
+    int status = kNeed;
+    bool fresh = false, adv = false;
+    uint32_t pix = 0;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f, dx = 1.0f, dy = 1.0f, dz = 1.0f;
+    Cell c;
+    c.t1x = c.t1y = c.t1z = c.t2x = c.t2y = c.t2z = 0.0f;
+    uint32_t flags = 0;
+    int level = 0, ch = 0;
+    uint2 node = make_uint2(0u, 0u);
+    uint64_t chpath = 0;
+    uint32_t visits = 0, color = 0;
+    float lam = 0.0f;
+
+    for (;;) {
+        // ---- refill idle lanes -------------------------------------------------
+        const uint64_t need = __ballot(status == kNeed);
+        if (need != 0) {
+            const uint64_t busy = __ballot(status == kPrimary || status == kShadow);
+            if (busy == 0 || __popcll(need) >= a.refill_min) {
+                if (qc.left == 0) {
+                    if (status == kNeed) status = kRetired;
+                } else {
+                    int x = 0, y = 0;
+                    if (take_ray(a, qc, need, status == kNeed, lane, x, y)) {
+                        pix = (uint32_t)y * (uint32_t)a.w + (uint32_t)x;
+                        const Ray3 d = pixel_dir(a.basis, x, y);
+                        dx = adjust_coord(d.x);
+                        dy = adjust_coord(d.y);
+                        dz = adjust_coord(d.z);
+                        ox = a.pos[0];
+                        oy = a.pos[1];
+                        oz = a.pos[2];
+                        visits = 0;
+                        status = kPrimary;
+                        fresh = true;
+                    }
+                }
+            }
+        }
+        if (__ballot(status != kRetired) == 0) break;
+
+        // ---- start fresh rays (primary or shadow) -------------------------------
+        if (fresh) {
+            fresh = false;
+            const float p[3] = {ox, oy, oz};
+            Ray3 d;
+            d.x = dx;
+            d.y = dy;
+            d.z = dz;
+            const bool ok = recs != nullptr && setup_trace(p, d, c, flags) && min3(c.t2x, c.t2y, c.t2z) > 0;
+            if (ok) {
+                ++visits;
+                node = recs[0];
+                level = 0;
+                adv = false;
+                ch = find_first_child(c);
+            } else if (status == kPrimary) {
+                write_miss(a.out, pix, visits);
+                status = kNeed;
+            } else {
+                write_final(a.out, pix, color, 0.1f + 0.9f * lam, 0, visits);  // shadow ray leaves: lit
+                status = kNeed;
+            }
+        }
+
+        if (status != kPrimary && status != kShadow) continue;
+
+        // ---- one step: test the current child ------------------------------------
+        if (!adv) {
+            const int real = ch ^ (int)flags;
+            const uint32_t cm = node.y & 0xffu, lm = (node.y >> 8) & 0xffu;
+            const uint32_t below = (cm | lm) & ((1u << real) - 1u);
+            const float mt2 = min3(c.t2x, c.t2y, c.t2z);
+            const bool is_leaf = (lm >> real) & 1u;
+            if (is_leaf && (status == kPrimary || mt2 > 0)) {
+                if (status == kShadow) {
+                    write_final(a.out, pix, color, 0.1f, 1, visits);  // occluded
+                    status = kNeed;
+                } else {
+                    const uint2 leaf = recs[node.x + (uint32_t)__builtin_popcount(below)];
+                    int vx = 0, vy = 0, vz = 0;
+                    for (int j = 0; j < level; ++j) {
+                        const uint32_t s = bits3(chpath, j) ^ flags;
+                        vx = (vx << 1) | (int)(s & 1u);
+                        vy = (vy << 1) | (int)((s >> 1) & 1u);
+                        vz = (vz << 1) | (int)((s >> 2) & 1u);
+                    }
+                    vx = (vx << 1) | (real & 1);
+                    vy = (vy << 1) | ((real >> 1) & 1);
+                    vz = (vz << 1) | ((real >> 2) & 1);
+                    const float t = max3(c.t1x, c.t1y, c.t1z);
+                    if (a.out.hit_id) a.out.hit_id[pix] = leaf.y * 8u + (uint32_t)real;
+                    if (a.out.voxel) {
+                        a.out.voxel[3 * (size_t)pix] = vx;
+                        a.out.voxel[3 * (size_t)pix + 1] = vy;
+                        a.out.voxel[3 * (size_t)pix + 2] = vz;
+                    }
+                    if (a.out.t) a.out.t[pix] = t;
+                    const int e = entry_axis(c);
+                    const float de = e == 0 ? dx : (e == 1 ? dy : dz);
+                    if (a.light_mode == YX_LIGHT_HEADLIGHT) {
+                        lam = fabsf(de);
+                    } else {
+                        const float le = e == 0 ? a.light[0] : (e == 1 ? a.light[1] : a.light[2]);
+                        lam = dmax((de < 0) ? le : -le, 0.0f);
+                    }
+                    color = leaf.x;
+                    if (a.light_mode == YX_LIGHT_SUN_SHADOW && lam > 0.0f) {
+                        // hit point stepped half a cell back across the entry face
+                        const float half = __builtin_bit_cast(float, (uint32_t)(127 - (level + 2)) << 23);
+                        const float off = (de < 0) ? half : -half;
+                        float px = ox + dx * t, py = oy + dy * t, pz = oz + dz * t;
+                        ox = (e == 0) ? px + off : px;
+                        oy = (e == 1) ? py + off : py;
+                        oz = (e == 2) ? pz + off : pz;
+                        dx = adjust_coord(a.light[0]);
+                        dy = adjust_coord(a.light[1]);
+                        dz = adjust_coord(a.light[2]);
+                        status = kShadow;
+                        fresh = true;
+                    } else {
+                        write_final(a.out, pix, color, 0.1f + 0.9f * lam, 2, visits);
+                        status = kNeed;
+                    }
+                }
+                continue;
+            }
+            if (!is_leaf && ((cm >> real) & 1u) && mt2 > 0) {
+                // push this level, descend into the child node
+                ++visits;
+                uint4* row = sh_stk + (size_t)(2 * level) * BLOCK + tid;
+                row[0] = make_uint4(__float_as_uint(c.t1x), __float_as_uint(c.t1y), __float_as_uint(c.t1z),
+                                    __float_as_uint(c.t2x));
+                row[BLOCK] = make_uint4(__float_as_uint(c.t2y), __float_as_uint(c.t2z), node.x, node.y);
+                chpath = set3(chpath, level, (uint32_t)ch);
+                node = recs[node.x + (uint32_t)__builtin_popcount(below)];
+                ++level;
+                ch = find_first_child(c);
+                continue;
+            }
+            adv = true;
+        }
+        // ---- one step: next sibling, or pop one level ----------------------------
+        if (go_next(ch, c)) {
+            adv = false;
+        } else if (level == 0) {
+            if (status == kPrimary)
+                write_miss(a.out, pix, visits);
+            else
+                write_final(a.out, pix, color, 0.1f + 0.9f * lam, 0, visits);
+            status = kNeed;
+        } else {
+            --level;
+            const uint4* row = sh_stk + (size_t)(2 * level) * BLOCK + tid;
+            const uint4 r0 = row[0], r1 = row[BLOCK];
+            c.t1x = __uint_as_float(r0.x);
+            c.t1y = __uint_as_float(r0.y);
+            c.t1z = __uint_as_float(r0.z);
+            c.t2x = __uint_as_float(r0.w);
+            c.t2y = __uint_as_float(r1.x);
+            c.t2z = __uint_as_float(r1.y);
+            node = make_uint2(r1.z, r1.w);
+            ch = (int)bits3(chpath, level);
+        }
+    }
+
+    // last wave out resets the queues for the next launch
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&a.counters[a.nq], 1u);
+        if (done + 1 == a.n_waves) {
+            for (int k = 0; k < a.nq; ++k) atomicExch(&a.counters[k], 0u);
+            atomicExch(&a.counters[a.nq], 0u);
+            __threadfence();
+        }
+    }
+}
+
+}  // namespace yx
