@@ -160,7 +160,8 @@ typedef struct yx_tuning {
     int32_t tile_h;       /* ray packet height, power of 2; tile_w*tile_h <= 4096 */
     int32_t refill_min;   /* idle lanes before a wave takes new rays, 1..64     */
     int32_t queues;       /* ray queues, 1..64 (8 = one per XCD)                */
-    int32_t window;       /* ancestor cells kept in registers: 0, 1 or 2        */
+    int32_t window;       /* stack levels kept in registers: 0..6 (stack kernel
+                             uses 0, 2, 4, 6; replay kernel 0, 1, 2)           */
     int32_t waves_per_cu; /* persistent launch size; 0 = occupancy limit        */
 } yx_tuning;
 int yx_renderer_set_tuning(yx_renderer* r, const yx_tuning* tuning);

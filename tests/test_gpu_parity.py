@@ -19,9 +19,10 @@ TUNINGS = [
     dict(kernel=2, window=0, block=512, tile_w=16, tile_h=4, refill_min=1, queues=1),
     dict(kernel=2, window=1, block=256, tile_w=64, tile_h=1, refill_min=64, queues=13),
     dict(kernel=2, window=2, block=512, tile_w=32, tile_h=32, refill_min=32, queues=64, waves_per_cu=4),
-    dict(kernel=1, block=256, tile_w=1, tile_h=1, refill_min=8, queues=8, waves_per_cu=1),
-    dict(kernel=1, block=128, tile_w=16, tile_h=16, refill_min=1, queues=3),
-    dict(kernel=1, block=64, tile_w=32, tile_h=2, refill_min=64, queues=64, waves_per_cu=2),
+    dict(kernel=1, block=256, window=0, tile_w=1, tile_h=1, refill_min=8, queues=8, waves_per_cu=1),
+    dict(kernel=1, block=128, window=2, tile_w=16, tile_h=16, refill_min=1, queues=3),
+    dict(kernel=1, block=64, window=6, tile_w=32, tile_h=2, refill_min=64, queues=64, waves_per_cu=2),
+    dict(kernel=1, block=256, window=4, tile_w=4, tile_h=16, refill_min=20, queues=2),
 ]
 
 

@@ -125,7 +125,7 @@ struct yx_renderer {
     int device = -1;
     int num_cus = 0;
     size_t max_lds = 0;  // dynamic LDS bytes a workgroup may use
-    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 16, 1, 2, 0};
+    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 32, 1, 4, 0};
 };
 
 #define YX_HIP(call)                                                                        \
@@ -268,17 +268,33 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
     int kernel = t.kernel;
     int block = t.block;
     if (kernel == YX_KERNEL_PERSISTENT) {
-        // 32 bytes per level and lane; shrink the workgroup, then fall back to
-        // the replay kernel (same results) when a row set does not fit
-        size_t lds = (size_t)a.levels * 32u * (size_t)block;
+        // 32 bytes per LDS level row and lane (rows for levels older than the
+        // register window); shrink the workgroup, then fall back to the replay
+        // kernel (same results) when the rows do not fit
+        const int win = t.window <= 0 ? 0 : (t.window <= 2 ? 2 : (t.window <= 4 ? 4 : 6));
+        const int rows = std::max(1, a.levels - win);
+        size_t lds = (size_t)rows * 32u * (size_t)block;
         while (lds > r->max_lds && block > 64) {
             block /= 2;
-            lds = (size_t)a.levels * 32u * (size_t)block;
+            lds = (size_t)rows * 32u * (size_t)block;
         }
         if (lds <= r->max_lds) {
-            if (block == 64) return YX_LAUNCH(stack_kernel<64>, 64);
-            if (block == 128) return YX_LAUNCH(stack_kernel<128>, 128);
-            return YX_LAUNCH(stack_kernel<256>, 256);
+            if (block == 64) {
+                if (win == 0) return YX_LAUNCH((stack_kernel<64, 0>), 64);
+                if (win == 2) return YX_LAUNCH((stack_kernel<64, 2>), 64);
+                if (win == 4) return YX_LAUNCH((stack_kernel<64, 4>), 64);
+                return YX_LAUNCH((stack_kernel<64, 6>), 64);
+            }
+            if (block == 128) {
+                if (win == 0) return YX_LAUNCH((stack_kernel<128, 0>), 128);
+                if (win == 2) return YX_LAUNCH((stack_kernel<128, 2>), 128);
+                if (win == 4) return YX_LAUNCH((stack_kernel<128, 4>), 128);
+                return YX_LAUNCH((stack_kernel<128, 6>), 128);
+            }
+            if (win == 0) return YX_LAUNCH((stack_kernel<256, 0>), 256);
+            if (win == 2) return YX_LAUNCH((stack_kernel<256, 2>), 256);
+            if (win == 4) return YX_LAUNCH((stack_kernel<256, 4>), 256);
+            return YX_LAUNCH((stack_kernel<256, 6>), 256);
         }
         kernel = YX_KERNEL_REPLAY;
         block = 256;
@@ -300,7 +316,7 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
     }
     if (t.window == 0) return YX_LAUNCH((persist_kernel<0, 512>), 512);
     if (t.window == 1) return YX_LAUNCH((persist_kernel<1, 512>), 512);
-    return YX_LAUNCH((persist_kernel<2, 512>), 512);
+    return YX_LAUNCH((persist_kernel<2, 512>), 512);  // windows above 2 use 2
 }
 
 extern "C" int yx_renderer_create(yx_renderer** out) {
@@ -332,7 +348,7 @@ extern "C" int yx_renderer_set_tuning(yx_renderer* r, const yx_tuning* t) {
         return fail(YX_ERR_INVALID_ARG, "tile %dx%d: powers of two, at most 4096 pixels", t->tile_w, t->tile_h);
     if (t->refill_min < 1 || t->refill_min > 64) return fail(YX_ERR_INVALID_ARG, "refill_min must be 1..64");
     if (t->queues < 1 || t->queues > kMaxQueues) return fail(YX_ERR_INVALID_ARG, "queues must be 1..%d", kMaxQueues);
-    if (t->window < 0 || t->window > 2) return fail(YX_ERR_INVALID_ARG, "window must be 0, 1 or 2");
+    if (t->window < 0 || t->window > 6) return fail(YX_ERR_INVALID_ARG, "window must be 0..6");
     if (t->waves_per_cu < 0 || t->waves_per_cu > 32) return fail(YX_ERR_INVALID_ARG, "waves_per_cu must be 0..32");
     r->tune = *t;
     return YX_OK;

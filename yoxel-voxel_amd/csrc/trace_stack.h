@@ -66,9 +66,11 @@ __device__ __forceinline__ void write_final(const yx_outputs& o, uint32_t pix, u
     if (o.visits) o.visits[pix] = visits;
 }
 
-template <int BLOCK>
+// W: stack entries of the W levels above the current one held in registers
+// (a shift window); older levels live in LDS rows indexed by level.
+template <int BLOCK, int W>
 __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
-    extern __shared__ uint4 sh_stk[];  // [levels][2][BLOCK]
+    extern __shared__ uint4 sh_stk[];  // [levels - W][2][BLOCK]
     const int tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint2* __restrict__ recs = a.recs;
@@ -89,6 +91,25 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     uint64_t chpath = 0;
     uint32_t visits = 0, color = 0;
     float lam = 0.0f;
+    WinEntry win[W > 0 ? W : 1];
+
+    auto lds_store = [&](int row, const Cell& cc, uint2 nn) {
+        uint4* p = sh_stk + (size_t)(2 * row) * BLOCK + tid;
+        p[0] = make_uint4(__float_as_uint(cc.t1x), __float_as_uint(cc.t1y), __float_as_uint(cc.t1z),
+                          __float_as_uint(cc.t2x));
+        p[BLOCK] = make_uint4(__float_as_uint(cc.t2y), __float_as_uint(cc.t2z), nn.x, nn.y);
+    };
+    auto lds_load = [&](int row, Cell& cc, uint2& nn) {
+        const uint4* p = sh_stk + (size_t)(2 * row) * BLOCK + tid;
+        const uint4 r0 = p[0], r1 = p[BLOCK];
+        cc.t1x = __uint_as_float(r0.x);
+        cc.t1y = __uint_as_float(r0.y);
+        cc.t1z = __uint_as_float(r0.z);
+        cc.t2x = __uint_as_float(r0.w);
+        cc.t2y = __uint_as_float(r1.x);
+        cc.t2z = __uint_as_float(r1.y);
+        nn = make_uint2(r1.z, r1.w);
+    };
 
     for (;;) {
         // ---- refill idle lanes -------------------------------------------------
@@ -207,10 +228,15 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
             if (!is_leaf && ((cm >> real) & 1u) && mt2 > 0) {
                 // push this level, descend into the child node
                 ++visits;
-                uint4* row = sh_stk + (size_t)(2 * level) * BLOCK + tid;
-                row[0] = make_uint4(__float_as_uint(c.t1x), __float_as_uint(c.t1y), __float_as_uint(c.t1z),
-                                    __float_as_uint(c.t2x));
-                row[BLOCK] = make_uint4(__float_as_uint(c.t2y), __float_as_uint(c.t2z), node.x, node.y);
+                if (W > 0) {
+                    if (level >= W) lds_store(level - W, win[W - 1].c, win[W - 1].rec);
+#pragma unroll
+                    for (int k = W - 1; k > 0; --k) win[k] = win[k - 1];
+                    win[0].c = c;
+                    win[0].rec = node;
+                } else {
+                    lds_store(level, c, node);
+                }
                 chpath = set3(chpath, level, (uint32_t)ch);
                 node = recs[node.x + (uint32_t)__builtin_popcount(below)];
                 ++level;
@@ -230,15 +256,15 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
             status = kNeed;
         } else {
             --level;
-            const uint4* row = sh_stk + (size_t)(2 * level) * BLOCK + tid;
-            const uint4 r0 = row[0], r1 = row[BLOCK];
-            c.t1x = __uint_as_float(r0.x);
-            c.t1y = __uint_as_float(r0.y);
-            c.t1z = __uint_as_float(r0.z);
-            c.t2x = __uint_as_float(r0.w);
-            c.t2y = __uint_as_float(r1.x);
-            c.t2z = __uint_as_float(r1.y);
-            node = make_uint2(r1.z, r1.w);
+            if (W > 0) {
+                c = win[0].c;
+                node = win[0].rec;
+#pragma unroll
+                for (int k = 0; k + 1 < W; ++k) win[k] = win[k + 1];
+                if (level >= W) lds_load(level - W, win[W - 1].c, win[W - 1].rec);
+            } else {
+                lds_load(level, c, node);
+            }
             ch = (int)bits3(chpath, level);
         }
     }
