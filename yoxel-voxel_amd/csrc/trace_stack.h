@@ -1,13 +1,20 @@
-// Persistent SVO ray-cast kernel with a per-lane traversal stack in LDS.
+// Persistent SVO ray-cast kernel with a per-lane traversal stack.
 //
-// Same work distribution as persist_kernel (atomic ray queues, ballot refill of
-// idle lanes, shadow rays continued in the lane that found the hit). The
-// traversal keeps, per lane, the exact state the reference recursion keeps on
-// its call stack (RecTrace, cell/ppu_renderer.cpp:18-41): one 32-byte row per
-// level holding the cell (t1, t2) and the node record, two 16-byte LDS
-// accesses per push or pop. Each iteration of the wave loop performs exactly
-// one step per lane - test the current child and descend, or GoNext, or pop
-// one level - so the wave never waits on a long per-lane inner loop.
+// Work distribution: atomic ray queues (workgroup b serves queue b % nq and
+// steals from the others once it drains); the last wave to retire resets the
+// queues for the next launch.
+//
+// Each lane runs one ray at a time: a primary ray, then, after a lit hit, its
+// pixel's shadow ray. The wave loop does one traversal step per tracing lane
+// per iteration - test the current child and descend, or GoNext, or pop one
+// level - with the exact state the reference recursion keeps on its call stack
+// (RecTrace, cell/ppu_renderer.cpp:18-41): per level the cell (t1, t2) and the
+// node record, the W most recent levels in registers (a shift window) and the
+// older ones in 32-byte LDS rows. Everything that happens once per ray - taking
+// ray ids, ray setup with its nine divisions, hit decoding, shading, the shadow
+// ray spawn, output writes - is parked and done for many lanes at once, only
+// when at least `refill_min` lanes wait (or none is tracing), so the wave does
+// not pay for one lane's rare event in most iterations.
 #pragma once
 
 #include "trace_persist.h"
@@ -66,6 +73,19 @@ __device__ __forceinline__ void write_final(const yx_outputs& o, uint32_t pix, u
     if (o.visits) o.visits[pix] = visits;
 }
 
+// lane states
+enum : int {
+    sIdle = 0,     // needs a ray id
+    sStartP = 1,   // primary ray assigned, not set up
+    sStartS = 2,   // shadow ray assigned, not set up
+    sTraceP = 3,   // tracing a primary ray
+    sTraceS = 4,   // tracing a shadow ray
+    sHitP = 5,     // primary ray stopped on a leaf
+    sMissP = 6,    // primary ray left the cube
+    sEndS = 7,     // shadow ray finished (occluded flag)
+    sRetired = 8,
+};
+
 // W: stack entries of the W levels above the current one held in registers
 // (a shift window); older levels live in LDS rows indexed by level.
 template <int BLOCK, int W>
@@ -79,8 +99,8 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     qc.q = (int)(blockIdx.x % (uint32_t)a.nq);
     qc.left = a.nq;
 
-    int status = kNeed;
-    bool fresh = false, adv = false;
+    int status = sIdle;
+    bool adv = false, occluded = false;
     uint32_t pix = 0;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, dx = 1.0f, dy = 1.0f, dz = 1.0f;
     Cell c;
@@ -112,16 +132,76 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     };
 
     for (;;) {
-        // ---- refill idle lanes -------------------------------------------------
-        const uint64_t need = __ballot(status == kNeed);
-        if (need != 0) {
-            const uint64_t busy = __ballot(status == kPrimary || status == kShadow);
-            if (busy == 0 || __popcll(need) >= a.refill_min) {
+        const bool tracing = status == sTraceP || status == sTraceS;
+        const uint64_t tr_mask = __ballot(tracing);
+        const uint64_t wait_mask = __ballot(!tracing && status != sRetired);
+        if (wait_mask != 0 && (tr_mask == 0 || __popcll(wait_mask) >= a.refill_min)) {
+            // ---- batched once-per-ray work ----------------------------------
+            if (status == sHitP) {
+                const int real = ch ^ (int)flags;
+                const uint32_t cm = node.y & 0xffu, lm = (node.y >> 8) & 0xffu;
+                const uint32_t below = (cm | lm) & ((1u << real) - 1u);
+                const uint2 leaf = recs[node.x + (uint32_t)__builtin_popcount(below)];
+                int vx = 0, vy = 0, vz = 0;
+                for (int j = 0; j < level; ++j) {
+                    const uint32_t s = bits3(chpath, j) ^ flags;
+                    vx = (vx << 1) | (int)(s & 1u);
+                    vy = (vy << 1) | (int)((s >> 1) & 1u);
+                    vz = (vz << 1) | (int)((s >> 2) & 1u);
+                }
+                vx = (vx << 1) | (real & 1);
+                vy = (vy << 1) | ((real >> 1) & 1);
+                vz = (vz << 1) | ((real >> 2) & 1);
+                const float t = max3(c.t1x, c.t1y, c.t1z);
+                if (a.out.hit_id) a.out.hit_id[pix] = leaf.y * 8u + (uint32_t)real;
+                if (a.out.voxel) {
+                    a.out.voxel[3 * (size_t)pix] = vx;
+                    a.out.voxel[3 * (size_t)pix + 1] = vy;
+                    a.out.voxel[3 * (size_t)pix + 2] = vz;
+                }
+                if (a.out.t) a.out.t[pix] = t;
+                const int e = entry_axis(c);
+                const float de = e == 0 ? dx : (e == 1 ? dy : dz);
+                if (a.light_mode == YX_LIGHT_HEADLIGHT) {
+                    lam = fabsf(de);
+                } else {
+                    const float le = e == 0 ? a.light[0] : (e == 1 ? a.light[1] : a.light[2]);
+                    lam = dmax((de < 0) ? le : -le, 0.0f);
+                }
+                color = leaf.x;
+                if (a.light_mode == YX_LIGHT_SUN_SHADOW && lam > 0.0f) {
+                    // hit point stepped half a cell back across the entry face
+                    const float half = __builtin_bit_cast(float, (uint32_t)(127 - (level + 2)) << 23);
+                    const float off = (de < 0) ? half : -half;
+                    const float px = ox + dx * t, py = oy + dy * t, pz = oz + dz * t;
+                    ox = (e == 0) ? px + off : px;
+                    oy = (e == 1) ? py + off : py;
+                    oz = (e == 2) ? pz + off : pz;
+                    dx = adjust_coord(a.light[0]);
+                    dy = adjust_coord(a.light[1]);
+                    dz = adjust_coord(a.light[2]);
+                    status = sStartS;
+                } else {
+                    write_final(a.out, pix, color, 0.1f + 0.9f * lam, 2, visits);
+                    status = sIdle;
+                }
+            } else if (status == sMissP) {
+                write_miss(a.out, pix, visits);
+                status = sIdle;
+            } else if (status == sEndS) {
+                if (occluded)
+                    write_final(a.out, pix, color, 0.1f, 1, visits);
+                else
+                    write_final(a.out, pix, color, 0.1f + 0.9f * lam, 0, visits);
+                status = sIdle;
+            }
+            const uint64_t need = __ballot(status == sIdle);
+            if (need != 0) {
                 if (qc.left == 0) {
-                    if (status == kNeed) status = kRetired;
+                    if (status == sIdle) status = sRetired;
                 } else {
                     int x = 0, y = 0;
-                    if (take_ray(a, qc, need, status == kNeed, lane, x, y)) {
+                    if (take_ray(a, qc, need, status == sIdle, lane, x, y)) {
                         pix = (uint32_t)y * (uint32_t)a.w + (uint32_t)x;
                         const Ray3 d = pixel_dir(a.basis, x, y);
                         dx = adjust_coord(d.x);
@@ -131,97 +211,49 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                         oy = a.pos[1];
                         oz = a.pos[2];
                         visits = 0;
-                        status = kPrimary;
-                        fresh = true;
+                        status = sStartP;
                     }
                 }
             }
-        }
-        if (__ballot(status != kRetired) == 0) break;
-
-        // ---- start fresh rays (primary or shadow) -------------------------------
-        if (fresh) {
-            fresh = false;
-            const float p[3] = {ox, oy, oz};
-            Ray3 d;
-            d.x = dx;
-            d.y = dy;
-            d.z = dz;
-            const bool ok = recs != nullptr && setup_trace(p, d, c, flags) && min3(c.t2x, c.t2y, c.t2z) > 0;
-            if (ok) {
-                ++visits;
-                node = recs[0];
-                level = 0;
-                adv = false;
-                ch = find_first_child(c);
-            } else if (status == kPrimary) {
-                write_miss(a.out, pix, visits);
-                status = kNeed;
-            } else {
-                write_final(a.out, pix, color, 0.1f + 0.9f * lam, 0, visits);  // shadow ray leaves: lit
-                status = kNeed;
+            if (status == sStartP || status == sStartS) {
+                const float p[3] = {ox, oy, oz};
+                Ray3 d;
+                d.x = dx;
+                d.y = dy;
+                d.z = dz;
+                const bool ok =
+                    recs != nullptr && setup_trace(p, d, c, flags) && min3(c.t2x, c.t2y, c.t2z) > 0;
+                if (ok) {
+                    ++visits;
+                    node = recs[0];
+                    level = 0;
+                    adv = false;
+                    ch = find_first_child(c);
+                    status = status == sStartP ? sTraceP : sTraceS;
+                } else if (status == sStartP) {
+                    write_miss(a.out, pix, visits);
+                    status = sIdle;
+                } else {
+                    write_final(a.out, pix, color, 0.1f + 0.9f * lam, 0, visits);  // lit
+                    status = sIdle;
+                }
             }
         }
+        if (__ballot(status != sRetired) == 0) break;
+        if (status != sTraceP && status != sTraceS) continue;
 
-        if (status != kPrimary && status != kShadow) continue;
-
-        // ---- one step: test the current child ------------------------------------
+        // ---- one traversal step ---------------------------------------------
         if (!adv) {
             const int real = ch ^ (int)flags;
             const uint32_t cm = node.y & 0xffu, lm = (node.y >> 8) & 0xffu;
-            const uint32_t below = (cm | lm) & ((1u << real) - 1u);
             const float mt2 = min3(c.t2x, c.t2y, c.t2z);
             const bool is_leaf = (lm >> real) & 1u;
-            if (is_leaf && (status == kPrimary || mt2 > 0)) {
-                if (status == kShadow) {
-                    write_final(a.out, pix, color, 0.1f, 1, visits);  // occluded
-                    status = kNeed;
+            if (is_leaf && (status == sTraceP || mt2 > 0)) {
+                if (status == sTraceP) {
+                    status = sHitP;
                 } else {
-                    const uint2 leaf = recs[node.x + (uint32_t)__builtin_popcount(below)];
-                    int vx = 0, vy = 0, vz = 0;
-                    for (int j = 0; j < level; ++j) {
-                        const uint32_t s = bits3(chpath, j) ^ flags;
-                        vx = (vx << 1) | (int)(s & 1u);
-                        vy = (vy << 1) | (int)((s >> 1) & 1u);
-                        vz = (vz << 1) | (int)((s >> 2) & 1u);
-                    }
-                    vx = (vx << 1) | (real & 1);
-                    vy = (vy << 1) | ((real >> 1) & 1);
-                    vz = (vz << 1) | ((real >> 2) & 1);
-                    const float t = max3(c.t1x, c.t1y, c.t1z);
-                    if (a.out.hit_id) a.out.hit_id[pix] = leaf.y * 8u + (uint32_t)real;
-                    if (a.out.voxel) {
-                        a.out.voxel[3 * (size_t)pix] = vx;
-                        a.out.voxel[3 * (size_t)pix + 1] = vy;
-                        a.out.voxel[3 * (size_t)pix + 2] = vz;
-                    }
-                    if (a.out.t) a.out.t[pix] = t;
-                    const int e = entry_axis(c);
-                    const float de = e == 0 ? dx : (e == 1 ? dy : dz);
-                    if (a.light_mode == YX_LIGHT_HEADLIGHT) {
-                        lam = fabsf(de);
-                    } else {
-                        const float le = e == 0 ? a.light[0] : (e == 1 ? a.light[1] : a.light[2]);
-                        lam = dmax((de < 0) ? le : -le, 0.0f);
-                    }
-                    color = leaf.x;
-                    if (a.light_mode == YX_LIGHT_SUN_SHADOW && lam > 0.0f) {
-                        // hit point stepped half a cell back across the entry face
-                        const float half = __builtin_bit_cast(float, (uint32_t)(127 - (level + 2)) << 23);
-                        const float off = (de < 0) ? half : -half;
-                        float px = ox + dx * t, py = oy + dy * t, pz = oz + dz * t;
-                        ox = (e == 0) ? px + off : px;
-                        oy = (e == 1) ? py + off : py;
-                        oz = (e == 2) ? pz + off : pz;
-                        dx = adjust_coord(a.light[0]);
-                        dy = adjust_coord(a.light[1]);
-                        dz = adjust_coord(a.light[2]);
-                        status = kShadow;
-                        fresh = true;
-                    } else {
-                        write_final(a.out, pix, color, 0.1f + 0.9f * lam, 2, visits);
-                        status = kNeed;
-                    }
+                    occluded = true;
+                    status = sEndS;
                 }
                 continue;
             }
@@ -238,6 +270,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     lds_store(level, c, node);
                 }
                 chpath = set3(chpath, level, (uint32_t)ch);
+                const uint32_t below = (cm | lm) & ((1u << real) - 1u);
                 node = recs[node.x + (uint32_t)__builtin_popcount(below)];
                 ++level;
                 ch = find_first_child(c);
@@ -245,15 +278,15 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
             }
             adv = true;
         }
-        // ---- one step: next sibling, or pop one level ----------------------------
         if (go_next(ch, c)) {
             adv = false;
         } else if (level == 0) {
-            if (status == kPrimary)
-                write_miss(a.out, pix, visits);
-            else
-                write_final(a.out, pix, color, 0.1f + 0.9f * lam, 0, visits);
-            status = kNeed;
+            if (status == sTraceP) {
+                status = sMissP;
+            } else {
+                occluded = false;
+                status = sEndS;
+            }
         } else {
             --level;
             if (W > 0) {
