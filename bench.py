@@ -34,7 +34,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--workload", default="large_1080p", choices=["large_1080p", "plain_512"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-frames", type=int, default=4, help="frames in the CPU baseline sample")
+    ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU baseline sample")
     return ap.parse_args()
 
 
@@ -141,7 +141,7 @@ def main():
                    "mean_rays_per_frame": round(float(np.mean(rays_f)), 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "frame_kernel", "kernel_ms": round(avg_kernel_s * 1e3, 4),
+                     "kernel": "stack_kernel<64, 0>", "kernel_ms": round(avg_kernel_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": int(np.mean(bytes_f))},
         "cpu_baseline": cpu,
     }

@@ -23,8 +23,9 @@ class SVOData:
         self._h = ctypes.c_void_p()
 
     def __del__(self):
-        if getattr(self, "_h", None) and self._h.value:
-            lib().yx_scene_free(self._h)
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.yx_scene_free(h)
             self._h = ctypes.c_void_p()
 
     def _replace(self, h):
@@ -100,8 +101,9 @@ class SVORenderer:
         check(lib().yx_renderer_create(ctypes.byref(self._h)))
 
     def __del__(self):
-        if getattr(self, "_h", None) and self._h.value:
-            lib().yx_renderer_destroy(self._h)
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.yx_renderer_destroy(h)
             self._h = ctypes.c_void_p()
 
     def SetScene(self, svo):
