@@ -35,6 +35,9 @@ def parse():
     ap.add_argument("--workload", default="large_1080p", choices=["large_1080p", "plain_512"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU baseline sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="after the timed steps, copy 256 MiB device-to-device 3 times (a kernel of known "
+                         "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -118,6 +121,14 @@ def main():
     avg_kernel_s = float(np.mean(kern_ms)) / 1e3
     achieved = total_bytes / args.steps / avg_kernel_s / 1e9
 
+    if args.copy_calibration:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, cams, w, h, mode, wl.SUN, args.cpu_frames)
@@ -140,12 +151,24 @@ def main():
                    "nodes": info["num_nodes"], "leaves": info["num_leaves"],
                    "mean_rays_per_frame": round(float(np.mean(rays_f)), 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload),
                      "kernel": "stack_kernel<64, 0>", "kernel_ms": round(avg_kernel_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": int(np.mean(bytes_f))},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the frame kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (scaled by the copy-kernel calibration), or
+    None when no measurement for this workload is recorded."""
+    import pathlib
+    f = pathlib.Path(__file__).resolve().parent / "profiles" / f"traffic_{workload}.json"
+    if not f.exists():
+        return None
+    rec = json.loads(f.read_text())
+    return rec.get("traffic_bytes_per_launch")
 
 
 def cpu_baseline(scene, cams, w, h, mode, sun, nframes):
