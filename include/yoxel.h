@@ -167,6 +167,77 @@ typedef struct yx_tuning {
 int yx_renderer_set_tuning(yx_renderer* r, const yx_tuning* tuning);
 int yx_renderer_get_tuning(const yx_renderer* r, yx_tuning* tuning);
 
+/* ---- differentiable front-to-back compositing over brick voxels ----------
+ * Forward model: reaction/trace/trace.cu:42-139 (castRay / walkBrick: brick
+ * grid with occupancy table, samples on the lattice t = k*dt with dt = one
+ * voxel, front-to-back accumulation); the thesis' recurrence
+ * reaction/report/voxel.tex:59-87. Per sample: sigma = softplus(raw),
+ * alpha = 1 - exp(-sigma), C += T*alpha*rgb, T *= exp(-sigma); nearest-voxel
+ * lookup; no early exit (the loss stays smooth in the densities). */
+
+typedef struct yx_volume_desc {
+    int32_t grid;      /* voxels per axis, power of two (512)               */
+    int32_t brick;     /* voxels per brick edge, power of two (8)            */
+    float occupancy;   /* fraction of bricks allocated (0.1)                 */
+    int32_t octaves;   /* fBm octaves of the density field                   */
+    float frequency;   /* fBm base frequency over the unit cube              */
+    uint64_t seed;
+} yx_volume_desc;
+
+typedef struct yx_volume_info {
+    int32_t grid;
+    int32_t brick;
+    uint32_t num_slots; /* allocated bricks */
+} yx_volume_info;
+
+typedef struct yx_volume yx_volume; /* host arrays of a synthetic volume */
+
+int yx_volume_build_synthetic(const yx_volume_desc* desc, yx_volume** out);
+int yx_volume_get_info(const yx_volume* v, yx_volume_info* info);
+/* Borrowed host views: brick table int32[(grid/brick)^3] (slot or -1, x
+ * fastest) and params float[num_slots * brick^3 * 4] = {raw, r, g, b} per
+ * voxel, x fastest inside a brick. */
+int yx_volume_arrays(const yx_volume* v, const int32_t** brick_table, const float** params);
+void yx_volume_free(yx_volume* v);
+
+/* A camera as 12 floats: pos[3] then RendererBase::InitRayDir's dir0, du, dv
+ * (cell/renderer_base.h:50-61); pixel (x, y) looks along
+ * normalize(dir0 + du*x + dv*y). */
+int yx_camera_basis(const float pos[3], const float dir[3], const float up[3], float fov_deg, int32_t width,
+                    int32_t height, float view[12]);
+
+/* Device views of a volume (caller-owned memory). */
+typedef struct yx_volume_dev {
+    const int32_t* table;
+    const float* params; /* float4 per voxel slot */
+    int32_t grid;
+    int32_t brick;
+    uint32_t num_slots;
+} yx_volume_dev;
+
+typedef struct yx_compositor yx_compositor; /* holds the backward's workspace */
+int yx_compositor_create(yx_compositor** out);
+void yx_compositor_destroy(yx_compositor* c);
+
+/* Forward over n_views cameras (device float[12 * n_views]) of width x height
+ * rays each; outputs per ray, view-major then row-major: rgb float[3],
+ * transmittance float, samples in allocated bricks uint32 (any may be NULL
+ * except nsamp when a backward follows). Asynchronous on `stream`. */
+int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, const float* views, int32_t n_views,
+                         int32_t width, int32_t height, float* rgb, float* trans, uint32_t* nsamp,
+                         void* stream);
+/* Adjoint of the forward: given the forward's rgb and nsamp and dL/drgb,
+ * writes dL/dparams (float4 per voxel slot: d raw, d r, d g, d b). Recomputes
+ * the exact sample sequence, emits one (voxel key, gradient) record per
+ * sample, sorts the records by key and sums each key's records in sorted
+ * order: no atomics, deterministic. Synchronizes `stream` once to size its
+ * workspace. */
+int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol, const float* views, int32_t n_views,
+                          int32_t width, int32_t height, const float* rgb, const uint32_t* nsamp,
+                          const float* grad_rgb, float* grad_params, void* stream);
+/* Records emitted by the last backward (sum of nsamp) and distinct keys. */
+int yx_compositor_last_stats(const yx_compositor* c, uint64_t* records, uint64_t* unique_keys);
+
 const char* yx_last_error(void);
 const char* yx_version(void);
 

@@ -134,3 +134,122 @@ class Oracle:
 
     def plant(self):
         return self.lib.oracle_plant()
+
+
+class CompositeOracle:
+    """FP64 reference of the brick-voxel compositing and its gradient
+    (composite_oracle.c). Arrays: table int32 [G^3], params float32 [slots*B^3, 4],
+    views float32 [n, 12]."""
+
+    def __init__(self, path=None):
+        path = pathlib.Path(path) if path else LIB
+        if not path.exists():
+            build(out=path)
+        l = ctypes.CDLL(str(path))
+        vp = ctypes.c_void_p
+        l.comp_oracle_forward.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_int,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+        l.comp_oracle_forward.restype = None
+        l.comp_oracle_grad.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, vp,
+                                       vp, vp, ctypes.c_int]
+        l.comp_oracle_grad.restype = None
+        l.comp_oracle_loss.argtypes = [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_int, vp, ctypes.c_int, vp, ctypes.c_int64,
+                                       ctypes.c_int, ctypes.c_double]
+        l.comp_oracle_loss.restype = ctypes.c_double
+        l.comp_oracle_ray_voxels.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_uint32, vp, ctypes.c_int]
+        l.comp_oracle_ray_voxels.restype = ctypes.c_int
+        l.oracle_plant.restype = ctypes.c_int
+        self.lib = l
+
+    @staticmethod
+    def _prep(table, params, views):
+        t = np.ascontiguousarray(table, np.int32)
+        p = np.ascontiguousarray(params, np.float32).reshape(-1, 4)
+        v = np.ascontiguousarray(views, np.float32).reshape(-1, 12)
+        return t, p, v
+
+    def _common(self, t, p, v, grid, brick, w, h):
+        nslots = p.shape[0] // brick ** 3
+        return (t.ctypes.data, p.ctypes.data, grid, brick, nslots, v.ctypes.data, v.shape[0], w, h)
+
+    def forward(self, table, params, grid, brick, views, w, h, threads=1):
+        t, p, v = self._prep(table, params, views)
+        n = v.shape[0] * w * h
+        rgb = np.zeros((n, 3)), np.zeros(n), np.zeros(n, np.uint32)
+        rgb, trans, ns = rgb
+        c = self._common(t, p, v, grid, brick, w, h)
+        bands = _bands(n, threads)
+
+        def run(a, b):
+            self.lib.comp_oracle_forward(*c, a, b, rgb[a:].ctypes.data, trans[a:].ctypes.data, ns[a:].ctypes.data)
+        _run_threads(run, bands)
+        shp = (v.shape[0], h, w)
+        return rgb.reshape(shp + (3,)), trans.reshape(shp), ns.reshape(shp)
+
+    def grad(self, table, params, grid, brick, views, w, h, target, selected, max_rays=256, threads=1):
+        """Loss sum |C - target|^2 and its FP64 gradient at the `selected` voxel
+        slot indices (rows [d raw, d r, d g, d b]); also the rays that sample
+        each selected voxel."""
+        t, p, v = self._prep(table, params, views)
+        n = v.shape[0] * w * h
+        tgt = np.ascontiguousarray(target, np.float32).reshape(-1)
+        sel = np.asarray(selected, np.int64)
+        sel_map = np.full(p.shape[0], -1, np.int32)
+        sel_map[sel] = np.arange(sel.size, dtype=np.int32)
+        c = self._common(t, p, v, grid, brick, w, h)
+        bands = _bands(n, threads)
+        parts = [(np.zeros((sel.size, 4)), np.zeros(1), np.zeros((sel.size, max_rays), np.uint32),
+                  np.zeros(sel.size, np.uint32)) for _ in bands]
+
+        def run(a, b, k):
+            g, L, rl, rc = parts[k]
+            self.lib.comp_oracle_grad(*c, a, b, tgt.ctypes.data, sel_map.ctypes.data, g.ctypes.data, L.ctypes.data,
+                                      rl.ctypes.data, rc.ctypes.data, max_rays)
+        _run_threads(run, bands, with_index=True)
+        grad = sum(pp[0] for pp in parts)
+        loss = float(sum(pp[1][0] for pp in parts))
+        rays = []
+        for i in range(sel.size):
+            lst = []
+            for pp in parts:
+                lst.extend(pp[2][i, :pp[3][i]].tolist())
+            rays.append(np.array(sorted(set(lst)), np.uint32))
+        return loss, grad, rays
+
+    def loss_rays(self, table, params, grid, brick, views, w, h, target, rays, p_idx, p_comp, delta):
+        t, p, v = self._prep(table, params, views)
+        tgt = np.ascontiguousarray(target, np.float32).reshape(-1)
+        r = np.ascontiguousarray(rays, np.uint32)
+        c = self._common(t, p, v, grid, brick, w, h)
+        return self.lib.comp_oracle_loss(*c, r.ctypes.data, r.size, tgt.ctypes.data, int(p_idx), int(p_comp),
+                                         float(delta))
+
+    def ray_voxels(self, table, grid, brick, num_slots, views, w, h, ray, max_out=4096):
+        t = np.ascontiguousarray(table, np.int32)
+        v = np.ascontiguousarray(views, np.float32).reshape(-1, 12)
+        out = np.zeros(max_out, np.int64)
+        n = self.lib.comp_oracle_ray_voxels(t.ctypes.data, grid, brick, num_slots, v.ctypes.data, v.shape[0], w,
+                                            h, ray, out.ctypes.data, max_out)
+        return out[:n]
+
+    def plant(self):
+        return self.lib.oracle_plant()
+
+
+def _bands(n, threads):
+    step = -(-n // max(1, threads))
+    return [(a, min(n, a + step)) for a in range(0, n, step)]
+
+
+def _run_threads(fn, bands, with_index=False):
+    if len(bands) == 1:
+        fn(*bands[0], 0) if with_index else fn(*bands[0])
+        return
+    ts = [threading.Thread(target=fn, args=(a, b, k) if with_index else (a, b)) for k, (a, b) in enumerate(bands)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()

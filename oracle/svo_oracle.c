@@ -433,3 +433,11 @@ void oracle_pixel_dirs(const float* basis, int w, int h, float* out) {
 }
 
 int oracle_plant(void) { return ORACLE_PLANT; }
+
+/* One pixel direction (shared with the compositing restatement). */
+void oracle_pixel_dir(const float* basis, int x, int y, float* out) {
+    f3 d = pixel_dir(basis, x, y);
+    out[0] = d.x;
+    out[1] = d.y;
+    out[2] = d.z;
+}

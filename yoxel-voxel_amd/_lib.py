@@ -93,6 +93,31 @@ class Tuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class VolumeDesc(ctypes.Structure):
+    _fields_ = [
+        ("grid", ctypes.c_int32),
+        ("brick", ctypes.c_int32),
+        ("occupancy", ctypes.c_float),
+        ("octaves", ctypes.c_int32),
+        ("frequency", ctypes.c_float),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class VolumeInfo(ctypes.Structure):
+    _fields_ = [("grid", ctypes.c_int32), ("brick", ctypes.c_int32), ("num_slots", ctypes.c_uint32)]
+
+
+class VolumeDev(ctypes.Structure):
+    _fields_ = [
+        ("table", ctypes.c_void_p),
+        ("params", ctypes.c_void_p),
+        ("grid", ctypes.c_int32),
+        ("brick", ctypes.c_int32),
+        ("num_slots", ctypes.c_uint32),
+    ]
+
+
 YX_KERNEL_PER_PIXEL = 0
 YX_KERNEL_PERSISTENT = 1
 YX_KERNEL_REPLAY = 2
@@ -127,6 +152,19 @@ SIGNATURES = {
     "yx_renderer_scene_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "yx_renderer_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "yx_renderer_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "yx_volume_build_synthetic": (ctypes.c_int, [ctypes.POINTER(VolumeDesc), ctypes.POINTER(_P)]),
+    "yx_volume_get_info": (ctypes.c_int, [_P, ctypes.POINTER(VolumeInfo)]),
+    "yx_volume_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),
+    "yx_volume_free": (None, [_P]),
+    "yx_camera_basis": (ctypes.c_int, [_F3, _F3, _F3, ctypes.c_float, ctypes.c_int32, ctypes.c_int32, _F3]),
+    "yx_compositor_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "yx_compositor_destroy": (None, [_P]),
+    "yx_composite_forward": (ctypes.c_int, [_P, ctypes.POINTER(VolumeDev), _P, ctypes.c_int32, ctypes.c_int32,
+                                            ctypes.c_int32, _P, _P, _P, _P]),
+    "yx_composite_backward": (ctypes.c_int, [_P, ctypes.POINTER(VolumeDev), _P, ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32, _P, _P, _P, _P, _P]),
+    "yx_compositor_last_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64),
+                                                ctypes.POINTER(ctypes.c_uint64)]),
     "yx_last_error": (ctypes.c_char_p, []),
     "yx_version": (ctypes.c_char_p, []),
 }

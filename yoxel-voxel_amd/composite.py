@@ -1,0 +1,140 @@
+"""Differentiable front-to-back compositing over brick voxels.
+
+Host side of the yx_volume / yx_compositor C ABI (include/yoxel.h) and a
+torch.autograd.Function over it. The forward model is castRay / walkBrick of
+reaction/trace/trace.cu:42-139 (brick grid with occupancy table, samples on the
+lattice t = k * dt with dt one voxel, front-to-back accumulation), with
+sigma = softplus(raw) and per-voxel RGB as the learnable parameters.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, f3, lib
+
+# differentiable compositing workload: 512^3 grid as 8^3 bricks, ~10 % of the
+# bricks allocated from an fBm threshold, 16 views of 512x512 rays
+COMPOSITE_512 = dict(grid=512, brick=8, occupancy=0.1, octaves=5, frequency=4.0, seed=7)
+
+
+class Volume:
+    """Synthetic brick volume: host arrays (numpy) built by the library."""
+
+    def __init__(self, grid, brick, occupancy=0.1, octaves=5, frequency=4.0, seed=0):
+        d = _lib.VolumeDesc(grid, brick, occupancy, octaves, frequency, seed)
+        h = ctypes.c_void_p()
+        check(lib().yx_volume_build_synthetic(ctypes.byref(d), ctypes.byref(h)))
+        try:
+            info = _lib.VolumeInfo()
+            check(lib().yx_volume_get_info(h, ctypes.byref(info)))
+            t = ctypes.c_void_p()
+            p = ctypes.c_void_p()
+            check(lib().yx_volume_arrays(h, ctypes.byref(t), ctypes.byref(p)))
+            g = grid // brick
+            self.grid, self.brick, self.num_slots = info.grid, info.brick, info.num_slots
+            self.table = np.frombuffer((ctypes.c_int32 * (g ** 3)).from_address(t.value), np.int32).copy()
+            nvals = self.num_slots * brick ** 3 * 4
+            self.params = (np.frombuffer((ctypes.c_float * nvals).from_address(p.value), np.float32).copy()
+                           if nvals else np.zeros(0, np.float32)).reshape(-1, 4)
+        finally:
+            lib().yx_volume_free(h)
+
+    def to(self, device):
+        import torch
+        return (torch.from_numpy(self.table).to(device), torch.from_numpy(self.params).to(device))
+
+
+def camera_views(cams, fov, w, h):
+    """[(pos, dir)] -> float32 array [n, 12] (pos + InitRayDir basis)."""
+    out = np.zeros((len(cams), 12), np.float32)
+    for i, (pos, d) in enumerate(cams):
+        v = (ctypes.c_float * 12)()
+        check(lib().yx_camera_basis(f3(pos), f3(d), f3((0, 0, 1)), float(fov), int(w), int(h), v))
+        out[i] = np.array(v[:], np.float32)
+    return out
+
+
+def _vol_dev(table, params, grid, brick):
+    n_slots = params.shape[0] // (brick ** 3)
+    return _lib.VolumeDev(table.data_ptr(), params.data_ptr(), grid, brick, n_slots)
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Compositor:
+    """Owns the backward's device workspace."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        check(lib().yx_compositor_create(ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.yx_compositor_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    def forward(self, table, params, grid, brick, views, w, h):
+        """params: [slots * brick^3, 4] float32 CUDA; views: [n, 12] float32 CUDA.
+        Returns rgb [n, h, w, 3], transmittance [n, h, w], samples [n, h, w]."""
+        import torch
+        n = views.shape[0]
+        dev = params.device
+        rgb = torch.empty((n, h, w, 3), dtype=torch.float32, device=dev)
+        trans = torch.empty((n, h, w), dtype=torch.float32, device=dev)
+        nsamp = torch.empty((n, h, w), dtype=torch.int32, device=dev)
+        vol = _vol_dev(table, params, grid, brick)
+        check(lib().yx_composite_forward(self._h, ctypes.byref(vol), views.data_ptr(), n, w, h, rgb.data_ptr(),
+                                         trans.data_ptr(), nsamp.data_ptr(), _stream()))
+        return rgb, trans, nsamp
+
+    def backward(self, table, params, grid, brick, views, w, h, rgb, nsamp, grad_rgb):
+        import torch
+        n = views.shape[0]
+        grad = torch.empty_like(params)
+        vol = _vol_dev(table, params, grid, brick)
+        g = grad_rgb.contiguous()
+        check(lib().yx_composite_backward(self._h, ctypes.byref(vol), views.data_ptr(), n, w, h, rgb.data_ptr(),
+                                          nsamp.data_ptr(), g.data_ptr(), grad.data_ptr(), _stream()))
+        return grad
+
+    def last_stats(self):
+        r = ctypes.c_uint64()
+        u = ctypes.c_uint64()
+        check(lib().yx_compositor_last_stats(self._h, ctypes.byref(r), ctypes.byref(u)))
+        return r.value, u.value
+
+
+def composite_function():
+    """torch.autograd.Function: rgb = composite(params) for fixed views/table."""
+    import torch
+
+    class VolumeComposite(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, params, table, views, grid, brick, w, h, comp):
+            rgb, trans, nsamp = comp.forward(table, params, grid, brick, views, w, h)
+            ctx.save_for_backward(params, table, views, rgb, nsamp)
+            ctx.meta = (grid, brick, w, h, comp)
+            return rgb
+
+        @staticmethod
+        def backward(ctx, grad_rgb):
+            params, table, views, rgb, nsamp = ctx.saved_tensors
+            grid, brick, w, h, comp = ctx.meta
+            grad = comp.backward(table, params, grid, brick, views, w, h, rgb, nsamp, grad_rgb)
+            return grad, None, None, None, None, None, None, None
+
+    return VolumeComposite
+
+
+def orbit_views(n, fov, w, h, seed=0):
+    from .workloads import orbit_cameras
+    return camera_views(orbit_cameras(n, seed=seed), fov, w, h)
+
+
+__all__ = ["Volume", "Compositor", "composite_function", "camera_views", "orbit_views", "COMPOSITE_512"]

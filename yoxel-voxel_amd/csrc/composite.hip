@@ -1,0 +1,461 @@
+// Differentiable front-to-back compositing over brick voxels (gfx950).
+//
+// Forward model (reaction/trace/trace.cu:42-139 castRay / walkBrick, in the
+// form of the thesis recurrence reaction/report/voxel.tex:59-87):
+//   ray: pixel direction of RendererBase::InitRayDir (cell/renderer_base.h:
+//        50-61), components below 1e-5 clamped with their sign (PrepareRay,
+//        reaction/trace/trace.cu:21,31-33); unit-cube slab test with 1/dir
+//        (intersect, :66-80), t_enter clamped at 0 (:104).
+//   samples: t_k = k * dt with dt = one voxel (walkBrick's lattice
+//        ceil(t_enter / dt) * dt, :44-48), t_enter <= t_k < t_exit; voxel =
+//        floor(p * N) clamped to the grid; samples in unallocated bricks are
+//        empty and skipped (castRay's mark test, :129).
+//   per sample: sigma = softplus(raw), alpha = 1 - exp(-sigma),
+//        C += T * alpha * rgb, T *= exp(-sigma) (walkBrick's accum += col *
+//        (1 - accum.w), :58). No early exit.
+// Adjoint, per sample k with g = dL/dC, S = C_final - sum_{j<=k} w_j rgb_j:
+//   dL/drgb_k   = w_k g,   w_k = T_k alpha_k
+//   dL/dsigma_k = g . (T_{k+1} rgb_k - S)
+//   dL/draw_k   = dL/dsigma_k * sigmoid(raw_k)
+// The backward recomputes the exact discrete sample sequence instead of
+// storing per-ray lists, writes one (voxel key, float4 gradient) record per
+// sample at the ray's exclusive-scan offset, radix-sorts the records by key
+// and reduces each key's run in sorted order: no atomics, deterministic.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "internal.h"
+#include "trace_device.h"
+
+namespace yx {
+
+struct CompArgs {
+    const int32_t* table;
+    const float4* params;
+    const float* views;
+    uint32_t num_slots;
+    int n, lb, g;  // grid, log2 brick, bricks per axis
+    int n_views, w, h;
+    float* rgb;
+    float* trans;
+    uint32_t* nsamp;
+    // backward
+    const float* rgb_in;
+    const float* grad_rgb;
+    const uint32_t* offsets;
+    const uint32_t* nsamp_in;
+    uint32_t* keys;
+    float4* vals;
+};
+
+struct CompRay {
+    float ox, oy, oz, dx, dy, dz, rx, ry, rz;  // origin, direction, 1/direction
+    int k, kend;                               // sample range [k, kend)
+};
+
+__device__ __forceinline__ float clamp_eps(float x) {
+    const float eps = 1e-5f;
+    return fabsf(x) < eps ? copysignf(eps, x) : x;
+}
+
+__device__ __forceinline__ bool comp_setup(const CompArgs& a, int view, int x, int y, CompRay& r) {
+    const float* v = a.views + 12 * view;
+    const Ray3 d = pixel_dir(v + 3, x, y);
+    r.ox = v[0];
+    r.oy = v[1];
+    r.oz = v[2];
+    r.dx = clamp_eps(d.x);
+    r.dy = clamp_eps(d.y);
+    r.dz = clamp_eps(d.z);
+    r.rx = 1.0f / r.dx;
+    r.ry = 1.0f / r.dy;
+    r.rz = 1.0f / r.dz;
+    const float lx = r.rx * (0.0f - r.ox), hx = r.rx * (1.0f - r.ox);
+    const float ly = r.ry * (0.0f - r.oy), hy = r.ry * (1.0f - r.oy);
+    const float lz = r.rz * (0.0f - r.oz), hz = r.rz * (1.0f - r.oz);
+    float te = fmaxf(fminf(lx, hx), fmaxf(fminf(ly, hy), fminf(lz, hz)));
+    const float tx = fminf(fmaxf(lx, hx), fminf(fmaxf(ly, hy), fmaxf(lz, hz)));
+    te = fmaxf(te, 0.0f);
+    if (tx < 0.0f || te > tx) return false;
+    const float nf = (float)a.n;
+    r.k = (int)ceilf(te * nf);
+    r.kend = (int)ceilf(tx * nf);
+    return r.k < r.kend;
+}
+
+__device__ __forceinline__ int vox(float o, float d, float t, float nf, int n) {
+    const float p = o + t * d;
+    int v = (int)floorf(p * nf);
+    return min(max(v, 0), n - 1);
+}
+
+// Walks the samples of one ray that fall in allocated bricks, in order,
+// calling f(param index, float4 params) for each.
+template <class F>
+__device__ __forceinline__ void comp_walk(const CompArgs& a, const CompRay& r, F&& f) {
+    const float nf = (float)a.n, inv_n = 1.0f / nf;
+    const int bmask = (1 << a.lb) - 1;
+    const float bsz = (float)(1 << a.lb) * inv_n;
+    int k = r.k;
+    while (k < r.kend) {
+        const float t = (float)k * inv_n;
+        const int vx = vox(r.ox, r.dx, t, nf, a.n), vy = vox(r.oy, r.dy, t, nf, a.n), vz = vox(r.oz, r.dz, t, nf, a.n);
+        const int bx = vx >> a.lb, by = vy >> a.lb, bz = vz >> a.lb;
+        const int slot = a.table[((size_t)bz * a.g + by) * a.g + bx];
+        if ((uint32_t)slot >= a.num_slots) {  // -1 (and anything out of range) is empty
+            // jump to the last sample still in this brick (exact: the
+            // candidate is checked and moved back while it left the brick)
+            const float ex = ((r.dx > 0.0f ? (float)(bx + 1) : (float)bx) * bsz - r.ox) * r.rx;
+            const float ey = ((r.dy > 0.0f ? (float)(by + 1) : (float)by) * bsz - r.oy) * r.ry;
+            const float ez = ((r.dz > 0.0f ? (float)(bz + 1) : (float)bz) * bsz - r.oz) * r.rz;
+            const float tb = fminf(ex, fminf(ey, ez));
+            int kk = (int)ceilf(tb * nf) - 1;
+            kk = min(max(kk, k), r.kend - 1);
+            while (kk > k) {
+                const float t2 = (float)kk * inv_n;
+                const int wx = vox(r.ox, r.dx, t2, nf, a.n) >> a.lb, wy = vox(r.oy, r.dy, t2, nf, a.n) >> a.lb,
+                          wz = vox(r.oz, r.dz, t2, nf, a.n) >> a.lb;
+                if (wx == bx && wy == by && wz == bz) break;
+                --kk;
+            }
+            k = kk + 1;
+            continue;
+        }
+        const uint32_t local = (uint32_t)((vx & bmask) + ((vy & bmask) << a.lb) + ((vz & bmask) << (2 * a.lb)));
+        const uint32_t idx = ((uint32_t)slot << (3 * a.lb)) + local;
+        f(idx, a.params[idx]);
+        ++k;
+    }
+}
+
+__device__ __forceinline__ float softplus(float x) { return x > 15.0f ? x : log1pf(expf(x)); }
+
+__global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int view = blockIdx.z;
+    if (x >= a.w || y >= a.h) return;
+    const size_t ray = ((size_t)view * a.h + y) * a.w + x;
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
+    uint32_t n = 0;
+    CompRay r;
+    if (comp_setup(a, view, x, y, r)) {
+        comp_walk(a, r, [&](uint32_t, float4 q) {
+            const float sig = softplus(q.x);
+            const float e = expf(-sig);
+            const float al = -expm1f(-sig);
+            const float wgt = T * al;
+            cr += wgt * q.y;
+            cg += wgt * q.z;
+            cb += wgt * q.w;
+            T = T * e;
+            ++n;
+        });
+    }
+    if (a.rgb) {
+        a.rgb[3 * ray] = cr;
+        a.rgb[3 * ray + 1] = cg;
+        a.rgb[3 * ray + 2] = cb;
+    }
+    if (a.trans) a.trans[ray] = T;
+    if (a.nsamp) a.nsamp[ray] = n;
+}
+
+__global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    const int view = blockIdx.z;
+    if (x >= a.w || y >= a.h) return;
+    const size_t ray = ((size_t)view * a.h + y) * a.w + x;
+    CompRay r;
+    if (!comp_setup(a, view, x, y, r)) return;
+    const float gr = a.grad_rgb[3 * ray], gg = a.grad_rgb[3 * ray + 1], gb = a.grad_rgb[3 * ray + 2];
+    const float fr = a.rgb_in[3 * ray], fg = a.rgb_in[3 * ray + 1], fb = a.rgb_in[3 * ray + 2];
+    float pr = 0.0f, pg = 0.0f, pb = 0.0f, T = 1.0f;
+    uint32_t out = a.offsets[ray];
+    const uint32_t end = out + a.nsamp_in[ray];  // never write past this ray's records
+    comp_walk(a, r, [&](uint32_t idx, float4 q) {
+        if (out >= end) return;
+        const float sig = softplus(q.x);
+        const float e = expf(-sig);
+        const float al = -expm1f(-sig);
+        const float wgt = T * al;
+        pr += wgt * q.y;
+        pg += wgt * q.z;
+        pb += wgt * q.w;
+        const float tn = T * e;
+        const float sr = fr - pr, sg = fg - pg, sb = fb - pb;
+        const float dsig = gr * (tn * q.y - sr) + gg * (tn * q.z - sg) + gb * (tn * q.w - sb);
+        const float sgm = 1.0f / (1.0f + expf(-q.x));
+        a.keys[out] = idx;
+        a.vals[out] = make_float4(dsig * sgm, wgt * gr, wgt * gg, wgt * gb);
+        ++out;
+        T = tn;
+    });
+}
+
+struct Float4Sum {
+    __device__ __forceinline__ float4 operator()(const float4& p, const float4& q) const {
+        return make_float4(p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w);
+    }
+};
+
+__global__ void comp_zero_kernel(float4* g, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) g[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+__global__ void comp_scatter_kernel(const uint32_t* keys, const float4* sums, const uint32_t* num_runs,
+                                    float4* grad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < *num_runs) grad[keys[i]] = sums[i];
+}
+
+}  // namespace yx
+
+using namespace yx;
+
+#define YX_HIP(call)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) return fail(YX_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+struct yx_compositor {
+    void* ws = nullptr;  // one device allocation, carved into the buffers below
+    size_t ws_bytes = 0;
+    uint64_t last_records = 0, last_unique = 0;
+    const uint32_t* d_runs = nullptr;  // distinct keys of the last backward (device)
+};
+
+static int check_vol(const yx_volume_dev* v, int32_t n_views, int32_t w, int32_t h) {
+    if (!v || !v->table || (!v->params && v->num_slots)) return fail(YX_ERR_INVALID_ARG, "null volume pointer");
+    auto pow2 = [](int x) { return x > 0 && (x & (x - 1)) == 0; };
+    if (!pow2(v->grid) || !pow2(v->brick) || v->brick > v->grid || v->grid > 4096)
+        return fail(YX_ERR_INVALID_ARG, "grid %d and brick %d must be powers of two, brick <= grid <= 4096", v->grid,
+                    v->brick);
+    if ((uint64_t)v->num_slots * (uint64_t)v->brick * v->brick * v->brick > 0xFFFFFFFFull)
+        return fail(YX_ERR_RANGE, "more than 2^32 voxel slots");
+    if (n_views < 1 || n_views > 65535 || w < 1 || h < 1 || w > 16384 || h > 16384)
+        return fail(YX_ERR_INVALID_ARG, "views %d of %dx%d", n_views, w, h);
+    return YX_OK;
+}
+
+static CompArgs make_args(const yx_volume_dev* v, const float* views, int32_t n_views, int32_t w, int32_t h) {
+    CompArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.table = v->table;
+    a.params = (const float4*)v->params;
+    a.num_slots = v->num_slots;
+    a.views = views;
+    a.n = v->grid;
+    a.lb = __builtin_ctz((unsigned)v->brick);
+    a.g = v->grid / v->brick;
+    a.n_views = n_views;
+    a.w = w;
+    a.h = h;
+    return a;
+}
+
+extern "C" int yx_camera_basis(const float pos[3], const float dir[3], const float up[3], float fov, int32_t w,
+                               int32_t h, float view[12]) {
+    clear_error();
+    if (!pos || !dir || !up || !view) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (w <= 0 || h <= 0 || !(fov > 0.0f && fov < 180.0f)) return fail(YX_ERR_INVALID_ARG, "bad camera");
+    auto norm = [](float* v) {
+        float n2 = 0.0f;
+        n2 += v[0] * v[0];
+        n2 += v[1] * v[1];
+        n2 += v[2] * v[2];
+        const float n = std::sqrt(n2);
+        v[0] = v[0] / n;
+        v[1] = v[1] / n;
+        v[2] = v[2] / n;
+    };
+    float fwd[3] = {dir[0], dir[1], dir[2]}, right[3], upv[3];
+    norm(fwd);
+    right[0] = fwd[1] * up[2] - fwd[2] * up[1];
+    right[1] = fwd[2] * up[0] - fwd[0] * up[2];
+    right[2] = fwd[0] * up[1] - fwd[1] * up[0];
+    norm(right);
+    upv[0] = right[1] * fwd[2] - right[2] * fwd[1];
+    upv[1] = right[2] * fwd[0] - right[0] * fwd[2];
+    upv[2] = right[0] * fwd[1] - right[1] * fwd[0];
+    const float deg2rad = (float)(3.14159265358979323846 / (double)180.0f);
+    const float da = tanf((fov / 2.0f) * deg2rad) / (float)w;
+    for (int i = 0; i < 3; ++i) {
+        const float du = (right[i] * 2.0f) * da, dv = (upv[i] * -2.0f) * da;
+        view[i] = pos[i];
+        view[6 + i] = du;
+        view[9 + i] = dv;
+        view[3 + i] = (fwd[i] - (du * (float)w) / 2.0f) - (dv * (float)h) / 2.0f;
+    }
+    return YX_OK;
+}
+
+extern "C" int yx_compositor_create(yx_compositor** out) {
+    clear_error();
+    if (!out) return fail(YX_ERR_INVALID_ARG, "null argument");
+    yx_compositor* c = new (std::nothrow) yx_compositor();
+    if (!c) return fail(YX_ERR_OOM, "out of host memory");
+    *out = c;
+    return YX_OK;
+}
+
+extern "C" void yx_compositor_destroy(yx_compositor* c) {
+    if (!c) return;
+    if (c->ws) (void)hipFree(c->ws);
+    delete c;
+}
+
+extern "C" int yx_compositor_last_stats(const yx_compositor* c, uint64_t* records, uint64_t* unique_keys) {
+    clear_error();
+    if (!c || !records || !unique_keys) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *records = c->last_records;
+    if (c->d_runs) {
+        uint32_t runs = 0;
+        YX_HIP(hipMemcpy(&runs, c->d_runs, 4, hipMemcpyDeviceToHost));
+        *unique_keys = runs;
+    } else {
+        *unique_keys = c->last_unique;
+    }
+    return YX_OK;
+}
+
+extern "C" int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, const float* views, int32_t n_views,
+                                    int32_t w, int32_t h, float* rgb, float* trans, uint32_t* nsamp, void* stream) {
+    clear_error();
+    if (!c || !views) return fail(YX_ERR_INVALID_ARG, "null argument");
+    int rc = check_vol(vol, n_views, w, h);
+    if (rc != YX_OK) return rc;
+    CompArgs a = make_args(vol, views, n_views, w, h);
+    a.rgb = rgb;
+    a.trans = trans;
+    a.nsamp = nsamp;
+    dim3 grid((w + 15) / 16, (h + 15) / 16, n_views);
+    hipLaunchKernelGGL(comp_forward_kernel, grid, dim3(256), 0, (hipStream_t)stream, a);
+    YX_HIP(hipGetLastError());
+    return YX_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol, const float* views,
+                                     int32_t n_views, int32_t w, int32_t h, const float* rgb, const uint32_t* nsamp,
+                                     const float* grad_rgb, float* grad_params, void* stream) {
+    clear_error();
+    if (!c || !views || !rgb || !nsamp || !grad_rgb || !grad_params)
+        return fail(YX_ERR_INVALID_ARG, "null argument");
+    int rc = check_vol(vol, n_views, w, h);
+    if (rc != YX_OK) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const size_t rays = (size_t)n_views * w * h;
+    const size_t nvox = (size_t)vol->num_slots * vol->brick * vol->brick * vol->brick;
+    if (rays >= 0xFFFFFFFFull) return fail(YX_ERR_RANGE, "too many rays");
+
+    // 1. exclusive scan of the per-ray sample counts -> record offsets, total
+    size_t scan_tmp = 0;
+    YX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, nsamp, (uint32_t*)nullptr, (int)rays, st));
+    const size_t off_bytes = align256((rays + 1) * sizeof(uint32_t));
+    const size_t need0 = off_bytes + align256(scan_tmp) + 256;
+    if (c->ws_bytes < need0) {
+        if (c->ws) YX_HIP(hipFree(c->ws));
+        c->ws = nullptr;
+        c->ws_bytes = 0;
+        hipError_t e = hipMalloc(&c->ws, need0);
+        if (e != hipSuccess) return fail(YX_ERR_OOM, "hipMalloc(%zu) workspace: %s", need0, hipGetErrorString(e));
+        c->ws_bytes = need0;
+    }
+    uint32_t* offsets = (uint32_t*)c->ws;
+    YX_HIP(hipcub::DeviceScan::ExclusiveSum((char*)c->ws + off_bytes, scan_tmp, nsamp, offsets, (int)rays, st));
+    uint32_t last[2];
+    YX_HIP(hipMemcpyAsync(&last[0], offsets + rays - 1, 4, hipMemcpyDeviceToHost, st));
+    YX_HIP(hipMemcpyAsync(&last[1], nsamp + rays - 1, 4, hipMemcpyDeviceToHost, st));
+    YX_HIP(hipStreamSynchronize(st));
+    const uint64_t total = (uint64_t)last[0] + last[1];
+    if (total >= 0x7FFFFFFFull) return fail(YX_ERR_RANGE, "%llu records exceed 2^31", (unsigned long long)total);
+    c->last_records = total;
+
+    // zero the gradient (voxels without samples keep 0)
+    {
+        const size_t n4 = nvox;
+        if (n4) hipLaunchKernelGGL(comp_zero_kernel, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st,
+                                   (float4*)grad_params, n4);
+        YX_HIP(hipGetLastError());
+    }
+    c->d_runs = nullptr;
+    if (total == 0) {
+        c->last_unique = 0;
+        return YX_OK;
+    }
+
+    // 2. workspace: keys, vals, sorted keys, sorted vals, unique keys, sums,
+    //    run count, library temporaries
+    int key_bits = 1;
+    while (key_bits < 32 && ((uint64_t)1 << key_bits) < (uint64_t)nvox) ++key_bits;
+    size_t sort_tmp = 0, red_tmp = 0;
+    YX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (float4*)nullptr, (float4*)nullptr, (int)total, 0, key_bits, st));
+    YX_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, red_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (float4*)nullptr, (float4*)nullptr, (uint32_t*)nullptr, Float4Sum(),
+                                             (int)total, st));
+    const size_t kb = align256(total * 4), vb = align256(total * 16);
+    const size_t need = off_bytes + 2 * kb + 2 * vb + 256 + align256(std::max(std::max(sort_tmp, red_tmp), scan_tmp));
+    if (c->ws_bytes < need) {
+        // keep the offsets: copy them aside is unnecessary, they are recomputed
+        if (c->ws) YX_HIP(hipFree(c->ws));
+        c->ws = nullptr;
+        c->ws_bytes = 0;
+        const size_t grow = need + need / 4;
+        hipError_t e = hipMalloc(&c->ws, grow);
+        if (e != hipSuccess) return fail(YX_ERR_OOM, "hipMalloc(%zu) workspace: %s", grow, hipGetErrorString(e));
+        c->ws_bytes = grow;
+        offsets = (uint32_t*)c->ws;
+        YX_HIP(hipcub::DeviceScan::ExclusiveSum((char*)c->ws + off_bytes, scan_tmp, nsamp, offsets, (int)rays, st));
+    }
+    char* p = (char*)c->ws + off_bytes;
+    uint32_t* keys = (uint32_t*)p;
+    p += kb;
+    uint32_t* keys_sorted = (uint32_t*)p;
+    p += kb;
+    float4* vals = (float4*)p;
+    p += vb;
+    float4* vals_sorted = (float4*)p;
+    p += vb;
+    uint32_t* num_runs = (uint32_t*)p;
+    p += 256;
+    void* tmp = p;
+    // unique keys and run sums reuse the unsorted record arrays
+    uint32_t* uniq = keys;
+    float4* sums = vals;
+
+    // 3. emit records
+    CompArgs a = make_args(vol, views, n_views, w, h);
+    a.rgb_in = rgb;
+    a.grad_rgb = grad_rgb;
+    a.offsets = offsets;
+    a.nsamp_in = nsamp;
+    a.keys = keys;
+    a.vals = vals;
+    dim3 grid((w + 15) / 16, (h + 15) / 16, n_views);
+    hipLaunchKernelGGL(comp_backward_kernel, grid, dim3(256), 0, st, a);
+    YX_HIP(hipGetLastError());
+
+    // 4. sort by voxel key (LSD radix, stable), 5. reduce each run in order
+    YX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys_sorted, vals, vals_sorted, (int)total, 0,
+                                              key_bits, st));
+    YX_HIP(hipcub::DeviceReduce::ReduceByKey(tmp, red_tmp, keys_sorted, uniq, vals_sorted, sums, num_runs,
+                                             Float4Sum(), (int)total, st));
+    // 6. scatter: one write per distinct voxel
+    const uint32_t max_runs = (uint32_t)std::min<uint64_t>(total, nvox);
+    hipLaunchKernelGGL(comp_scatter_kernel, dim3((max_runs + 255) / 256), dim3(256), 0, st, uniq, sums, num_runs,
+                       (float4*)grad_params);
+    YX_HIP(hipGetLastError());
+    c->d_runs = num_runs;
+    return YX_OK;
+}
