@@ -35,6 +35,8 @@ def parse():
     ap.add_argument("--workload", default="large_1080p", choices=["large_1080p", "plain_512"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU baseline sample")
+    ap.add_argument("--no-composite", action="store_true", help="skip the fwd+bwd compositing measurement")
+    ap.add_argument("--composite-steps", type=int, default=8)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy 256 MiB device-to-device 3 times (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -132,6 +134,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, cams, w, h, mode, wl.SUN, args.cpu_frames)
+    fwd_bwd = None if args.no_composite else composite_bench(yx, torch, dev, args.composite_steps)
 
     line = {
         "metric": BASELINE_METRIC,
@@ -155,8 +158,69 @@ def main():
                      "kernel": "stack_kernel<64, 0>", "kernel_ms": round(avg_kernel_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": int(np.mean(bytes_f))},
         "cpu_baseline": cpu,
+        "fwd_bwd": fwd_bwd,
     }
     print(json.dumps(line), flush=True)
+
+
+def composite_bench(yx, torch, dev, steps):
+    """fwd+bwd of the differentiable compositing workload composite_512: 512^3
+    grid as 8^3 bricks (~10 % allocated), 16 views of 512x512 rays, L2 loss
+    against targets rendered with raw densities perturbed by N(0, 0.1). One
+    step = forward, dL/dC, adjoint with the sorted-key gradient reduction.
+    value = rays per second (each ray gets one forward and one backward pass)."""
+    comp_mod = yx.composite
+    cfg = comp_mod.COMPOSITE_512
+    grid, brick, w, h, nv = cfg["grid"], cfg["brick"], 512, 512, 16
+    vol = comp_mod.Volume(**cfg)
+    table, params = vol.to(dev)
+    views = torch.from_numpy(comp_mod.orbit_views(nv, 60.0, w, h, seed=0)).to(dev)
+    rng = np.random.default_rng(1)
+    p2 = vol.params.copy()
+    p2[:, 0] += rng.normal(0.0, 0.1, size=p2.shape[0]).astype(np.float32)
+    comp = comp_mod.Compositor()
+    target, _, _ = comp.forward(table, torch.from_numpy(p2).to(dev), grid, brick, views, w, h)
+    stream = torch.cuda.current_stream()
+
+    def step():
+        rgb, trans, ns = comp.forward(table, params, grid, brick, views, w, h)
+        g = 2.0 * (rgb - target)
+        return comp.backward(table, params, grid, brick, views, w, h, rgb, ns, g), ns
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    ev = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a, b, c = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        a.record(stream)
+        rgb, trans, ns = comp.forward(table, params, grid, brick, views, w, h)
+        b.record(stream)
+        g = 2.0 * (rgb - target)
+        comp.backward(table, params, grid, brick, views, w, h, rgb, ns, g)
+        c.record(stream)
+        ev.append((a, b, c))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    fwd_ms = float(np.mean([a.elapsed_time(b) for a, b, _ in ev]))
+    bwd_ms = float(np.mean([b.elapsed_time(c) for _, b, c in ev]))
+    records, unique = comp.last_stats()
+    rays = nv * w * h
+    nvox = params.shape[0]
+    # algorithmic bytes: forward 16 B per sample + 20 B per ray out; backward
+    # 16 B per sample recomputed, 36 B per ray in (dL/dC, C, offset, count), a
+    # 20 B record per sample written, 3 radix passes reading and writing the
+    # 20 B records, the reduce (20 B in per record, 20 B out per key), the
+    # scatter (16 B per key) and the zeroed gradient (16 B per voxel)
+    fwd_bytes = 16 * records + 20 * rays
+    bwd_bytes = 16 * records + 36 * rays + 20 * records + 3 * 40 * records + 20 * records + 36 * unique + 16 * nvox
+    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "workload": "composite_512",
+            "rays_per_step": rays, "ms_per_step": round(dt * 1e3, 3), "fwd_ms": round(fwd_ms, 3),
+            "bwd_ms": round(bwd_ms, 3), "records": records, "unique_voxels": unique,
+            "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                         "fwd_achieved": round(fwd_bytes / (fwd_ms / 1e3) / 1e9, 1),
+                         "bwd_achieved": round(bwd_bytes / (bwd_ms / 1e3) / 1e9, 1)}}
 
 
 def measured_traffic(workload):

@@ -113,13 +113,14 @@ def test_autograd_function(yx, cuda):
     rgb2, _, ns = comp.forward(table, params, grid, brick, vd, w, h)
     g = comp.backward(table, params, grid, brick, vd, w, h, rgb2, ns, 2.0 * (rgb2 - t_dev))
     assert torch.equal(p.grad, g)
-    # a few descent steps reduce the loss
-    opt = torch.optim.Adam([p], lr=0.05)
+    # plain gradient descent with a step scaled to the first gradient lowers the loss
+    lr = 0.02 / float(p.grad.abs().max())
     losses = []
     for _ in range(15):
-        opt.zero_grad()
+        p.grad = None
         l = ((fn.apply(p, table, vd, grid, brick, w, h, comp) - t_dev) ** 2).sum()
         l.backward()
-        opt.step()
-        losses.append(float(l))
-    assert losses[-1] < 0.5 * losses[0]
+        with torch.no_grad():
+            p -= lr * p.grad
+        losses.append(float(l.detach()))
+    assert all(b <= a for a, b in zip(losses, losses[1:])) and losses[-1] < 0.9 * losses[0]

@@ -49,9 +49,14 @@ struct CompArgs {
     const float* grad_rgb;
     const uint32_t* offsets;
     const uint32_t* nsamp_in;
+    uint32_t total_records;
     uint32_t* keys;
     float4* vals;
 };
+
+__device__ __forceinline__ uint32_t lanes_rank(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
 
 struct CompRay {
     float ox, oy, oz, dx, dy, dz, rx, ry, rz;  // origin, direction, 1/direction
@@ -94,68 +99,90 @@ __device__ __forceinline__ int vox(float o, float d, float t, float nf, int n) {
     return min(max(v, 0), n - 1);
 }
 
-// Walks the samples of one ray that fall in allocated bricks, in order,
-// calling f(param index, float4 params) for each.
-template <class F>
-__device__ __forceinline__ void comp_walk(const CompArgs& a, const CompRay& r, F&& f) {
-    const float nf = (float)a.n, inv_n = 1.0f / nf;
-    const int bmask = (1 << a.lb) - 1;
-    const float bsz = (float)(1 << a.lb) * inv_n;
-    int k = r.k;
-    while (k < r.kend) {
-        const float t = (float)k * inv_n;
-        const int vx = vox(r.ox, r.dx, t, nf, a.n), vy = vox(r.oy, r.dy, t, nf, a.n), vz = vox(r.oz, r.dz, t, nf, a.n);
-        const int bx = vx >> a.lb, by = vy >> a.lb, bz = vz >> a.lb;
-        const int slot = a.table[((size_t)bz * a.g + by) * a.g + bx];
-        if ((uint32_t)slot >= a.num_slots) {  // -1 (and anything out of range) is empty
-            // jump to the last sample still in this brick (exact: the
-            // candidate is checked and moved back while it left the brick)
-            const float ex = ((r.dx > 0.0f ? (float)(bx + 1) : (float)bx) * bsz - r.ox) * r.rx;
-            const float ey = ((r.dy > 0.0f ? (float)(by + 1) : (float)by) * bsz - r.oy) * r.ry;
-            const float ez = ((r.dz > 0.0f ? (float)(bz + 1) : (float)bz) * bsz - r.oz) * r.rz;
-            const float tb = fminf(ex, fminf(ey, ez));
-            int kk = (int)ceilf(tb * nf) - 1;
-            kk = min(max(kk, k), r.kend - 1);
-            while (kk > k) {
-                const float t2 = (float)kk * inv_n;
-                const int wx = vox(r.ox, r.dx, t2, nf, a.n) >> a.lb, wy = vox(r.oy, r.dy, t2, nf, a.n) >> a.lb,
-                          wz = vox(r.oz, r.dz, t2, nf, a.n) >> a.lb;
-                if (wx == bx && wy == by && wz == bz) break;
-                --kk;
+// Iterator over the samples of one ray that fall in allocated bricks, in
+// order: next() returns the parameter index of the next such sample (and its
+// float4 parameters) or -1 when the ray is done.
+struct SampleIter {
+    CompRay r;
+    int k;
+    bool live;
+
+    __device__ __forceinline__ int64_t next(const CompArgs& a, float4& q) {
+        if (!live) return -1;
+        const float nf = (float)a.n, inv_n = 1.0f / nf;
+        const int bmask = (1 << a.lb) - 1;
+        const float bsz = (float)(1 << a.lb) * inv_n;
+        while (k < r.kend) {
+            const float t = (float)k * inv_n;
+            const int vx = vox(r.ox, r.dx, t, nf, a.n), vy = vox(r.oy, r.dy, t, nf, a.n),
+                      vz = vox(r.oz, r.dz, t, nf, a.n);
+            const int bx = vx >> a.lb, by = vy >> a.lb, bz = vz >> a.lb;
+            const int slot = a.table[((size_t)bz * a.g + by) * a.g + bx];
+            if ((uint32_t)slot >= a.num_slots) {  // -1 (and anything out of range) is empty
+                // jump to the last sample still in this brick (exact: the
+                // candidate is checked and moved back while it left the brick)
+                const float ex = ((r.dx > 0.0f ? (float)(bx + 1) : (float)bx) * bsz - r.ox) * r.rx;
+                const float ey = ((r.dy > 0.0f ? (float)(by + 1) : (float)by) * bsz - r.oy) * r.ry;
+                const float ez = ((r.dz > 0.0f ? (float)(bz + 1) : (float)bz) * bsz - r.oz) * r.rz;
+                const float tb = fminf(ex, fminf(ey, ez));
+                int kk = (int)ceilf(tb * nf) - 1;
+                kk = min(max(kk, k), r.kend - 1);
+                while (kk > k) {
+                    const float t2 = (float)kk * inv_n;
+                    const int wx = vox(r.ox, r.dx, t2, nf, a.n) >> a.lb, wy = vox(r.oy, r.dy, t2, nf, a.n) >> a.lb,
+                              wz = vox(r.oz, r.dz, t2, nf, a.n) >> a.lb;
+                    if (wx == bx && wy == by && wz == bz) break;
+                    --kk;
+                }
+                k = kk + 1;
+                continue;
             }
-            k = kk + 1;
-            continue;
+            const uint32_t local =
+                (uint32_t)((vx & bmask) + ((vy & bmask) << a.lb) + ((vz & bmask) << (2 * a.lb)));
+            const uint32_t idx = ((uint32_t)slot << (3 * a.lb)) + local;
+            q = a.params[idx];
+            ++k;
+            return idx;
         }
-        const uint32_t local = (uint32_t)((vx & bmask) + ((vy & bmask) << a.lb) + ((vz & bmask) << (2 * a.lb)));
-        const uint32_t idx = ((uint32_t)slot << (3 * a.lb)) + local;
-        f(idx, a.params[idx]);
-        ++k;
+        live = false;
+        return -1;
     }
-}
+};
 
 __device__ __forceinline__ float softplus(float x) { return x > 15.0f ? x : log1pf(expf(x)); }
 
+// Rays are numbered view-major, then row-major; thread i of the launch takes
+// ray i, so a wave holds 64 consecutive rays of a row.
+__device__ __forceinline__ bool ray_of(const CompArgs& a, size_t ray, int& view, int& x, int& y) {
+    const size_t per_view = (size_t)a.w * a.h;
+    if (ray >= per_view * (size_t)a.n_views) return false;
+    view = (int)(ray / per_view);
+    const size_t rem = ray - (size_t)view * per_view;
+    y = (int)(rem / (size_t)a.w);
+    x = (int)(rem - (size_t)y * a.w);
+    return true;
+}
+
 __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int view = blockIdx.z;
-    if (x >= a.w || y >= a.h) return;
-    const size_t ray = ((size_t)view * a.h + y) * a.w + x;
+    const size_t ray = (size_t)blockIdx.x * 256 + threadIdx.x;
+    int view, x, y;
+    if (!ray_of(a, ray, view, x, y)) return;
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
     uint32_t n = 0;
-    CompRay r;
-    if (comp_setup(a, view, x, y, r)) {
-        comp_walk(a, r, [&](uint32_t, float4 q) {
-            const float sig = softplus(q.x);
-            const float e = expf(-sig);
-            const float al = -expm1f(-sig);
-            const float wgt = T * al;
-            cr += wgt * q.y;
-            cg += wgt * q.z;
-            cb += wgt * q.w;
-            T = T * e;
-            ++n;
-        });
+    SampleIter it;
+    it.live = comp_setup(a, view, x, y, it.r);
+    it.k = it.r.k;
+    float4 q;
+    while (it.next(a, q) >= 0) {
+        const float sig = softplus(q.x);
+        const float e = expf(-sig);
+        const float al = -expm1f(-sig);
+        const float wgt = T * al;
+        cr += wgt * q.y;
+        cg += wgt * q.z;
+        cb += wgt * q.w;
+        T = T * e;
+        ++n;
     }
     if (a.rgb) {
         a.rgb[3 * ray] = cr;
@@ -166,54 +193,86 @@ __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
     if (a.nsamp) a.nsamp[ray] = n;
 }
 
+// One record per sample. A wave's records fill the contiguous range that the
+// exclusive scan of the forward's per-ray counts gives its 64 rays; within it
+// the lanes that emit in one iteration take consecutive slots (ballot rank),
+// so stores coalesce and the record order is fixed by the inputs alone.
 __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
-    const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-    const int view = blockIdx.z;
-    if (x >= a.w || y >= a.h) return;
-    const size_t ray = ((size_t)view * a.h + y) * a.w + x;
-    CompRay r;
-    if (!comp_setup(a, view, x, y, r)) return;
-    const float gr = a.grad_rgb[3 * ray], gg = a.grad_rgb[3 * ray + 1], gb = a.grad_rgb[3 * ray + 2];
-    const float fr = a.rgb_in[3 * ray], fg = a.rgb_in[3 * ray + 1], fb = a.rgb_in[3 * ray + 2];
+    const size_t ray = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t first = ray & ~(size_t)63;  // the wave's first ray
+    const size_t total_rays = (size_t)a.w * a.h * a.n_views;
+    if (first >= total_rays) return;  // whole wave past the end
+    int view = 0, x = 0, y = 0;
+    const bool valid = ray_of(a, ray, view, x, y);
+    SampleIter it;
+    it.live = valid && comp_setup(a, view, x, y, it.r);
+    it.k = it.r.k;
+    float gr = 0.0f, gg = 0.0f, gb = 0.0f, fr = 0.0f, fg = 0.0f, fb = 0.0f;
+    if (valid) {
+        gr = a.grad_rgb[3 * ray];
+        gg = a.grad_rgb[3 * ray + 1];
+        gb = a.grad_rgb[3 * ray + 2];
+        fr = a.rgb_in[3 * ray];
+        fg = a.rgb_in[3 * ray + 1];
+        fb = a.rgb_in[3 * ray + 2];
+    }
+    uint32_t cursor = a.offsets[first];
     float pr = 0.0f, pg = 0.0f, pb = 0.0f, T = 1.0f;
-    uint32_t out = a.offsets[ray];
-    const uint32_t end = out + a.nsamp_in[ray];  // never write past this ray's records
-    comp_walk(a, r, [&](uint32_t idx, float4 q) {
-        if (out >= end) return;
-        const float sig = softplus(q.x);
-        const float e = expf(-sig);
-        const float al = -expm1f(-sig);
-        const float wgt = T * al;
-        pr += wgt * q.y;
-        pg += wgt * q.z;
-        pb += wgt * q.w;
-        const float tn = T * e;
-        const float sr = fr - pr, sg = fg - pg, sb = fb - pb;
-        const float dsig = gr * (tn * q.y - sr) + gg * (tn * q.z - sg) + gb * (tn * q.w - sb);
-        const float sgm = 1.0f / (1.0f + expf(-q.x));
-        a.keys[out] = idx;
-        a.vals[out] = make_float4(dsig * sgm, wgt * gr, wgt * gg, wgt * gb);
-        ++out;
-        T = tn;
-    });
+    for (;;) {
+        float4 q;
+        const int64_t idx = it.next(a, q);
+        const uint64_t mask = __ballot(idx >= 0);
+        if (mask == 0) break;
+        if (idx >= 0) {
+            const float sig = softplus(q.x);
+            const float e = expf(-sig);
+            const float al = -expm1f(-sig);
+            const float wgt = T * al;
+            pr += wgt * q.y;
+            pg += wgt * q.z;
+            pb += wgt * q.w;
+            const float tn = T * e;
+            const float sr = fr - pr, sg = fg - pg, sb = fb - pb;
+            const float dsig = gr * (tn * q.y - sr) + gg * (tn * q.z - sg) + gb * (tn * q.w - sb);
+            const float sgm = 1.0f / (1.0f + expf(-q.x));
+            const uint32_t pos = cursor + lanes_rank(mask);
+            if (pos < a.total_records) {
+                a.keys[pos] = (uint32_t)idx;
+                a.vals[pos] = make_float4(dsig * sgm, wgt * gr, wgt * gg, wgt * gb);
+            }
+            T = tn;
+        }
+        cursor += (uint32_t)__popcll(mask);
+    }
 }
 
-struct Float4Sum {
-    __device__ __forceinline__ float4 operator()(const float4& p, const float4& q) const {
-        return make_float4(p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w);
-    }
-};
+// Statistics only (not on the gradient path): number of distinct sorted keys.
+__global__ void comp_count_runs_kernel(const uint32_t* keys, uint32_t n, uint32_t* count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (i == 0 || keys[i - 1] != keys[i])) atomicAdd(count, 1u);
+}
 
 __global__ void comp_zero_kernel(float4* g, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) g[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-__global__ void comp_scatter_kernel(const uint32_t* keys, const float4* sums, const uint32_t* num_runs,
-                                    float4* grad) {
+// Records sorted by key: the first record of each key's run sums the run in
+// sorted order and writes that voxel's gradient (no atomics; deterministic).
+__global__ void comp_reduce_runs_kernel(const uint32_t* keys, const float4* vals, uint32_t n, float4* grad) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < *num_runs) grad[keys[i]] = sums[i];
+    if (i >= n) return;
+    const uint32_t key = keys[i];
+    if (i > 0 && keys[i - 1] == key) return;
+    float4 s = vals[i];
+    for (uint32_t j = i + 1; j < n && keys[j] == key; ++j) {
+        const float4 v = vals[j];
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+    }
+    grad[key] = s;
 }
 
 }  // namespace yx
@@ -229,8 +288,9 @@ using namespace yx;
 struct yx_compositor {
     void* ws = nullptr;  // one device allocation, carved into the buffers below
     size_t ws_bytes = 0;
-    uint64_t last_records = 0, last_unique = 0;
-    const uint32_t* d_runs = nullptr;  // distinct keys of the last backward (device)
+    uint64_t last_records = 0;
+    const uint32_t* sorted_keys = nullptr;  // the last backward's sorted keys (workspace)
+    uint32_t* counter = nullptr;
 };
 
 static int check_vol(const yx_volume_dev* v, int32_t n_views, int32_t w, int32_t h) {
@@ -317,12 +377,16 @@ extern "C" int yx_compositor_last_stats(const yx_compositor* c, uint64_t* record
     clear_error();
     if (!c || !records || !unique_keys) return fail(YX_ERR_INVALID_ARG, "null argument");
     *records = c->last_records;
-    if (c->d_runs) {
+    *unique_keys = 0;
+    if (c->sorted_keys && c->last_records) {
+        YX_HIP(hipMemset(c->counter, 0, 4));
+        const uint32_t n = (uint32_t)c->last_records;
+        hipLaunchKernelGGL(comp_count_runs_kernel, dim3((n + 255) / 256), dim3(256), 0, nullptr, c->sorted_keys, n,
+                           c->counter);
+        YX_HIP(hipGetLastError());
         uint32_t runs = 0;
-        YX_HIP(hipMemcpy(&runs, c->d_runs, 4, hipMemcpyDeviceToHost));
+        YX_HIP(hipMemcpy(&runs, c->counter, 4, hipMemcpyDeviceToHost));
         *unique_keys = runs;
-    } else {
-        *unique_keys = c->last_unique;
     }
     return YX_OK;
 }
@@ -337,8 +401,9 @@ extern "C" int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, 
     a.rgb = rgb;
     a.trans = trans;
     a.nsamp = nsamp;
-    dim3 grid((w + 15) / 16, (h + 15) / 16, n_views);
-    hipLaunchKernelGGL(comp_forward_kernel, grid, dim3(256), 0, (hipStream_t)stream, a);
+    const size_t rays = (size_t)n_views * w * h;
+    hipLaunchKernelGGL(comp_forward_kernel, dim3((unsigned)((rays + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       a);
     YX_HIP(hipGetLastError());
     return YX_OK;
 }
@@ -388,26 +453,19 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
                                    (float4*)grad_params, n4);
         YX_HIP(hipGetLastError());
     }
-    c->d_runs = nullptr;
-    if (total == 0) {
-        c->last_unique = 0;
-        return YX_OK;
-    }
+    c->sorted_keys = nullptr;
+    if (total == 0) return YX_OK;
 
-    // 2. workspace: keys, vals, sorted keys, sorted vals, unique keys, sums,
-    //    run count, library temporaries
+    // 2. workspace: keys, vals, sorted keys, sorted vals, library temporaries
     int key_bits = 1;
     while (key_bits < 32 && ((uint64_t)1 << key_bits) < (uint64_t)nvox) ++key_bits;
-    size_t sort_tmp = 0, red_tmp = 0;
+    size_t sort_tmp = 0;
     YX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                               (float4*)nullptr, (float4*)nullptr, (int)total, 0, key_bits, st));
-    YX_HIP(hipcub::DeviceReduce::ReduceByKey(nullptr, red_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (float4*)nullptr, (float4*)nullptr, (uint32_t*)nullptr, Float4Sum(),
-                                             (int)total, st));
     const size_t kb = align256(total * 4), vb = align256(total * 16);
-    const size_t need = off_bytes + 2 * kb + 2 * vb + 256 + align256(std::max(std::max(sort_tmp, red_tmp), scan_tmp));
+    const size_t need = off_bytes + 2 * kb + 2 * vb + 256 + align256(std::max(sort_tmp, scan_tmp));
     if (c->ws_bytes < need) {
-        // keep the offsets: copy them aside is unnecessary, they are recomputed
+        // grow, then recompute the offsets into the new block
         if (c->ws) YX_HIP(hipFree(c->ws));
         c->ws = nullptr;
         c->ws_bytes = 0;
@@ -427,35 +485,29 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     p += vb;
     float4* vals_sorted = (float4*)p;
     p += vb;
-    uint32_t* num_runs = (uint32_t*)p;
+    uint32_t* counter = (uint32_t*)p;
     p += 256;
     void* tmp = p;
-    // unique keys and run sums reuse the unsorted record arrays
-    uint32_t* uniq = keys;
-    float4* sums = vals;
 
-    // 3. emit records
+    // 3. emit records (one wave = 64 consecutive rays)
     CompArgs a = make_args(vol, views, n_views, w, h);
     a.rgb_in = rgb;
     a.grad_rgb = grad_rgb;
     a.offsets = offsets;
     a.nsamp_in = nsamp;
+    a.total_records = (uint32_t)total;
     a.keys = keys;
     a.vals = vals;
-    dim3 grid((w + 15) / 16, (h + 15) / 16, n_views);
-    hipLaunchKernelGGL(comp_backward_kernel, grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL(comp_backward_kernel, dim3((unsigned)((rays + 255) / 256)), dim3(256), 0, st, a);
     YX_HIP(hipGetLastError());
 
-    // 4. sort by voxel key (LSD radix, stable), 5. reduce each run in order
+    // 4. sort by voxel key (LSD radix, stable), 5. sum each key's run in order
     YX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys_sorted, vals, vals_sorted, (int)total, 0,
                                               key_bits, st));
-    YX_HIP(hipcub::DeviceReduce::ReduceByKey(tmp, red_tmp, keys_sorted, uniq, vals_sorted, sums, num_runs,
-                                             Float4Sum(), (int)total, st));
-    // 6. scatter: one write per distinct voxel
-    const uint32_t max_runs = (uint32_t)std::min<uint64_t>(total, nvox);
-    hipLaunchKernelGGL(comp_scatter_kernel, dim3((max_runs + 255) / 256), dim3(256), 0, st, uniq, sums, num_runs,
-                       (float4*)grad_params);
+    hipLaunchKernelGGL(comp_reduce_runs_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, keys_sorted,
+                       vals_sorted, (uint32_t)total, (float4*)grad_params);
     YX_HIP(hipGetLastError());
-    c->d_runs = num_runs;
+    c->sorted_keys = keys_sorted;
+    c->counter = counter;
     return YX_OK;
 }
