@@ -50,6 +50,7 @@ struct CompArgs {
     const uint32_t* offsets;
     const uint32_t* nsamp_in;
     uint32_t total_records;
+    uint32_t n_waves;
     uint32_t* keys;
     float4* vals;
 };
@@ -151,22 +152,29 @@ struct SampleIter {
 
 __device__ __forceinline__ float softplus(float x) { return x > 15.0f ? x : log1pf(expf(x)); }
 
-// Rays are numbered view-major, then row-major; thread i of the launch takes
-// ray i, so a wave holds 64 consecutive rays of a row.
-__device__ __forceinline__ bool ray_of(const CompArgs& a, size_t ray, int& view, int& x, int& y) {
-    const size_t per_view = (size_t)a.w * a.h;
-    if (ray >= per_view * (size_t)a.n_views) return false;
-    view = (int)(ray / per_view);
-    const size_t rem = ray - (size_t)view * per_view;
-    y = (int)(rem / (size_t)a.w);
-    x = (int)(rem - (size_t)y * a.w);
+// Thread t of a launch takes lane t % 64 of wave t / 64; each wave covers one
+// 8x8 pixel tile (tiles row-major per view, views in order) so its rays stay
+// coherent. Rays (outputs) are numbered view-major, then row-major.
+__device__ __forceinline__ bool ray_of(const CompArgs& a, size_t t, int& view, int& x, int& y, size_t& ray) {
+    const size_t wave = t >> 6;
+    const int lane = (int)(t & 63);
+    const int tx_n = (a.w + 7) >> 3, ty_n = (a.h + 7) >> 3;
+    const size_t per_view = (size_t)tx_n * ty_n;
+    if (wave >= per_view * (size_t)a.n_views) return false;
+    view = (int)(wave / per_view);
+    const int tile = (int)(wave - (size_t)view * per_view);
+    const int ty = tile / tx_n, tx = tile - ty * tx_n;
+    x = tx * 8 + (lane & 7);
+    y = ty * 8 + (lane >> 3);
+    if (x >= a.w || y >= a.h) return false;
+    ray = ((size_t)view * a.h + y) * a.w + x;
     return true;
 }
 
 __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
-    const size_t ray = (size_t)blockIdx.x * 256 + threadIdx.x;
     int view, x, y;
-    if (!ray_of(a, ray, view, x, y)) return;
+    size_t ray;
+    if (!ray_of(a, (size_t)blockIdx.x * 256 + threadIdx.x, view, x, y, ray)) return;
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
     uint32_t n = 0;
     SampleIter it;
@@ -193,17 +201,28 @@ __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
     if (a.nsamp) a.nsamp[ray] = n;
 }
 
+// Sum of the forward's per-ray sample counts over each wave's tile.
+__global__ __launch_bounds__(256) void comp_wave_sums_kernel(CompArgs a, uint32_t* sums, uint32_t n_waves) {
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    int view, x, y;
+    size_t ray = 0;
+    const bool valid = ray_of(a, t, view, x, y, ray);
+    uint32_t n = valid ? a.nsamp_in[ray] : 0u;
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off);
+    if ((t & 63) == 0 && (t >> 6) < n_waves) sums[t >> 6] = n;
+}
+
 // One record per sample. A wave's records fill the contiguous range that the
-// exclusive scan of the forward's per-ray counts gives its 64 rays; within it
-// the lanes that emit in one iteration take consecutive slots (ballot rank),
-// so stores coalesce and the record order is fixed by the inputs alone.
+// exclusive scan of the per-wave sample counts gives its tile; within it the
+// lanes that emit in one iteration take consecutive slots (ballot rank), so
+// stores coalesce and the record order is fixed by the inputs alone.
 __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
-    const size_t ray = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t first = ray & ~(size_t)63;  // the wave's first ray
-    const size_t total_rays = (size_t)a.w * a.h * a.n_views;
-    if (first >= total_rays) return;  // whole wave past the end
+    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t wave = t >> 6;
+    if (wave >= a.n_waves) return;  // whole wave past the end
     int view = 0, x = 0, y = 0;
-    const bool valid = ray_of(a, ray, view, x, y);
+    size_t ray = 0;
+    const bool valid = ray_of(a, t, view, x, y, ray);
     SampleIter it;
     it.live = valid && comp_setup(a, view, x, y, it.r);
     it.k = it.r.k;
@@ -216,7 +235,7 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
         fg = a.rgb_in[3 * ray + 1];
         fb = a.rgb_in[3 * ray + 2];
     }
-    uint32_t cursor = a.offsets[first];
+    uint32_t cursor = a.offsets[wave];
     float pr = 0.0f, pg = 0.0f, pb = 0.0f, T = 1.0f;
     for (;;) {
         float4 q;
@@ -247,9 +266,14 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
 }
 
 // Statistics only (not on the gradient path): number of distinct sorted keys.
-__global__ void comp_count_runs_kernel(const uint32_t* keys, uint32_t n, uint32_t* count) {
+__global__ __launch_bounds__(256) void comp_count_runs_kernel(const uint32_t* keys, uint32_t n, uint32_t* count) {
+    __shared__ uint32_t part[4];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && (i == 0 || keys[i - 1] != keys[i])) atomicAdd(count, 1u);
+    const bool head = i < n && (i == 0 || keys[i - 1] != keys[i]);
+    const uint32_t c = (uint32_t)__popcll(__ballot(head));
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(count, part[0] + part[1] + part[2] + part[3]);
 }
 
 __global__ void comp_zero_kernel(float4* g, size_t n) {
@@ -401,9 +425,9 @@ extern "C" int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, 
     a.rgb = rgb;
     a.trans = trans;
     a.nsamp = nsamp;
-    const size_t rays = (size_t)n_views * w * h;
-    hipLaunchKernelGGL(comp_forward_kernel, dim3((unsigned)((rays + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
-                       a);
+    const size_t waves = (size_t)n_views * ((w + 7) / 8) * ((h + 7) / 8);
+    hipLaunchKernelGGL(comp_forward_kernel, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, a);
     YX_HIP(hipGetLastError());
     return YX_OK;
 }
@@ -423,10 +447,17 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     const size_t nvox = (size_t)vol->num_slots * vol->brick * vol->brick * vol->brick;
     if (rays >= 0xFFFFFFFFull) return fail(YX_ERR_RANGE, "too many rays");
 
-    // 1. exclusive scan of the per-ray sample counts -> record offsets, total
+    // 1. per-wave sample counts (8x8 ray tiles), exclusive scan -> each
+    //    wave's record offset, and the total
+    const size_t n_waves = (size_t)n_views * ((w + 7) / 8) * ((h + 7) / 8);
+    CompArgs a = make_args(vol, views, n_views, w, h);
+    a.nsamp_in = nsamp;
+    a.n_waves = (uint32_t)n_waves;
     size_t scan_tmp = 0;
-    YX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, nsamp, (uint32_t*)nullptr, (int)rays, st));
-    const size_t off_bytes = align256((rays + 1) * sizeof(uint32_t));
+    YX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                            (int)n_waves, st));
+    const size_t wb = align256(n_waves * sizeof(uint32_t));
+    const size_t off_bytes = 2 * wb;
     const size_t need0 = off_bytes + align256(scan_tmp) + 256;
     if (c->ws_bytes < need0) {
         if (c->ws) YX_HIP(hipFree(c->ws));
@@ -436,11 +467,21 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
         if (e != hipSuccess) return fail(YX_ERR_OOM, "hipMalloc(%zu) workspace: %s", need0, hipGetErrorString(e));
         c->ws_bytes = need0;
     }
-    uint32_t* offsets = (uint32_t*)c->ws;
-    YX_HIP(hipcub::DeviceScan::ExclusiveSum((char*)c->ws + off_bytes, scan_tmp, nsamp, offsets, (int)rays, st));
+    auto wave_offsets = [&]() -> int {
+        uint32_t* sums = (uint32_t*)c->ws;
+        uint32_t* offs = (uint32_t*)((char*)c->ws + wb);
+        hipLaunchKernelGGL(comp_wave_sums_kernel, dim3((unsigned)((n_waves * 64 + 255) / 256)), dim3(256), 0, st, a,
+                           sums, (uint32_t)n_waves);
+        YX_HIP(hipGetLastError());
+        YX_HIP(hipcub::DeviceScan::ExclusiveSum((char*)c->ws + off_bytes, scan_tmp, sums, offs, (int)n_waves, st));
+        return YX_OK;
+    };
+    rc = wave_offsets();
+    if (rc != YX_OK) return rc;
+    uint32_t* offsets = (uint32_t*)((char*)c->ws + wb);
     uint32_t last[2];
-    YX_HIP(hipMemcpyAsync(&last[0], offsets + rays - 1, 4, hipMemcpyDeviceToHost, st));
-    YX_HIP(hipMemcpyAsync(&last[1], nsamp + rays - 1, 4, hipMemcpyDeviceToHost, st));
+    YX_HIP(hipMemcpyAsync(&last[0], offsets + n_waves - 1, 4, hipMemcpyDeviceToHost, st));
+    YX_HIP(hipMemcpyAsync(&last[1], (uint32_t*)c->ws + n_waves - 1, 4, hipMemcpyDeviceToHost, st));
     YX_HIP(hipStreamSynchronize(st));
     const uint64_t total = (uint64_t)last[0] + last[1];
     if (total >= 0x7FFFFFFFull) return fail(YX_ERR_RANGE, "%llu records exceed 2^31", (unsigned long long)total);
@@ -473,8 +514,9 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
         hipError_t e = hipMalloc(&c->ws, grow);
         if (e != hipSuccess) return fail(YX_ERR_OOM, "hipMalloc(%zu) workspace: %s", grow, hipGetErrorString(e));
         c->ws_bytes = grow;
-        offsets = (uint32_t*)c->ws;
-        YX_HIP(hipcub::DeviceScan::ExclusiveSum((char*)c->ws + off_bytes, scan_tmp, nsamp, offsets, (int)rays, st));
+        rc = wave_offsets();
+        if (rc != YX_OK) return rc;
+        offsets = (uint32_t*)((char*)c->ws + wb);
     }
     char* p = (char*)c->ws + off_bytes;
     uint32_t* keys = (uint32_t*)p;
@@ -489,16 +531,14 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     p += 256;
     void* tmp = p;
 
-    // 3. emit records (one wave = 64 consecutive rays)
-    CompArgs a = make_args(vol, views, n_views, w, h);
+    // 3. emit records (one wave = one 8x8 ray tile)
     a.rgb_in = rgb;
     a.grad_rgb = grad_rgb;
     a.offsets = offsets;
-    a.nsamp_in = nsamp;
     a.total_records = (uint32_t)total;
     a.keys = keys;
     a.vals = vals;
-    hipLaunchKernelGGL(comp_backward_kernel, dim3((unsigned)((rays + 255) / 256)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(comp_backward_kernel, dim3((unsigned)((n_waves * 64 + 255) / 256)), dim3(256), 0, st, a);
     YX_HIP(hipGetLastError());
 
     // 4. sort by voxel key (LSD radix, stable), 5. sum each key's run in order
