@@ -100,53 +100,54 @@ __device__ __forceinline__ int vox(float o, float d, float t, float nf, int n) {
     return min(max(v, 0), n - 1);
 }
 
-// Iterator over the samples of one ray that fall in allocated bricks, in
-// order: next() returns the parameter index of the next such sample (and its
-// float4 parameters) or -1 when the ray is done.
+// Walk over the samples of one ray that fall in allocated bricks, in order,
+// one step per call: step() either jumps over the samples of one empty brick
+// or yields one sample (returns true with its parameter index and float4
+// parameters); `live` turns false at the end of the ray. One step per call
+// keeps the lanes of a wave in one flat loop: a lane crossing empty space does
+// not hold up the others' samples.
 struct SampleIter {
     CompRay r;
     int k;
     bool live;
 
-    __device__ __forceinline__ int64_t next(const CompArgs& a, float4& q) {
-        if (!live) return -1;
+    __device__ __forceinline__ bool step(const CompArgs& a, uint32_t& idx, float4& q) {
+        if (!live) return false;
+        if (k >= r.kend) {
+            live = false;
+            return false;
+        }
         const float nf = (float)a.n, inv_n = 1.0f / nf;
         const int bmask = (1 << a.lb) - 1;
-        const float bsz = (float)(1 << a.lb) * inv_n;
-        while (k < r.kend) {
-            const float t = (float)k * inv_n;
-            const int vx = vox(r.ox, r.dx, t, nf, a.n), vy = vox(r.oy, r.dy, t, nf, a.n),
-                      vz = vox(r.oz, r.dz, t, nf, a.n);
-            const int bx = vx >> a.lb, by = vy >> a.lb, bz = vz >> a.lb;
-            const int slot = a.table[((size_t)bz * a.g + by) * a.g + bx];
-            if ((uint32_t)slot >= a.num_slots) {  // -1 (and anything out of range) is empty
-                // jump to the last sample still in this brick (exact: the
-                // candidate is checked and moved back while it left the brick)
-                const float ex = ((r.dx > 0.0f ? (float)(bx + 1) : (float)bx) * bsz - r.ox) * r.rx;
-                const float ey = ((r.dy > 0.0f ? (float)(by + 1) : (float)by) * bsz - r.oy) * r.ry;
-                const float ez = ((r.dz > 0.0f ? (float)(bz + 1) : (float)bz) * bsz - r.oz) * r.rz;
-                const float tb = fminf(ex, fminf(ey, ez));
-                int kk = (int)ceilf(tb * nf) - 1;
-                kk = min(max(kk, k), r.kend - 1);
-                while (kk > k) {
-                    const float t2 = (float)kk * inv_n;
-                    const int wx = vox(r.ox, r.dx, t2, nf, a.n) >> a.lb, wy = vox(r.oy, r.dy, t2, nf, a.n) >> a.lb,
-                              wz = vox(r.oz, r.dz, t2, nf, a.n) >> a.lb;
-                    if (wx == bx && wy == by && wz == bz) break;
-                    --kk;
-                }
-                k = kk + 1;
-                continue;
+        const float t = (float)k * inv_n;
+        const int vx = vox(r.ox, r.dx, t, nf, a.n), vy = vox(r.oy, r.dy, t, nf, a.n), vz = vox(r.oz, r.dz, t, nf, a.n);
+        const int bx = vx >> a.lb, by = vy >> a.lb, bz = vz >> a.lb;
+        const int slot = a.table[((size_t)bz * a.g + by) * a.g + bx];
+        if ((uint32_t)slot >= a.num_slots) {  // -1 (and anything out of range) is empty
+            // jump to the last sample still in this brick (exact: the candidate
+            // is checked and moved back while it left the brick)
+            const float bsz = (float)(1 << a.lb) * inv_n;
+            const float ex = ((r.dx > 0.0f ? (float)(bx + 1) : (float)bx) * bsz - r.ox) * r.rx;
+            const float ey = ((r.dy > 0.0f ? (float)(by + 1) : (float)by) * bsz - r.oy) * r.ry;
+            const float ez = ((r.dz > 0.0f ? (float)(bz + 1) : (float)bz) * bsz - r.oz) * r.rz;
+            const float tb = fminf(ex, fminf(ey, ez));
+            int kk = (int)ceilf(tb * nf) - 1;
+            kk = min(max(kk, k), r.kend - 1);
+            while (kk > k) {
+                const float t2 = (float)kk * inv_n;
+                const int wx = vox(r.ox, r.dx, t2, nf, a.n) >> a.lb, wy = vox(r.oy, r.dy, t2, nf, a.n) >> a.lb,
+                          wz = vox(r.oz, r.dz, t2, nf, a.n) >> a.lb;
+                if (wx == bx && wy == by && wz == bz) break;
+                --kk;
             }
-            const uint32_t local =
-                (uint32_t)((vx & bmask) + ((vy & bmask) << a.lb) + ((vz & bmask) << (2 * a.lb)));
-            const uint32_t idx = ((uint32_t)slot << (3 * a.lb)) + local;
-            q = a.params[idx];
-            ++k;
-            return idx;
+            k = kk + 1;
+            return false;
         }
-        live = false;
-        return -1;
+        const uint32_t local = (uint32_t)((vx & bmask) + ((vy & bmask) << a.lb) + ((vz & bmask) << (2 * a.lb)));
+        idx = ((uint32_t)slot << (3 * a.lb)) + local;
+        q = a.params[idx];
+        ++k;
+        return true;
     }
 };
 
@@ -181,7 +182,9 @@ __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
     it.live = comp_setup(a, view, x, y, it.r);
     it.k = it.r.k;
     float4 q;
-    while (it.next(a, q) >= 0) {
+    uint32_t idx;
+    while (it.live) {
+        if (!it.step(a, idx, q)) continue;
         const float sig = softplus(q.x);
         const float e = expf(-sig);
         const float al = -expm1f(-sig);
@@ -239,10 +242,11 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
     float pr = 0.0f, pg = 0.0f, pb = 0.0f, T = 1.0f;
     for (;;) {
         float4 q;
-        const int64_t idx = it.next(a, q);
-        const uint64_t mask = __ballot(idx >= 0);
-        if (mask == 0) break;
-        if (idx >= 0) {
+        uint32_t idx = 0;
+        const bool emit = it.step(a, idx, q);
+        const uint64_t mask = __ballot(emit);
+        if (mask == 0 && __ballot(it.live) == 0) break;
+        if (emit) {
             const float sig = softplus(q.x);
             const float e = expf(-sig);
             const float al = -expm1f(-sig);
@@ -256,7 +260,7 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
             const float sgm = 1.0f / (1.0f + expf(-q.x));
             const uint32_t pos = cursor + lanes_rank(mask);
             if (pos < a.total_records) {
-                a.keys[pos] = (uint32_t)idx;
+                a.keys[pos] = idx;
                 a.vals[pos] = make_float4(dsig * sgm, wgt * gr, wgt * gg, wgt * gb);
             }
             T = tn;
