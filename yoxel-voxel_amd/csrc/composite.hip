@@ -183,17 +183,22 @@ __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
     it.k = it.r.k;
     float4 q;
     uint32_t idx;
-    while (it.live) {
-        if (!it.step(a, idx, q)) continue;
-        const float sig = softplus(q.x);
-        const float e = expf(-sig);
-        const float al = -expm1f(-sig);
-        const float wgt = T * al;
-        cr += wgt * q.y;
-        cg += wgt * q.z;
-        cb += wgt * q.w;
-        T = T * e;
-        ++n;
+    // one step per lane per iteration; the wave-wide vote keeps the loop flat
+    // (without it the skip steps get folded into an inner loop that holds the
+    // wave until every lane has found its next sample)
+    for (;;) {
+        if (it.step(a, idx, q)) {
+            const float sig = softplus(q.x);
+            const float e = expf(-sig);
+            const float al = -expm1f(-sig);
+            const float wgt = T * al;
+            cr += wgt * q.y;
+            cg += wgt * q.z;
+            cb += wgt * q.w;
+            T = T * e;
+            ++n;
+        }
+        if (__ballot(it.live) == 0) break;
     }
     if (a.rgb) {
         a.rgb[3 * ray] = cr;
@@ -272,9 +277,10 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
 // Statistics only (not on the gradient path): number of distinct sorted keys.
 __global__ __launch_bounds__(256) void comp_count_runs_kernel(const uint32_t* keys, uint32_t n, uint32_t* count) {
     __shared__ uint32_t part[4];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool head = i < n && (i == 0 || keys[i - 1] != keys[i]);
-    const uint32_t c = (uint32_t)__popcll(__ballot(head));
+    uint32_t c = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        c += (i == 0 || keys[i - 1] != keys[i]) ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(count, part[0] + part[1] + part[2] + part[3]);
@@ -409,8 +415,7 @@ extern "C" int yx_compositor_last_stats(const yx_compositor* c, uint64_t* record
     if (c->sorted_keys && c->last_records) {
         YX_HIP(hipMemset(c->counter, 0, 4));
         const uint32_t n = (uint32_t)c->last_records;
-        hipLaunchKernelGGL(comp_count_runs_kernel, dim3((n + 255) / 256), dim3(256), 0, nullptr, c->sorted_keys, n,
-                           c->counter);
+        hipLaunchKernelGGL(comp_count_runs_kernel, dim3(1024), dim3(256), 0, nullptr, c->sorted_keys, n, c->counter);
         YX_HIP(hipGetLastError());
         uint32_t runs = 0;
         YX_HIP(hipMemcpy(&runs, c->counter, 4, hipMemcpyDeviceToHost));
