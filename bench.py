@@ -37,6 +37,10 @@ def parse():
     ap.add_argument("--cpu-frames", type=int, default=8, help="frames in the CPU baseline sample")
     ap.add_argument("--no-composite", action="store_true", help="skip the fwd+bwd compositing measurement")
     ap.add_argument("--composite-steps", type=int, default=8)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch measurement")
+    ap.add_argument("--batch-frames", type=int, default=4)
+    ap.add_argument("--batch-tune-cache", default=None,
+                    help="JSON file for the batch autotuner's result (default: tune every run)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the timed steps, copy 256 MiB device-to-device 3 times (a kernel of known "
                          "traffic for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -135,6 +139,7 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, cams, w, h, mode, wl.SUN, args.cpu_frames)
     fwd_bwd = None if args.no_composite else composite_bench(yx, torch, dev, args.composite_steps)
+    batched = None if args.no_batch else batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
 
     line = {
         "metric": BASELINE_METRIC,
@@ -159,6 +164,7 @@ def main():
                      "algorithmic_bytes_per_launch": int(np.mean(bytes_f))},
         "cpu_baseline": cpu,
         "fwd_bwd": fwd_bwd,
+        "batched": batched,
     }
     print(json.dumps(line), flush=True)
 
@@ -221,6 +227,80 @@ def composite_bench(yx, torch, dev, steps):
             "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
                          "fwd_achieved": round(fwd_bytes / (fwd_ms / 1e3) / 1e9, 1),
                          "bwd_achieved": round(bwd_bytes / (bwd_ms / 1e3) / 1e9, 1)}}
+
+
+def batch_bench(yx, torch, frames, tune_cache):
+    """batched_48: 48 octrees of depth 6-11 (sphere and sphere + heightfield
+    recipes), 416 cameras at 256x256 .. 1920x1080 (348.5 M primary rays, plus
+    shadow rays on the heightfield scenes). One frame of every job in one
+    persistent launch replayed from a hipGraph, with the launch shape from the
+    library's autotuner, against per-scene plain launches (one SVORenderer
+    per scene, one Render per camera, default launch shape) writing the same
+    output buffers. Both are timed with HIP events on the same stream over
+    `frames` frames after one untimed frame."""
+    wl = yx.workloads
+    recipes = wl.batched_recipes()
+    jobs = wl.batch_jobs(recipes)
+    t0 = time.time()
+    scenes = [yx.SVOData.build_synthetic(**d) for d, _ in recipes]
+    print(f"[bench] batched_48: {len(scenes)} scenes built in {time.time() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    b = yx.BatchRenderer()
+    b.set_scenes(scenes)
+    total = b.set_jobs(jobs)
+    keys = ("hit_id", "voxel", "t", "rgba", "shadow")
+    out = b.alloc_outputs(keys)
+    t0 = time.time()
+    best, trials = b.autotune(out, reps=1, cache=tune_cache)
+    tune_s = time.time() - t0
+    print(f"[bench] batch autotune: {len(trials)} trials in {tune_s:.1f} s, best {best}", file=sys.stderr,
+          flush=True)
+    stream = torch.cuda.current_stream()
+    b.render(stream=stream, **out)
+    torch.cuda.synchronize()
+    rays = total + int((out["shadow"] < 2).sum().item())
+
+    def timed(frame):
+        frame()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(frames):
+            frame()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / frames
+
+    batch_ms = timed(lambda: b.render(stream=stream, **out))
+    rs = []
+    for s in scenes:
+        r = yx.SVORenderer()
+        r.SetScene(s)
+        rs.append(r)
+    views = []
+    for (off, w, h), j in zip(b.job_slices(), jobs):
+        n = w * h
+        views.append((j, {k: out[k][off:off + n] for k in keys}))
+
+    def per_scene():
+        for j, o in views:
+            r = rs[j["scene"]]
+            r.SetViewPos(j["pos"])
+            r.SetViewDir(j["dir"])
+            r.SetFOV(j["fov"])
+            r.SetResolution(j["width"], j["height"])
+            r.SetLight(j["light"], j["light_mode"])
+            r.Render(stream=stream, **o)
+
+    plain_ms = timed(per_scene)
+    del rs
+    return {"value": round(rays / (batch_ms / 1e3) / 1e6, 3), "unit": "Mrays/s", "workload": "batched_48",
+            "scenes": len(scenes), "jobs": len(jobs), "primary_rays": total, "rays_per_frame": rays,
+            "ms_per_frame": round(batch_ms, 3), "per_scene_ms_per_frame": round(plain_ms, 3),
+            "per_scene_value": round(rays / (plain_ms / 1e3) / 1e6, 3),
+            "speedup_vs_per_scene": round(plain_ms / batch_ms, 3), "frames": frames,
+            "graph": True, "tuning": best, "autotune_trials": len(trials), "autotune_s": round(tune_s, 1),
+            "scene_bytes": b.scene_bytes()}
 
 
 def measured_traffic(workload):

@@ -167,6 +167,60 @@ typedef struct yx_tuning {
 int yx_renderer_set_tuning(yx_renderer* r, const yx_tuning* tuning);
 int yx_renderer_get_tuning(const yx_renderer* r, yx_tuning* tuning);
 
+/* ---- autotuner ------------------------------------------------------------
+ * Coordinate descent over the tuning space of the stack kernel: ray packet
+ * shape (tile_w x tile_h), queue chunk (refill_min: idle lanes a wave gathers
+ * before it takes that many ray ids with one atomic), register window (levels
+ * kept in registers; the rest of the traversal stack is LDS-resident),
+ * workgroup size and queue count. Each candidate renders `reps` timed frames
+ * (after one untimed one) into `out` on `stream`; the best is kept in the
+ * object. `trials` (capacity `cap`, may be NULL) receives every candidate in
+ * the order tried with its mean frame time in ms. No reference counterpart
+ * (the reference hard-codes its launch shape, trace_cuda.py:26). */
+typedef struct yx_tune_trial {
+    yx_tuning tuning;
+    float ms;
+} yx_tune_trial;
+int yx_renderer_autotune(yx_renderer* r, const yx_outputs* out, int32_t reps, yx_tune_trial* trials,
+                         int32_t cap, int32_t* n_trials, void* stream);
+
+/* ---- batched rendering: many scenes x many cameras in one launch ---------
+ * Replaces a loop of per-scene SetScene / SetView / Render calls
+ * (demo/Demo.cpp:138-174 drives one SVORenderer per scene and view) with one
+ * persistent launch over the rays of every job. A job is one camera on one
+ * scene of the batch; results are identical to rendering each job alone with
+ * yx_render. Outputs are per pixel as in yx_outputs, jobs concatenated in job
+ * order (job j starts at pixel yx_batch_job_offset(j)). */
+typedef struct yx_batch yx_batch;
+typedef struct yx_batch_job {
+    int32_t scene;         /* index into the batch's scene list             */
+    int32_t width, height;
+    float pos[3], dir[3], up[3];
+    float fov;             /* horizontal, degrees                           */
+    int32_t light_mode;    /* YX_LIGHT_*                                     */
+    float light[3];        /* unit direction toward the light               */
+} yx_batch_job;
+int yx_batch_create(yx_batch** out);
+void yx_batch_destroy(yx_batch* b);
+/* Uploads all scenes into one device record pool (each re-laid out as for
+ * yx_renderer_set_scene). Replaces any previous scenes and jobs. */
+int yx_batch_set_scenes(yx_batch* b, const yx_scene* const* scenes, int32_t n);
+/* Sets the jobs; total_pixels (may be NULL) receives the output size. Calling
+ * it again with only cameras and lights changed keeps the captured graph. */
+int yx_batch_set_jobs(yx_batch* b, const yx_batch_job* jobs, int32_t n, uint64_t* total_pixels);
+int yx_batch_job_offset(const yx_batch* b, int32_t job, uint64_t* first_pixel);
+int yx_batch_set_tuning(yx_batch* b, const yx_tuning* tuning); /* stack kernel only */
+int yx_batch_get_tuning(const yx_batch* b, yx_tuning* tuning);
+/* use_graph != 0 (default): each frame replays a hipGraph captured from the
+ * frame's launch (re-captured when outputs, tuning or job layout change);
+ * 0: plain kernel launch. */
+int yx_batch_set_graph(yx_batch* b, int32_t use_graph);
+/* One frame: every job's rays in one persistent launch on `stream`. */
+int yx_render_batch(yx_batch* b, const yx_outputs* out, void* stream);
+int yx_batch_autotune(yx_batch* b, const yx_outputs* out, int32_t reps, yx_tune_trial* trials, int32_t cap,
+                      int32_t* n_trials, void* stream);
+int yx_batch_scene_bytes(const yx_batch* b, uint64_t* bytes);
+
 /* ---- differentiable front-to-back compositing over brick voxels ----------
  * Forward model: reaction/trace/trace.cu:42-139 (castRay / walkBrick: brick
  * grid with occupancy table, samples on the lattice t = k*dt with dt = one

@@ -93,6 +93,24 @@ class Tuning(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class TuneTrial(ctypes.Structure):
+    _fields_ = [("tuning", Tuning), ("ms", ctypes.c_float)]
+
+
+class BatchJob(ctypes.Structure):
+    _fields_ = [
+        ("scene", ctypes.c_int32),
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("pos", ctypes.c_float * 3),
+        ("dir", ctypes.c_float * 3),
+        ("up", ctypes.c_float * 3),
+        ("fov", ctypes.c_float),
+        ("light_mode", ctypes.c_int32),
+        ("light", ctypes.c_float * 3),
+    ]
+
+
 class VolumeDesc(ctypes.Structure):
     _fields_ = [
         ("grid", ctypes.c_int32),
@@ -152,6 +170,21 @@ SIGNATURES = {
     "yx_renderer_scene_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "yx_renderer_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "yx_renderer_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "yx_renderer_autotune": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), ctypes.c_int32, ctypes.POINTER(TuneTrial),
+                                            ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _P]),
+    "yx_batch_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "yx_batch_destroy": (None, [_P]),
+    "yx_batch_set_scenes": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32]),
+    "yx_batch_set_jobs": (ctypes.c_int, [_P, ctypes.POINTER(BatchJob), ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_uint64)]),
+    "yx_batch_job_offset": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]),
+    "yx_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "yx_batch_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
+    "yx_batch_set_graph": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "yx_render_batch": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), _P]),
+    "yx_batch_autotune": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), ctypes.c_int32, ctypes.POINTER(TuneTrial),
+                                         ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _P]),
+    "yx_batch_scene_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "yx_volume_build_synthetic": (ctypes.c_int, [ctypes.POINTER(VolumeDesc), ctypes.POINTER(_P)]),
     "yx_volume_get_info": (ctypes.c_int, [_P, ctypes.POINTER(VolumeInfo)]),
     "yx_volume_arrays": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(_P)]),

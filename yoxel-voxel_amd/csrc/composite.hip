@@ -293,12 +293,43 @@ __global__ void comp_zero_kernel(float4* g, size_t n) {
 
 // Records sorted by key: the first record of each key's run sums the run in
 // sorted order and writes that voxel's gradient (no atomics; deterministic).
-__global__ void comp_reduce_runs_kernel(const uint32_t* keys, const float4* vals, uint32_t n, float4* grad) {
+// Sum each run of equal keys of the sorted records into grad[key]. A wave
+// takes 64 consecutive records and does a segmented inclusive scan over them
+// (fixed shuffle order, so the result is the same on every call); the last
+// lane of each run piece writes. A run that leaves the wave is finished by
+// the wave holding its first record, reading on serially; waves that only
+// hold the continuation of an earlier run write nothing for it.
+__global__ __launch_bounds__(256) void comp_reduce_runs_kernel(const uint32_t* keys, const float4* vals, uint32_t n,
+                                                               float4* grad) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t key = keys[i];
-    if (i > 0 && keys[i - 1] == key) return;
-    float4 s = vals[i];
+    const int lane = (int)(threadIdx.x & 63);
+    const bool in = i < n;
+    const uint32_t key = in ? keys[i] : 0xFFFFFFFFu;
+    float4 s = in ? vals[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    const uint32_t up = __shfl_up(key, 1u);
+    const uint32_t before = lane > 0 ? up : (i > 0 && in ? keys[i - 1] : ~key);
+    const bool head = in && before != key;
+    const uint64_t heads = __ballot(head);
+    const uint64_t starts = heads | 1ull;
+    const uint64_t below = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int start = 63 - __builtin_clzll(starts & below);
+    for (int off = 1; off < 64; off <<= 1) {
+        const float ox = __shfl_up(s.x, (unsigned)off), oy = __shfl_up(s.y, (unsigned)off);
+        const float oz = __shfl_up(s.z, (unsigned)off), ow = __shfl_up(s.w, (unsigned)off);
+        if (lane - off >= start) {
+            s.x += ox;
+            s.y += oy;
+            s.z += oz;
+            s.w += ow;
+        }
+    }
+    if (!in) return;
+    const uint32_t down = __shfl_down(key, 1u);
+    const bool last_in_wave = lane == 63 || i + 1 >= n;
+    const uint32_t after = last_in_wave ? (i + 1 < n ? keys[i + 1] : ~key) : down;
+    if (after == key && !last_in_wave) return;  // not the end of this run piece
+    const bool owns = (heads >> start) & 1ull;  // the run's first record is in this wave
+    if (!owns) return;
     for (uint32_t j = i + 1; j < n && keys[j] == key; ++j) {
         const float4 v = vals[j];
         s.x += v.x;
@@ -361,34 +392,8 @@ extern "C" int yx_camera_basis(const float pos[3], const float dir[3], const flo
     clear_error();
     if (!pos || !dir || !up || !view) return fail(YX_ERR_INVALID_ARG, "null argument");
     if (w <= 0 || h <= 0 || !(fov > 0.0f && fov < 180.0f)) return fail(YX_ERR_INVALID_ARG, "bad camera");
-    auto norm = [](float* v) {
-        float n2 = 0.0f;
-        n2 += v[0] * v[0];
-        n2 += v[1] * v[1];
-        n2 += v[2] * v[2];
-        const float n = std::sqrt(n2);
-        v[0] = v[0] / n;
-        v[1] = v[1] / n;
-        v[2] = v[2] / n;
-    };
-    float fwd[3] = {dir[0], dir[1], dir[2]}, right[3], upv[3];
-    norm(fwd);
-    right[0] = fwd[1] * up[2] - fwd[2] * up[1];
-    right[1] = fwd[2] * up[0] - fwd[0] * up[2];
-    right[2] = fwd[0] * up[1] - fwd[1] * up[0];
-    norm(right);
-    upv[0] = right[1] * fwd[2] - right[2] * fwd[1];
-    upv[1] = right[2] * fwd[0] - right[0] * fwd[2];
-    upv[2] = right[0] * fwd[1] - right[1] * fwd[0];
-    const float deg2rad = (float)(3.14159265358979323846 / (double)180.0f);
-    const float da = tanf((fov / 2.0f) * deg2rad) / (float)w;
-    for (int i = 0; i < 3; ++i) {
-        const float du = (right[i] * 2.0f) * da, dv = (upv[i] * -2.0f) * da;
-        view[i] = pos[i];
-        view[6 + i] = du;
-        view[9 + i] = dv;
-        view[3 + i] = (fwd[i] - (du * (float)w) / 2.0f) - (dv * (float)h) / 2.0f;
-    }
+    for (int i = 0; i < 3; ++i) view[i] = pos[i];
+    camera_basis(dir, up, fov, w, h, view + 3);
     return YX_OK;
 }
 

@@ -57,8 +57,9 @@ struct Perlin {
 };
 
 // Unified GPU record array built from a VoxNode tree (breadth-first, each
-// node's present children stored contiguously in slot order).
-//   node record: x = index of the first child record,
+// node's present children stored contiguously in slot order). `base` is the
+// index the array will start at in a shared pool (0 for a scene of its own).
+//   node record: x = pool index of the first child record,
 //                y = child_mask | leaf_mask << 8
 //   leaf record: x = leaf payload (RGBA8), y = canonical id of the parent node
 struct GpuTree {
@@ -67,7 +68,11 @@ struct GpuTree {
     int node_levels = 0;         // deepest node level + 1 (root only: 1)
     bool empty = true;
 };
-int build_gpu_tree(const yx_scene& s, GpuTree& out);
+int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base = 0);
+
+// Pixel-ray basis of a pinhole camera: dir0, du, dv as 9 floats; the ray of
+// pixel (x, y) is normalize(dir0 + du x + dv y).
+void camera_basis(const float dir[3], const float up[3], float fov, int w, int h, float basis[9]);
 
 // 3D Morton code, x in bit 0 of each triple (child slot convention).
 inline uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
@@ -91,6 +96,98 @@ void build_tree_from_codes(const std::vector<uint64_t>& codes, int depth, uint64
 // Leaf colour of a synthetic scene: a hash of the Morton code, RGBA8.
 inline uint32_t leaf_color(uint64_t code, uint64_t seed) {
     return (uint32_t)mix64(code ^ (seed * 0x9E3779B97F4A7C15ull));
+}
+
+}  // namespace yx
+
+namespace yx {
+
+// Coordinate descent over the stack kernel's tuning space, shared by the
+// single-scene renderer and the batch. `time_ms(t, &ms)` renders with tuning
+// t and returns a status (a candidate that is rejected as invalid is skipped).
+// Each axis in turn is set to the value with the lowest time while the others
+// stay fixed; sweeps repeat until one changes nothing (at most 3). A new value
+// must beat the current one by 1 % to replace it, so timing noise does not
+// move the result. Returns the best tuning found in *best.
+template <class TimeFn>
+int tune_coordinate_descent(const yx_tuning& start, TimeFn time_ms, yx_tune_trial* trials, int32_t cap,
+                            int32_t* n_trials, yx_tuning* best) {
+    struct Seen {
+        yx_tuning t;
+        float ms;
+    };
+    std::vector<Seen> seen;
+    int32_t n = 0;
+    auto same = [](const yx_tuning& a, const yx_tuning& b) {
+        return a.kernel == b.kernel && a.block == b.block && a.tile_w == b.tile_w && a.tile_h == b.tile_h &&
+               a.refill_min == b.refill_min && a.queues == b.queues && a.window == b.window &&
+               a.waves_per_cu == b.waves_per_cu;
+    };
+    auto eval = [&](const yx_tuning& t, float* ms) -> int {
+        for (const Seen& s : seen)
+            if (same(s.t, t)) {
+                *ms = s.ms;
+                return YX_OK;
+            }
+        const int rc = time_ms(t, ms);
+        if (rc == YX_ERR_INVALID_ARG || rc == YX_ERR_RANGE) {
+            *ms = -1.0f;
+        } else if (rc != YX_OK) {
+            return rc;
+        }
+        seen.push_back({t, *ms});
+        if (trials && n < cap) trials[n] = {t, *ms};
+        ++n;
+        return YX_OK;
+    };
+    static const int tiles[][2] = {{8, 8}, {16, 4}, {4, 16}, {16, 8}, {8, 16}, {16, 16}, {32, 8}};
+    static const int refills[] = {8, 16, 24, 32, 40, 48};
+    static const int windows[] = {0, 2, 4};
+    static const int blocks[] = {64, 128, 256};
+    static const int queues[] = {1, 8, 32};
+    yx_tuning cur = start;
+    cur.kernel = YX_KERNEL_PERSISTENT;
+    float cur_ms = 0.0f;
+    int rc = eval(cur, &cur_ms);
+    if (rc != YX_OK) return rc;
+    if (cur_ms < 0.0f) return fail(YX_ERR_INVALID_ARG, "the starting tuning is not valid");
+    auto try_axis = [&](auto apply, int count) -> int {
+        yx_tuning pick = cur;
+        float pick_ms = cur_ms;
+        for (int i = 0; i < count; ++i) {
+            yx_tuning t = cur;
+            apply(t, i);
+            float ms = 0.0f;
+            const int e = eval(t, &ms);
+            if (e != YX_OK) return e;
+            if (ms >= 0.0f && ms < pick_ms * 0.99f) {
+                pick = t;
+                pick_ms = ms;
+            }
+        }
+        const bool moved = !same(pick, cur);
+        cur = pick;
+        cur_ms = pick_ms;
+        return moved ? 1 : 0;
+    };
+    for (int sweep = 0; sweep < 3; ++sweep) {
+        int moved = 0, e;
+        if ((e = try_axis([](yx_tuning& t, int i) { t.tile_w = tiles[i][0]; t.tile_h = tiles[i][1]; }, 7)) < 0)
+            return e;
+        moved += e;
+        if ((e = try_axis([](yx_tuning& t, int i) { t.refill_min = refills[i]; }, 6)) < 0) return e;
+        moved += e;
+        if ((e = try_axis([](yx_tuning& t, int i) { t.window = windows[i]; }, 3)) < 0) return e;
+        moved += e;
+        if ((e = try_axis([](yx_tuning& t, int i) { t.block = blocks[i]; }, 3)) < 0) return e;
+        moved += e;
+        if ((e = try_axis([](yx_tuning& t, int i) { t.queues = queues[i]; }, 3)) < 0) return e;
+        moved += e;
+        if (moved == 0) break;
+    }
+    if (n_trials) *n_trials = n;
+    *best = cur;
+    return YX_OK;
 }
 
 }  // namespace yx

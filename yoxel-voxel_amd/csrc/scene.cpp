@@ -481,7 +481,45 @@ static int build_synthetic(const yx_scene_desc& d, yx_scene& out) {
 
 // ---- GPU re-layout --------------------------------------------------------
 
-int build_gpu_tree(const yx_scene& s, GpuTree& out) {
+// RendererBase::InitRayDir (cell/renderer_base.h:50-61): fwd = normalize(dir),
+// right = normalize(fwd x up), up' = right x fwd, da = tan(fov/2) / w,
+// du = 2 right da, dv = -2 up' da, dir0 = fwd - du w/2 - dv h/2. Integer
+// factors are converted to float first, as the cg point operators do.
+void camera_basis(const float dir[3], const float up[3], float fov, int w, int h, float basis[9]) {
+    auto norm = [](float* v) {
+        float n2 = 0.0f;
+        n2 += v[0] * v[0];
+        n2 += v[1] * v[1];
+        n2 += v[2] * v[2];
+        const float n = std::sqrt(n2);
+        v[0] = v[0] / n;
+        v[1] = v[1] / n;
+        v[2] = v[2] / n;
+    };
+    auto cross = [](const float* a, const float* b, float* o) {
+        o[0] = a[1] * b[2] - a[2] * b[1];
+        o[1] = a[2] * b[0] - a[0] * b[2];
+        o[2] = a[0] * b[1] - a[1] * b[0];
+    };
+    float fwd[3] = {dir[0], dir[1], dir[2]};
+    norm(fwd);
+    float right[3], upv[3];
+    cross(fwd, up, right);
+    norm(right);
+    cross(right, fwd, upv);
+    const float deg2rad = (float)(3.14159265358979323846 / (double)180.0f);
+    const float da = tanf((fov / 2.0f) * deg2rad) / (float)w;
+    const float wf = (float)w, hf = (float)h;
+    for (int i = 0; i < 3; ++i) {
+        const float du = (right[i] * 2.0f) * da;
+        const float dv = (upv[i] * -2.0f) * da;
+        basis[3 + i] = du;
+        basis[6 + i] = dv;
+        basis[i] = (fwd[i] - (du * wf) / 2.0f) - (dv * hf) / 2.0f;
+    }
+}
+
+int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base) {
     out.recs.clear();
     out.depth = 0;
     out.empty = true;
@@ -524,8 +562,8 @@ int build_gpu_tree(const yx_scene& s, GpuTree& out) {
         if (lvl + 1 > kMaxDepth)
             return fail(YX_ERR_RANGE, "tree deeper than %d levels", kMaxDepth);
         const size_t first = out.recs.size() / 2;
-        if (first + 8 > 0xFFFFFFF0ull) return fail(YX_ERR_RANGE, "more than 2^32 records");
-        out.recs[2 * (size_t)q_rec[h]] = (uint32_t)first;
+        if ((uint64_t)base + first + 8 > 0xFFFFFFF0ull) return fail(YX_ERR_RANGE, "more than 2^32 records");
+        out.recs[2 * (size_t)q_rec[h]] = (uint32_t)(base + first);
         out.recs[2 * (size_t)q_rec[h] + 1] = child_mask | (leaf_mask << 8);
         for (int c = 0; c < 8; ++c) {
             const uint32_t bit = 1u << c;

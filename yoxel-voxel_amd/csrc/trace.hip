@@ -14,6 +14,7 @@
 #include <new>
 
 #include "internal.h"
+#include "timing.h"
 #include "trace_device.h"
 #include "trace_persist.h"
 #include "trace_stack.h"
@@ -136,41 +137,8 @@ struct yx_renderer {
         if (e_ != hipSuccess) return fail(YX_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
     } while (0)
 
-// RendererBase::InitRayDir (cell/renderer_base.h:50-61); integer factors are
-// converted to float first, as the cg point operators do.
-static void init_ray_dir(const yx_renderer* r, float* out) {
-    auto norm = [](float* v) {
-        float n2 = 0.0f;
-        n2 += v[0] * v[0];
-        n2 += v[1] * v[1];
-        n2 += v[2] * v[2];
-        float n = std::sqrt(n2);
-        v[0] = v[0] / n;
-        v[1] = v[1] / n;
-        v[2] = v[2] / n;
-    };
-    auto cross = [](const float* a, const float* b, float* o) {
-        o[0] = a[1] * b[2] - a[2] * b[1];
-        o[1] = a[2] * b[0] - a[0] * b[2];
-        o[2] = a[0] * b[1] - a[1] * b[0];
-    };
-    float fwd[3] = {r->dir[0], r->dir[1], r->dir[2]};
-    norm(fwd);
-    float right[3], upv[3];
-    cross(fwd, r->up, right);
-    norm(right);
-    cross(right, fwd, upv);
-    const float deg2rad = (float)(3.14159265358979323846 / (double)180.0f);
-    const float da = tanf((r->fov / 2.0f) * deg2rad) / (float)r->w;
-    const float wf = (float)r->w, hf = (float)r->h;
-    for (int i = 0; i < 3; ++i) {
-        const float du = (right[i] * 2.0f) * da;
-        const float dv = (upv[i] * -2.0f) * da;
-        out[3 + i] = du;
-        out[6 + i] = dv;
-        out[i] = (fwd[i] - (du * wf) / 2.0f) - (dv * hf) / 2.0f;
-    }
-}
+// RendererBase::InitRayDir (cell/renderer_base.h:50-61).
+static void init_ray_dir(const yx_renderer* r, float* out) { camera_basis(r->dir, r->up, r->fov, r->w, r->h, out); }
 
 static int ensure_device(yx_renderer* r) {
     if (r->device >= 0) return YX_OK;
@@ -282,21 +250,21 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
         }
         if (lds <= r->max_lds) {
             if (block == 64) {
-                if (win == 0) return YX_LAUNCH((stack_kernel<64, 0>), 64);
-                if (win == 2) return YX_LAUNCH((stack_kernel<64, 2>), 64);
-                if (win == 4) return YX_LAUNCH((stack_kernel<64, 4>), 64);
-                return YX_LAUNCH((stack_kernel<64, 6>), 64);
+                if (win == 0) return YX_LAUNCH((stack_kernel<64, 0, false>), 64);
+                if (win == 2) return YX_LAUNCH((stack_kernel<64, 2, false>), 64);
+                if (win == 4) return YX_LAUNCH((stack_kernel<64, 4, false>), 64);
+                return YX_LAUNCH((stack_kernel<64, 6, false>), 64);
             }
             if (block == 128) {
-                if (win == 0) return YX_LAUNCH((stack_kernel<128, 0>), 128);
-                if (win == 2) return YX_LAUNCH((stack_kernel<128, 2>), 128);
-                if (win == 4) return YX_LAUNCH((stack_kernel<128, 4>), 128);
-                return YX_LAUNCH((stack_kernel<128, 6>), 128);
+                if (win == 0) return YX_LAUNCH((stack_kernel<128, 0, false>), 128);
+                if (win == 2) return YX_LAUNCH((stack_kernel<128, 2, false>), 128);
+                if (win == 4) return YX_LAUNCH((stack_kernel<128, 4, false>), 128);
+                return YX_LAUNCH((stack_kernel<128, 6, false>), 128);
             }
-            if (win == 0) return YX_LAUNCH((stack_kernel<256, 0>), 256);
-            if (win == 2) return YX_LAUNCH((stack_kernel<256, 2>), 256);
-            if (win == 4) return YX_LAUNCH((stack_kernel<256, 4>), 256);
-            return YX_LAUNCH((stack_kernel<256, 6>), 256);
+            if (win == 0) return YX_LAUNCH((stack_kernel<256, 0, false>), 256);
+            if (win == 2) return YX_LAUNCH((stack_kernel<256, 2, false>), 256);
+            if (win == 4) return YX_LAUNCH((stack_kernel<256, 4, false>), 256);
+            return YX_LAUNCH((stack_kernel<256, 6, false>), 256);
         }
         kernel = YX_KERNEL_REPLAY;
         block = 256;
@@ -508,4 +476,23 @@ extern "C" int yx_renderer_scene_bytes(const yx_renderer* r, uint64_t* bytes) {
     if (!r || !bytes) return fail(YX_ERR_INVALID_ARG, "null argument");
     *bytes = (uint64_t)r->n_recs * 8u;
     return YX_OK;
+}
+
+extern "C" int yx_renderer_autotune(yx_renderer* r, const yx_outputs* out, int32_t reps, yx_tune_trial* trials,
+                                    int32_t cap, int32_t* n_trials, void* stream) {
+    clear_error();
+    if (!r || !out) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (!r->has_scene) return fail(YX_ERR_NO_SCENE, "no scene set");
+    if (reps < 1) return fail(YX_ERR_INVALID_ARG, "reps must be >= 1");
+    const yx_tuning keep = r->tune;
+    const hipStream_t st = (hipStream_t)stream;
+    auto time_ms = [&](const yx_tuning& t, float* ms) -> int {
+        int rc = yx_renderer_set_tuning(r, &t);
+        if (rc != YX_OK) return rc;
+        return time_frames([&]() { return launch_frame(r, *out, st); }, reps, st, ms);
+    };
+    yx_tuning best = keep;
+    const int rc = tune_coordinate_descent(keep, time_ms, trials, cap, n_trials, &best);
+    r->tune = rc == YX_OK ? best : keep;
+    return rc;
 }

@@ -29,6 +29,24 @@ namespace yx {
 
 enum : int { kNeed = 0, kPrimary = 1, kShadow = 2, kRetired = 3 };
 
+constexpr uint32_t kNoRoot = 0xFFFFFFFFu;  // job whose scene has no nodes: every ray misses
+
+// One (scene, camera) job of a batched launch. The scenes share one record
+// pool (node records hold absolute pool indices), so a job only names its
+// root record; its rays are the tiles [tile_begin, next job's tile_begin) of
+// the launch's ray space and its outputs the pixels from pix_begin on.
+struct BatchJob {
+    float pos[3];
+    float basis[9];
+    float light[3];
+    int light_mode;
+    uint32_t root;
+    int w, h, tiles_x;
+    uint32_t pix_begin;
+    uint32_t pad[3];
+};
+static_assert(sizeof(BatchJob) == 96, "BatchJob layout");
+
 struct PersistArgs {
     const uint2* recs;
     float pos[3];
@@ -46,6 +64,10 @@ struct PersistArgs {
     int tiles_x;
     int refill_min;
     int levels;  // LDS rows per lane (node levels of the tree)
+    // batched launches only
+    const BatchJob* jobs;
+    const uint32_t* job_tiles;  // first tile of each job, n_jobs + 1 entries (last = all tiles)
+    int n_jobs;
 };
 
 struct WinEntry {

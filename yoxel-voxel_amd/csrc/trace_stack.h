@@ -28,8 +28,11 @@ struct QueueCursor {
     int left;  // queues not yet found drained (wave-uniform)
 };
 
+// In a batched launch the ray id's tile is looked up in the job table (binary
+// search over the jobs' first tiles) and x, y are within that job's image.
+template <bool BATCH>
 __device__ __forceinline__ bool take_ray(const PersistArgs& a, QueueCursor& qc, uint64_t need, bool idle,
-                                         uint32_t lane, int& x, int& y) {
+                                         uint32_t lane, int& job, int& x, int& y) {
     const uint32_t cnt = (uint32_t)__popcll(need);
     const int leader = __ffsll((unsigned long long)need) - 1;
     uint32_t got = 0;
@@ -45,12 +48,30 @@ __device__ __forceinline__ bool take_ray(const PersistArgs& a, QueueCursor& qc, 
     const uint32_t my = qbeg + got + lanes_below(need);
     if (my >= qend) return false;
     const int tl = a.tile_wl + a.tile_hl;
-    const uint32_t tile = my >> tl, within = my & ((1u << tl) - 1u);
-    const int tx = (int)(tile % (uint32_t)a.tiles_x);
-    const int ty = (int)(tile / (uint32_t)a.tiles_x);
+    uint32_t tile = my >> tl;
+    const uint32_t within = my & ((1u << tl) - 1u);
+    int tiles_x = a.tiles_x, w = a.w, h = a.h;
+    if constexpr (BATCH) {
+        // job_tiles[lo] <= tile < job_tiles[hi]
+        int lo = 0, hi = a.n_jobs;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (a.job_tiles[mid] <= tile)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        job = lo;
+        tile -= a.job_tiles[lo];
+        tiles_x = a.jobs[lo].tiles_x;
+        w = a.jobs[lo].w;
+        h = a.jobs[lo].h;
+    }
+    const int tx = (int)(tile % (uint32_t)tiles_x);
+    const int ty = (int)(tile / (uint32_t)tiles_x);
     x = (tx << a.tile_wl) + (int)(within & ((1u << a.tile_wl) - 1u));
     y = (ty << a.tile_hl) + (int)(within >> a.tile_wl);
-    return x < a.w && y < a.h;
+    return x < w && y < h;
 }
 
 __device__ __forceinline__ void write_miss(const yx_outputs& o, uint32_t pix, uint32_t visits) {
@@ -88,7 +109,9 @@ enum : int {
 
 // W: stack entries of the W levels above the current one held in registers
 // (a shift window); older levels live in LDS rows indexed by level.
-template <int BLOCK, int W>
+// BATCH: rays come from the (scene, camera) jobs of a.jobs (camera, light,
+// root record and output offset per job) instead of the single view in `a`.
+template <int BLOCK, int W, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     extern __shared__ uint4 sh_stk[];  // [levels - W][2][BLOCK]
     const int tid = threadIdx.x;
@@ -102,6 +125,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     int status = sIdle;
     bool adv = false, occluded = false;
     uint32_t pix = 0;
+    int job = 0;
     float ox = 0.0f, oy = 0.0f, oz = 0.0f, dx = 1.0f, dy = 1.0f, dz = 1.0f;
     Cell c;
     c.t1x = c.t1y = c.t1z = c.t2x = c.t2y = c.t2z = 0.0f;
@@ -162,14 +186,20 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                 if (a.out.t) a.out.t[pix] = t;
                 const int e = entry_axis(c);
                 const float de = e == 0 ? dx : (e == 1 ? dy : dz);
-                if (a.light_mode == YX_LIGHT_HEADLIGHT) {
+                const float* light = a.light;
+                int light_mode = a.light_mode;
+                if constexpr (BATCH) {
+                    light = a.jobs[job].light;
+                    light_mode = a.jobs[job].light_mode;
+                }
+                if (light_mode == YX_LIGHT_HEADLIGHT) {
                     lam = fabsf(de);
                 } else {
-                    const float le = e == 0 ? a.light[0] : (e == 1 ? a.light[1] : a.light[2]);
+                    const float le = e == 0 ? light[0] : (e == 1 ? light[1] : light[2]);
                     lam = dmax((de < 0) ? le : -le, 0.0f);
                 }
                 color = leaf.x;
-                if (a.light_mode == YX_LIGHT_SUN_SHADOW && lam > 0.0f) {
+                if (light_mode == YX_LIGHT_SUN_SHADOW && lam > 0.0f) {
                     // hit point stepped half a cell back across the entry face
                     const float half = __builtin_bit_cast(float, (uint32_t)(127 - (level + 2)) << 23);
                     const float off = (de < 0) ? half : -half;
@@ -177,9 +207,9 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     ox = (e == 0) ? px + off : px;
                     oy = (e == 1) ? py + off : py;
                     oz = (e == 2) ? pz + off : pz;
-                    dx = adjust_coord(a.light[0]);
-                    dy = adjust_coord(a.light[1]);
-                    dz = adjust_coord(a.light[2]);
+                    dx = adjust_coord(light[0]);
+                    dy = adjust_coord(light[1]);
+                    dz = adjust_coord(light[2]);
                     status = sStartS;
                 } else {
                     write_final(a.out, pix, color, 0.1f + 0.9f * lam, 2, visits);
@@ -201,15 +231,24 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     if (status == sIdle) status = sRetired;
                 } else {
                     int x = 0, y = 0;
-                    if (take_ray(a, qc, need, status == sIdle, lane, x, y)) {
-                        pix = (uint32_t)y * (uint32_t)a.w + (uint32_t)x;
-                        const Ray3 d = pixel_dir(a.basis, x, y);
+                    if (take_ray<BATCH>(a, qc, need, status == sIdle, lane, job, x, y)) {
+                        const float* basis = a.basis;
+                        const float* pos = a.pos;
+                        if constexpr (BATCH) {
+                            const BatchJob& jb = a.jobs[job];
+                            pix = jb.pix_begin + (uint32_t)y * (uint32_t)jb.w + (uint32_t)x;
+                            basis = jb.basis;
+                            pos = jb.pos;
+                        } else {
+                            pix = (uint32_t)y * (uint32_t)a.w + (uint32_t)x;
+                        }
+                        const Ray3 d = pixel_dir(basis, x, y);
                         dx = adjust_coord(d.x);
                         dy = adjust_coord(d.y);
                         dz = adjust_coord(d.z);
-                        ox = a.pos[0];
-                        oy = a.pos[1];
-                        oz = a.pos[2];
+                        ox = pos[0];
+                        oy = pos[1];
+                        oz = pos[2];
                         visits = 0;
                         status = sStartP;
                     }
@@ -221,11 +260,13 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                 d.x = dx;
                 d.y = dy;
                 d.z = dz;
-                const bool ok =
-                    recs != nullptr && setup_trace(p, d, c, flags) && min3(c.t2x, c.t2y, c.t2z) > 0;
+                uint32_t root = 0;
+                if constexpr (BATCH) root = a.jobs[job].root;
+                const bool ok = recs != nullptr && root != kNoRoot && setup_trace(p, d, c, flags) &&
+                                min3(c.t2x, c.t2y, c.t2z) > 0;
                 if (ok) {
                     ++visits;
-                    node = recs[0];
+                    node = recs[root];
                     level = 0;
                     adv = false;
                     ch = find_first_child(c);
@@ -260,7 +301,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
             if (!is_leaf && ((cm >> real) & 1u) && mt2 > 0) {
                 // push this level, descend into the child node
                 ++visits;
-                if (W > 0) {
+                if constexpr (W > 0) {
                     if (level >= W) lds_store(level - W, win[W - 1].c, win[W - 1].rec);
 #pragma unroll
                     for (int k = W - 1; k > 0; --k) win[k] = win[k - 1];
@@ -289,7 +330,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
             }
         } else {
             --level;
-            if (W > 0) {
+            if constexpr (W > 0) {
                 c = win[0].c;
                 node = win[0].rec;
 #pragma unroll

@@ -9,6 +9,9 @@ Error behaviour follows the reference: RenderFrame returns None when no scene
 is set (cell/ppu_renderer.cpp:78-79); other failures raise YoxelError.
 """
 import ctypes
+import json
+import os
+import pathlib
 
 import numpy as np
 
@@ -191,9 +194,57 @@ class SVORenderer:
         check(lib().yx_renderer_scene_bytes(self._h, ctypes.byref(b)))
         return b.value
 
+    def autotune(self, out, reps=3, cache=None, signature="", stream=None):
+        """Pick the launch shape for the current scene and camera with the
+        library's autotuner (yx_renderer_autotune); `out` is a dict of output
+        tensors as for Render. With `cache` (a JSON path) a result stored for
+        the same device and `signature` is reused without timing. Returns
+        (tuning dict, [(tuning dict, ms)] trials)."""
+        w, h = self.GetResolution()
+        return _autotune(self, lib().yx_renderer_autotune, out, reps, cache, stream,
+                         f"renderer:{w}x{h}:{signature}")
+
+    @staticmethod
+    def _outputs(out):
+        return _lib.Outputs(*[_addr(out.get(k)) for k in ("hit_id", "voxel", "t", "rgba", "shadow", "visits")])
+
     @property
     def handle(self):
         return self._h
+
+
+def _device_key():
+    import torch
+    p = torch.cuda.get_device_properties(torch.cuda.current_device())
+    return f"{p.name}/{getattr(p, 'gcnArchName', '')}/{p.multi_processor_count}"
+
+
+def _autotune(obj, fn, out, reps, cache, stream, sig):
+    key = f"{_device_key()}|{sig}"
+    if cache is not None:
+        path = pathlib.Path(cache)
+        if path.exists():
+            stored = json.loads(path.read_text()).get(key)
+            if stored is not None:
+                obj.set_tuning(**stored)
+                return obj.get_tuning(), []
+    cap = 256
+    trials = (_lib.TuneTrial * cap)()
+    n = ctypes.c_int32()
+    o = obj._outputs(out) if isinstance(out, dict) else out
+    check(fn(obj.handle, ctypes.byref(o), int(reps), trials, cap, ctypes.byref(n),
+             ctypes.c_void_p(_stream(stream))))
+    best = obj.get_tuning()
+    log = [(trials[i].tuning.as_dict(), float(trials[i].ms)) for i in range(min(n.value, cap))]
+    if cache is not None:
+        path = pathlib.Path(cache)
+        data = json.loads(path.read_text()) if path.exists() else {}
+        data[key] = best
+        path.parent.mkdir(parents=True, exist_ok=True)
+        tmp = path.with_suffix(path.suffix + f".{os.getpid()}.tmp")
+        tmp.write_text(json.dumps(data, indent=1, sort_keys=True))
+        os.replace(tmp, path)
+    return best, log
 
 
 def _addr(x):

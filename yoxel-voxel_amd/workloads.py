@@ -41,3 +41,46 @@ def orbit_cameras(n, seed=0, radius=ORBIT_RADIUS, elevation_deg=ELEVATION_DEG):
         d = np.array(CENTER) - p
         cams.append((tuple(np.float32(p)), tuple(np.float32(d))))
     return cams
+
+
+# batched_48: 48 octrees of depth 6..11, sphere-only (plain_512-like, headlight)
+# and sphere + heightfield (large_1080p-like, sun with shadow rays) recipes,
+# seeded per scene, each with 1..16 orbit cameras at resolutions drawn from
+# RESOLUTIONS.
+RESOLUTIONS = ((256, 256), (640, 480), (1280, 720), (1920, 1080))
+
+
+def batched_recipes(n_scenes=48, seed=48):
+    """Scene i: depth 6 + i % 6; recipe plain-like for (i // 6) even, else
+    large-like; sphere radii scale with the leaf grid (2^(depth-8) for the plain
+    recipe, 2^(depth-11) for the large one, at least one voxel); seed 1000 + i.
+    Returns [(scene desc dict, [camera dicts])], cameras as
+    {width, height, pos, dir, light_mode, light}."""
+    out = []
+    for i in range(n_scenes):
+        depth = 6 + i % 6
+        large = (i // 6) % 2 == 1
+        base = LARGE_1080P if large else PLAIN_512
+        s = 2.0 ** (depth - (11 if large else 8))
+        desc = dict(base)
+        desc.update(depth=depth, seed=1000 + i, radius_min=max(1.0, base["radius_min"] * s),
+                    radius_max=max(1.5, base["radius_max"] * s))
+        rng = np.random.default_rng(seed * 1000 + i)
+        n_cams = int(rng.integers(1, 17))
+        cams = []
+        for (pos, d) in orbit_cameras(n_cams, seed=seed * 1000 + i):
+            w, h = RESOLUTIONS[int(rng.integers(0, len(RESOLUTIONS)))]
+            cams.append(dict(width=w, height=h, pos=pos, dir=d,
+                             light_mode=2 if large else 0, light=SUN if large else (0.0, 0.0, 1.0)))
+        out.append((desc, cams))
+    return out
+
+
+def batch_jobs(recipes, fov=60.0):
+    """Flatten recipes into BatchRenderer jobs (scene index = recipe index)."""
+    jobs = []
+    for si, (_, cams) in enumerate(recipes):
+        for c in cams:
+            jobs.append(dict(scene=si, width=c["width"], height=c["height"], pos=c["pos"], dir=c["dir"],
+                             up=UP, fov=fov, light_mode=c["light_mode"], light=c["light"]))
+    return jobs
