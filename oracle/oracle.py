@@ -253,3 +253,85 @@ def _run_threads(fn, bands, with_index=False):
         t.start()
     for t in ts:
         t.join()
+
+
+class NTreeOracle:
+    """Brick n-tree volume and ray cast restated on the CPU (ntree_oracle.c)."""
+
+    class Desc(ctypes.Structure):
+        _fields_ = [("size", ctypes.c_int32), ("levels", ctypes.c_int32), ("octaves", ctypes.c_int32),
+                    ("frequency", ctypes.c_float), ("threshold", ctypes.c_float), ("scale", ctypes.c_float),
+                    ("seed", ctypes.c_uint64)]
+
+    def __init__(self, path=None):
+        path = pathlib.Path(path) if path else LIB
+        if not path.exists():
+            build(out=path)
+        l = ctypes.CDLL(str(path))
+        vp = ctypes.c_void_p
+        D = ctypes.POINTER(NTreeOracle.Desc)
+        l.nt_perm.argtypes = [ctypes.c_uint64, vp]
+        l.nt_density_box.argtypes = [D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, vp]
+        l.nt_occupancy_sample.argtypes = [D, ctypes.c_int64, ctypes.c_uint64]
+        l.nt_occupancy_sample.restype = ctypes.c_double
+        l.nt_build_dense.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp, vp, vp, ctypes.c_uint64, ctypes.c_uint64,
+                                     ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64)]
+        l.nt_render.argtypes = [vp, vp, vp, ctypes.c_int, ctypes.c_int, vp, vp, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_int, ctypes.c_int, vp, vp, vp, vp, vp]
+        l.oracle_plant.restype = ctypes.c_int
+        self.lib = l
+
+    @staticmethod
+    def desc(size, levels, octaves, frequency, threshold, scale, seed):
+        return NTreeOracle.Desc(size, levels, octaves, frequency, threshold, scale, seed)
+
+    def perm(self, seed):
+        p = np.zeros(512, np.int32)
+        self.lib.nt_perm(seed, p.ctypes.data)
+        return p
+
+    def density_box(self, d, x0, y0, z0, nx, ny, nz):
+        out = np.zeros((nz, ny, nx), np.uint8)
+        self.lib.nt_density_box(ctypes.byref(d), x0, y0, z0, nx, ny, nz, out.ctypes.data)
+        return out
+
+    def occupancy(self, d, n=200000, seed=1):
+        return self.lib.nt_occupancy_sample(ctypes.byref(d), n, seed)
+
+    def build_dense(self, dense, levels):
+        """dense[z, y, x] u8 -> (root[R^3] u32, nodes[n, 4] u32, bricks[m, 512] u8)."""
+        dense = np.ascontiguousarray(dense, np.uint8)
+        size = dense.shape[0]
+        nn, nb = ctypes.c_uint64(), ctypes.c_uint64()
+        rc = self.lib.nt_build_dense(dense.ctypes.data, size, levels, None, None, None, 0, 0, ctypes.byref(nn),
+                                     ctypes.byref(nb))
+        if rc != 0:
+            raise ValueError(f"bad n-tree shape: size {size}, levels {levels}")
+        R = (size // 8) >> (2 * levels)
+        root = np.zeros(R ** 3, np.uint32)
+        nodes = np.zeros((max(1, nn.value), 4), np.uint32)
+        bricks = np.zeros((max(1, nb.value), 512), np.uint8)
+        self.lib.nt_build_dense(dense.ctypes.data, size, levels, root.ctypes.data, nodes.ctypes.data,
+                                bricks.ctypes.data, nn.value, nb.value, ctypes.byref(nn), ctypes.byref(nb))
+        return root, nodes[:nn.value], bricks[:nb.value]
+
+    def render(self, root, nodes, bricks, size, levels, view, palette, w, h, ystep=1, threads=8):
+        """Outputs for rows 0, ystep, 2 ystep, ... (other rows stay zero)."""
+        root = np.ascontiguousarray(root, np.uint32)
+        nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
+        bricks = np.ascontiguousarray(bricks, np.uint8).reshape(-1)
+        view = np.ascontiguousarray(view, np.float32)
+        palette = np.ascontiguousarray(palette, np.uint32)
+        out = {"rgba": np.zeros((h, w), np.uint32), "t": np.zeros((h, w), np.float32),
+               "voxel": np.zeros((h, w, 3), np.int32), "nodes": np.zeros((h, w), np.uint32),
+               "voxels": np.zeros((h, w), np.uint32)}
+        self.lib.nt_render(root.ctypes.data, nodes.ctypes.data if nodes.size else None,
+                           bricks.ctypes.data if bricks.size else None, size, levels, view.ctypes.data,
+                           palette.ctypes.data, w, h, ystep, threads, out["rgba"].ctypes.data,
+                           out["t"].ctypes.data, out["voxel"].ctypes.data, out["nodes"].ctypes.data,
+                           out["voxels"].ctypes.data)
+        return out
+
+    def plant(self):
+        return self.lib.oracle_plant()
