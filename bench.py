@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--no-composite", action="store_true", help="skip the fwd+bwd compositing measurement")
     ap.add_argument("--composite-steps", type=int, default=8)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch measurement")
+    ap.add_argument("--no-brick", action="store_true", help="skip the brick n-tree measurement")
+    ap.add_argument("--brick-frames", type=int, default=8)
     ap.add_argument("--autotune", action="store_true",
                     help="time the headline workload with the autotuner's launch shape instead of the "
                          "library default (the committed traffic profile is for the default)")
@@ -151,6 +153,7 @@ def main():
         cpu = cpu_baseline(scene, cams, w, h, mode, wl.SUN, args.cpu_frames)
     fwd_bwd = None if args.no_composite else composite_bench(yx, torch, dev, args.composite_steps)
     batched = None if args.no_batch else batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
+    brick = None if args.no_brick else brick_bench(yx, torch, args.brick_frames, not args.no_cpu_baseline)
 
     line = {
         "metric": BASELINE_METRIC,
@@ -177,6 +180,7 @@ def main():
         "cpu_baseline": cpu,
         "fwd_bwd": fwd_bwd,
         "batched": batched,
+        "brick": brick,
     }
     print(json.dumps(line), flush=True)
 
@@ -320,6 +324,78 @@ def batch_bench(yx, torch, frames, tune_cache):
             "speedup_vs_per_scene_tuned": round(plain_tuned_ms / batch_ms, 3), "frames": frames,
             "graph": True, "tuning": best, "autotune_trials": len(trials), "autotune_s": round(tune_s, 1),
             "scene_bytes": b.scene_bytes()}
+
+
+def brick_bench(yx, torch, frames, with_cpu):
+    """brick_4096: 4096^3 voxels as an 8^3 root grid over three levels of 4^3
+    nodes and 8^3 u8 bricks (fBm density, ~3 % occupancy), built on the GPU;
+    1920x1080 primary rays per frame from `frames` orbit cameras, cast by the
+    n-tree DDA with front-to-back compositing. value = rays / mean kernel time
+    (HIP events on the launch stream). Algorithmic bytes per frame: 16 B per
+    node record read + 1 B per brick voxel read + 20 B of outputs per pixel
+    (RGBA8, t, voxel), the read counts taken from the kernel's own counters in
+    an untimed pass."""
+    wl = yx.workloads
+    cfg = wl.BRICK_4096
+    stream = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    tree = yx.BrickTree.build_fbm(stream=stream, **cfg)
+    torch.cuda.synchronize()
+    build_s = time.time() - t0
+    info = tree.info()
+    print(f"[bench] brick_4096 built in {build_s:.2f} s: {info}", file=sys.stderr, flush=True)
+    w, h = 1920, 1080
+    pal_np = wl.brick_palette()
+    pal = torch.from_numpy(pal_np.view(np.int32)).cuda()
+    views = yx.composite.camera_views(wl.orbit_cameras(frames, seed=2), 60.0, w, h)
+    out = tree.render_tensors(views[0], w, h, pal, outputs=("rgba", "t", "voxel", "nodes", "voxels"))
+    algo = []
+    reads = []
+    for v in views:
+        tree.render(v, w, h, pal, stream=stream, **out)
+        torch.cuda.synchronize()
+        nn = int(out["nodes"].to(torch.int64).sum().item())
+        nv = int(out["voxels"].to(torch.int64).sum().item())
+        algo.append(16 * nn + nv + 20 * w * h)
+        reads.append((nn, nv))
+    timed = {k: out[k] for k in ("rgba", "t", "voxel")}
+    tree.render(views[0], w, h, pal, stream=stream, **timed)
+    ev = []
+    for v in views:
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        tree.render(v, w, h, pal, stream=stream, **timed)
+        b.record(stream)
+        ev.append((a, b))
+    torch.cuda.synchronize()
+    ms = [a.elapsed_time(b) for a, b in ev]
+    kern_ms = float(np.mean(ms))
+    achieved = float(np.mean(algo)) / (kern_ms / 1e3) / 1e9
+    res = {"value": round(w * h / (kern_ms / 1e3) / 1e6, 3), "unit": "Mrays/s", "workload": "brick_4096",
+           "resolution": [w, h], "frames": frames, "ms_per_frame": round(kern_ms, 4),
+           "build_s": round(build_s, 3), "bricks": info["num_bricks"], "nodes": info["num_nodes"],
+           "tree_bytes": info["bytes"],
+           "mean_node_reads_per_ray": round(float(np.mean([r[0] for r in reads])) / (w * h), 3),
+           "mean_voxel_reads_per_ray": round(float(np.mean([r[1] for r in reads])) / (w * h), 3),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 5), "kernel": "nt_trace_kernel<3>",
+                        "algorithmic_bytes_per_launch": int(np.mean(algo))},
+           "cpu_baseline": None}
+    if with_cpu:
+        oracle = entry.load_oracle()
+        o = oracle.NTreeOracle()
+        root, nodes, bricks = tree.download()
+        threads = min(16, os.cpu_count() or 1)
+        ystep = 8
+        t0 = time.perf_counter()
+        o.render(root, nodes, bricks, cfg["size"], cfg["levels"], views[0], pal_np, w, h, ystep=ystep,
+                 threads=threads)
+        dt = time.perf_counter() - t0
+        rays = w * len(range(0, h, ystep))
+        res["cpu_baseline"] = {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                               "kind": "port", "sample": f"every {ystep}th row of orbit frame 0 ({rays} rays)"}
+    return res
 
 
 def measured_traffic(workload):

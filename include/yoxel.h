@@ -292,6 +292,53 @@ int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol, const floa
 /* Records emitted by the last backward (sum of nsamp) and distinct keys. */
 int yx_compositor_last_stats(const yx_compositor* c, uint64_t* records, uint64_t* unique_keys);
 
+/* ---- brick n-tree volumes ---------------------------------------------------
+ * The n-tree of ntree/ntree/nodes.h:13-100 (Grid nodes of 4^3 cells with the
+ * ch2ofs cell order (z*4 + y)*4 + x, :99; Brick leaves, ntree/BrickManager.h
+ * brick pool) with 8^3 u8 voxel bricks: size = 8 * 4^levels * R voxels per
+ * axis, an R^3 root grid (R <= 16) above `levels` levels of 4^3 nodes.
+ * Built on the GPU (MarkBricks / PackBricks, reaction/voxtree/density.cu:
+ * 131-221, as classify -> compact -> pack); device-resident. */
+typedef struct yx_ntree yx_ntree;
+/* u8 density from 3D fBm: f = sum_{o < octaves} 0.5^o noise3(p * frequency *
+ * 2^o), p = (voxel + 0.5) / size, improved Perlin noise over a permutation
+ * seeded by `seed`; density 0 where f < threshold, else 1 + min(254,
+ * (int)((f - threshold) * scale)); float32 throughout. */
+typedef struct yx_ntree_desc {
+    int32_t size, levels, octaves;
+    float frequency, threshold, scale;
+    uint64_t seed;
+} yx_ntree_desc;
+typedef struct yx_ntree_info {
+    int32_t size, levels, root_side;
+    uint64_t num_nodes, num_bricks, bytes; /* bytes: root + nodes + bricks in HBM */
+} yx_ntree_info;
+int yx_ntree_build_fbm(const yx_ntree_desc* desc, yx_ntree** out, void* stream);
+/* From a host dense volume (size^3 u8, x fastest) - RangeBuilder::build
+ * (ntree/ntree/ntree.h:11-63) over the whole range. */
+int yx_ntree_from_dense(const uint8_t* dense, int32_t size, int32_t levels, yx_ntree** out);
+int yx_ntree_get_info(const yx_ntree* t, yx_ntree_info* info);
+/* Host copies: root (R^3 u32: level-1 node / brick index, 0xFFFFFFFF empty),
+ * nodes (4 u32 each: child mask lo, hi, first child index, 0), bricks (512 B
+ * each, voxel x + 8y + 64z); any pointer may be NULL. */
+int yx_ntree_download(const yx_ntree* t, uint32_t* root, uint32_t* nodes, uint8_t* bricks);
+void yx_ntree_free(yx_ntree* t);
+/* Per-pixel outputs of the n-tree cast (device pointers, any may be NULL). */
+typedef struct yx_ntree_outputs {
+    uint32_t* rgba;   /* front-to-back composite, float2uint packing            */
+    float* t;         /* entry t of the first non-empty voxel, -1 if none       */
+    int32_t* voxel;   /* 3 x int32: that voxel, (-1,-1,-1) if none              */
+    uint32_t* nodes;  /* node records read                                      */
+    uint32_t* voxels; /* brick voxels read                                      */
+} yx_ntree_outputs;
+/* castRay (reaction/trace/trace.cu:83-139) over the n-tree: pinhole rays of
+ * `view` (eye position + InitRayDir basis dir0, du, dv as from
+ * yx_camera_basis), empty cells skipped at every level, brick voxels crossed
+ * by a 3D DDA and composited front to back through `palette` (256 RGBA8,
+ * device pointer; entry 0 unused) until alpha > 0.99. Asynchronous. */
+int yx_ntree_render(const yx_ntree* t, const float view[12], int32_t width, int32_t height,
+                    const uint32_t* palette, const yx_ntree_outputs* out, void* stream);
+
 const char* yx_last_error(void);
 const char* yx_version(void);
 

@@ -7,9 +7,10 @@ from ._lib import (YX_EMPTY_NODE, YX_FULL_NODE, YX_LIGHT_HEADLIGHT, YX_LIGHT_SUN
                    YX_LIGHT_SUN_SHADOW, YX_MISS, LIB_PATH, YoxelError, lib)
 from .renderer import SVOData, SVORenderer
 from .batch import BatchRenderer
+from .ntree import BrickTree
 from . import workloads
 from . import composite
 
-__all__ = ["SVOData", "SVORenderer", "BatchRenderer", "YoxelError", "lib", "workloads", "composite", "LIB_PATH",
+__all__ = ["SVOData", "SVORenderer", "BatchRenderer", "BrickTree", "YoxelError", "lib", "workloads", "composite", "LIB_PATH",
            "YX_EMPTY_NODE", "YX_FULL_NODE", "YX_MISS", "YX_LIGHT_HEADLIGHT", "YX_LIGHT_SUN",
            "YX_LIGHT_SUN_SHADOW"]

@@ -111,6 +111,22 @@ class BatchJob(ctypes.Structure):
     ]
 
 
+class NTreeDesc(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int32), ("levels", ctypes.c_int32), ("octaves", ctypes.c_int32),
+                ("frequency", ctypes.c_float), ("threshold", ctypes.c_float), ("scale", ctypes.c_float),
+                ("seed", ctypes.c_uint64)]
+
+
+class NTreeInfo(ctypes.Structure):
+    _fields_ = [("size", ctypes.c_int32), ("levels", ctypes.c_int32), ("root_side", ctypes.c_int32),
+                ("num_nodes", ctypes.c_uint64), ("num_bricks", ctypes.c_uint64), ("bytes", ctypes.c_uint64)]
+
+
+class NTreeOutputs(ctypes.Structure):
+    _fields_ = [("rgba", ctypes.c_void_p), ("t", ctypes.c_void_p), ("voxel", ctypes.c_void_p),
+                ("nodes", ctypes.c_void_p), ("voxels", ctypes.c_void_p)]
+
+
 class VolumeDesc(ctypes.Structure):
     _fields_ = [
         ("grid", ctypes.c_int32),
@@ -198,6 +214,13 @@ SIGNATURES = {
                                              ctypes.c_int32, _P, _P, _P, _P, _P]),
     "yx_compositor_last_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64),
                                                 ctypes.POINTER(ctypes.c_uint64)]),
+    "yx_ntree_build_fbm": (ctypes.c_int, [ctypes.POINTER(NTreeDesc), ctypes.POINTER(_P), _P]),
+    "yx_ntree_from_dense": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_P)]),
+    "yx_ntree_get_info": (ctypes.c_int, [_P, ctypes.POINTER(NTreeInfo)]),
+    "yx_ntree_download": (ctypes.c_int, [_P, _P, _P, _P]),
+    "yx_ntree_free": (None, [_P]),
+    "yx_ntree_render": (ctypes.c_int, [_P, _F3, ctypes.c_int32, ctypes.c_int32, _P, ctypes.POINTER(NTreeOutputs),
+                                       _P]),
     "yx_last_error": (ctypes.c_char_p, []),
     "yx_version": (ctypes.c_char_p, []),
 }
