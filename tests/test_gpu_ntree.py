@@ -6,7 +6,7 @@ checked on sampled bricks and the cast on every 8th row of a 1080p frame."""
 import numpy as np
 import pytest
 
-from test_ntree_oracle import palette as test_palette
+from test_ntree_oracle import palette as make_palette
 
 pytestmark = pytest.mark.gpu
 
@@ -34,7 +34,7 @@ def assert_cast_equal(g, o, rows=None, what=""):
 
 def views_for(yx, w, h, seed=0):
     cams = yx.workloads.orbit_cameras(3, seed=seed) + [((0.41, 0.37, 0.55), (0.4, 1.0, -0.3)),
-                                                         ((0.5, 0.5, 2.5), (0.0, 0.0, -1.0))]
+                                                         ((0.5, 0.5, 2.5), (0.0, 1e-3, -1.0))]
     return yx.composite.camera_views(cams, 60.0, w, h)
 
 
@@ -49,7 +49,7 @@ def test_dense_build_and_cast(yx, oracle_mod, cuda, size, levels):
     assert (info["num_nodes"], info["num_bricks"]) == (len(nodes), len(bricks))
     g_root, g_nodes, g_bricks = t.download()
     assert np.array_equal(g_root, root) and np.array_equal(g_nodes, nodes) and np.array_equal(g_bricks, bricks)
-    pal = test_palette(size)
+    pal = make_palette(size)
     w, h = 67, 45
     for i, view in enumerate(views_for(yx, w, h, seed=size)):
         print(f"n-tree {size}/{levels} view {i}", flush=True)
