@@ -39,6 +39,8 @@ def parse():
     ap.add_argument("--composite-steps", type=int, default=8)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch measurement")
     ap.add_argument("--no-brick", action="store_true", help="skip the brick n-tree measurement")
+    ap.add_argument("--only", choices=["composite", "batch", "brick"], default=None,
+                    help="run one secondary workload alone and print its object (profiling)")
     ap.add_argument("--brick-frames", type=int, default=8)
     ap.add_argument("--autotune", action="store_true",
                     help="time the headline workload with the autotuner's launch shape instead of the "
@@ -62,6 +64,18 @@ def main():
 
     yx = entry.load_package()
     wl = yx.workloads
+    if args.only:
+        # one secondary workload alone (for profiling); prints that leg's object
+        dev = torch.device("cuda", 0)
+        torch.cuda.set_device(dev)
+        if args.only == "composite":
+            leg = composite_bench(yx, torch, dev, args.composite_steps)
+        elif args.only == "batch":
+            leg = batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
+        else:
+            leg = brick_bench(yx, torch, args.brick_frames, not args.no_cpu_baseline)
+        print(json.dumps(leg), flush=True)
+        return
     if args.workload == "large_1080p":
         desc, w, h, nframes, mode = wl.LARGE_1080P, 1920, 1080, 32, yx.YX_LIGHT_SUN_SHADOW
     else:
