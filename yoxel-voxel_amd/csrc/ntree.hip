@@ -279,115 +279,162 @@ __global__ __launch_bounds__(256) void nt_trace_kernel(NtArgs a) {
         int vy = min(max((int)floorf((oy + te * dy) * nf), 0), N - 1);
         int vz = min(max((int)floorf((oz + te * dz) * nf), 0), N - 1);
         const int sx = dx > 0.0f ? 1 : -1, sy = dy > 0.0f ? 1 : -1, sz = dz > 0.0f ? 1 : -1;
+        const int ux = sx > 0 ? 1 : 0, uy = sy > 0 ? 1 : 0, uz = sz > 0 ? 1 : 0;
+        constexpr int sr = 3 + 2 * L;  // log2 of the root cell side
         float t = te;
         bool first = true;
-        bool bk_valid = false;
-        int bkx = 0, bky = 0, bkz = 0;
-        uint32_t bk_idx = 0;
         bool nc_valid[L > 0 ? L : 1];
         int nck[L > 0 ? L : 1][3];
         uint32_t ncm_lo[L > 0 ? L : 1], ncm_hi[L > 0 ? L : 1], ncb[L > 0 ? L : 1];
 #pragma unroll
         for (int l = 0; l < L; ++l) nc_valid[l] = false;
-        for (int guard = 0; guard < 4 * N; ++guard) {
-            int S = N / R;
-            uint32_t idx;
-            if (bk_valid && (vx >> 3) == bkx && (vy >> 3) == bky && (vz >> 3) == bkz) {
-                idx = bk_idx;
-                S = 8;
-            } else {
-                bk_valid = false;
-                constexpr int sr = 3 + 2 * L;
-                idx = s_root[((vz >> sr) * R + (vy >> sr)) * R + (vx >> sr)];
+        // One flat loop: a lane outside a brick looks its voxel up from the
+        // root (node records re-read only when the voxel has left that level's
+        // node) and either leaves an empty cell in one step or enters the
+        // brick; a lane inside a brick crosses up to four of its voxels.
+        bool inb = false;
+        const uint8_t* brick = a.bricks;
+        int bx = 0, by = 0, bz = 0;
+        bool end = false;
+        for (int guard = 0; guard < 4 * N && !end; ++guard) {
+          if (!inb) {
+            int lS = sr;  // log2 of the side of the cell reached
+            uint32_t idx = s_root[((vz >> sr) * R + (vy >> sr)) * R + (vx >> sr)];
 #pragma unroll
-                for (int l = 1; l <= L; ++l) {
-                    if (idx != kNtEmpty) {
-                        const int sn = 3 + 2 * (L - l + 1);
-                        const int kx = vx >> sn, ky = vy >> sn, kz = vz >> sn;
-                        if (!(nc_valid[l - 1] && nck[l - 1][0] == kx && nck[l - 1][1] == ky && nck[l - 1][2] == kz)) {
-                            const uint4 nd = a.nodes[idx];
-                            ++nn;
-                            ncm_lo[l - 1] = nd.x;
-                            ncm_hi[l - 1] = nd.y;
-                            ncb[l - 1] = nd.z;
-                            nck[l - 1][0] = kx;
-                            nck[l - 1][1] = ky;
-                            nck[l - 1][2] = kz;
-                            nc_valid[l - 1] = true;
-                        }
-                        S >>= 2;
-                        const int s = sn - 2;
-                        const int c = ((((vz >> s) & 3) * 4) + ((vy >> s) & 3)) * 4 + ((vx >> s) & 3);
-                        const uint32_t mlo = ncm_lo[l - 1], mhi = ncm_hi[l - 1];
-                        const bool set = c < 32 ? ((mlo >> c) & 1u) : ((mhi >> (c - 32)) & 1u);
-                        if (!set) {
-                            idx = kNtEmpty;
-                        } else {
-                            const uint32_t below = c < 32 ? __builtin_popcount(mlo & ((1u << c) - 1u))
-                                                          : __builtin_popcount(mlo) +
-                                                                __builtin_popcount(mhi & ((1u << (c - 32)) - 1u));
-                            idx = ncb[l - 1] + below;
-                        }
-                    }
-                }
+            for (int l = 1; l <= L; ++l) {
                 if (idx != kNtEmpty) {
-                    bk_valid = true;
-                    bkx = vx >> 3;
-                    bky = vy >> 3;
-                    bkz = vz >> 3;
-                    bk_idx = idx;
-                }
-            }
-            int cs;
-            if (idx == kNtEmpty) {
-                cs = S;
-            } else {
-                cs = 1;
-                const uint32_t dd = a.bricks[(size_t)idx * 512u + (uint32_t)((vx & 7) + 8 * (vy & 7) + 64 * (vz & 7))];
-                ++nv;
-                if (dd) {
-                    const uint32_t p = s_pal[dd];
-                    const float k = 1.0f / 255.0f;
-                    float cr = (float)(p & 255u) * k, cg = (float)((p >> 8) & 255u) * k;
-                    float cb = (float)((p >> 16) & 255u) * k;
-                    const float ca = (float)(p >> 24) * k;
-                    cr = cr * ca;
-                    cg = cg * ca;
-                    cb = cb * ca;
-                    const float s1 = 1.0f - aa;
-                    ar = ar + cr * s1;
-                    ag = ag + cg * s1;
-                    ab = ab + cb * s1;
-                    aa = aa + ca * s1;
-                    if (first) {
-                        first = false;
-                        thit = t;
-                        hx = vx;
-                        hy = vy;
-                        hz = vz;
+                    const int sn = 3 + 2 * (L - l + 1);
+                    const int kx = vx >> sn, ky = vy >> sn, kz = vz >> sn;
+                    if (!(nc_valid[l - 1] && nck[l - 1][0] == kx && nck[l - 1][1] == ky && nck[l - 1][2] == kz)) {
+                        const uint4 nd = a.nodes[idx];
+                        ++nn;
+                        ncm_lo[l - 1] = nd.x;
+                        ncm_hi[l - 1] = nd.y;
+                        ncb[l - 1] = nd.z;
+                        nck[l - 1][0] = kx;
+                        nck[l - 1][1] = ky;
+                        nck[l - 1][2] = kz;
+                        nc_valid[l - 1] = true;
                     }
-                    if (aa > 0.99f) break;
+                    lS -= 2;
+                    const int s = sn - 2;
+                    const int c = ((((vz >> s) & 3) * 4) + ((vy >> s) & 3)) * 4 + ((vx >> s) & 3);
+                    const uint32_t mlo = ncm_lo[l - 1], mhi = ncm_hi[l - 1];
+                    const bool set = c < 32 ? ((mlo >> c) & 1u) : ((mhi >> (c - 32)) & 1u);
+                    if (!set) {
+                        idx = kNtEmpty;
+                    } else {
+                        const uint32_t below = c < 32 ? __builtin_popcount(mlo & ((1u << c) - 1u))
+                                                      : __builtin_popcount(mlo) +
+                                                            __builtin_popcount(mhi & ((1u << (c - 32)) - 1u));
+                        idx = ncb[l - 1] + below;
+                    }
                 }
             }
-            // leave the cell of side cs containing v
-            const int cx = vx / cs, cy = vy / cs, cz = vz / cs;
-            const float tX = ((float)((cx + (sx > 0 ? 1 : 0)) * cs) * invN - ox) * ix;
-            const float tY = ((float)((cy + (sy > 0 ? 1 : 0)) * cs) * invN - oy) * iy;
-            const float tZ = ((float)((cz + (sz > 0 ? 1 : 0)) * cs) * invN - oz) * iz;
-            const int ax = (tX <= tY && tX <= tZ) ? 0 : (tY <= tZ ? 1 : 2);
-            t = pick3(ax, tX, tY, tZ);
-            if (t > tx_) break;
-            const int nx = ax == 0 ? (sx > 0 ? (cx + 1) * cs : cx * cs - 1)
-                                   : nt_follow(ox, dx, t, nf, vx, sx, cx * cs, cs);
-            const int ny = ax == 1 ? (sy > 0 ? (cy + 1) * cs : cy * cs - 1)
-                                   : nt_follow(oy, dy, t, nf, vy, sy, cy * cs, cs);
-            const int nz = ax == 2 ? (sz > 0 ? (cz + 1) * cs : cz * cs - 1)
-                                   : nt_follow(oz, dz, t, nf, vz, sz, cz * cs, cs);
-            const int na = pick3(ax, nx, ny, nz);
-            if (na < 0 || na >= N) break;
-            vx = nx;
-            vy = ny;
-            vz = nz;
+            if (idx == kNtEmpty) {
+                // leave the empty cell of side 2^lS in one step
+                const int cx = vx >> lS, cy = vy >> lS, cz = vz >> lS;
+                const float tX = ((float)((cx + ux) << lS) * invN - ox) * ix;
+                const float tY = ((float)((cy + uy) << lS) * invN - oy) * iy;
+                const float tZ = ((float)((cz + uz) << lS) * invN - oz) * iz;
+                const int ax = (tX <= tY && tX <= tZ) ? 0 : (tY <= tZ ? 1 : 2);
+                t = pick3(ax, tX, tY, tZ);
+                if (t > tx_) break;
+                const int cs = 1 << lS;
+                const int nx = ax == 0 ? (sx > 0 ? (cx + 1) << lS : (cx << lS) - 1)
+                                       : nt_follow(ox, dx, t, nf, vx, sx, cx << lS, cs);
+                const int ny = ax == 1 ? (sy > 0 ? (cy + 1) << lS : (cy << lS) - 1)
+                                       : nt_follow(oy, dy, t, nf, vy, sy, cy << lS, cs);
+                const int nz = ax == 2 ? (sz > 0 ? (cz + 1) << lS : (cz << lS) - 1)
+                                       : nt_follow(oz, dz, t, nf, vz, sz, cz << lS, cs);
+                const int na = pick3(ax, nx, ny, nz);
+                if (na < 0 || na >= N) break;
+                vx = nx;
+                vy = ny;
+                vz = nz;
+            } else {
+                inb = true;
+                brick = a.bricks + (size_t)idx * 512u;
+                bx = vx >> 3;
+                by = vy >> 3;
+                bz = vz >> 3;
+            }
+          }
+          if (inb) {
+            // cross up to four voxels of the brick (a voxel step moves only the
+            // exit axis); the positions do not depend on the data, so the four
+            // loads are issued together and composited in order
+            {
+                int px[4], py[4], pz[4];
+                float pt[4];
+                bool valid[4];
+                bool more = true;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    valid[j] = more;
+                    px[j] = vx;
+                    py[j] = vy;
+                    pz[j] = vz;
+                    pt[j] = t;
+                    if (more) {
+                        const float tX = ((float)(vx + ux) * invN - ox) * ix;
+                        const float tY = ((float)(vy + uy) * invN - oy) * iy;
+                        const float tZ = ((float)(vz + uz) * invN - oz) * iz;
+                        const int ax = (tX <= tY && tX <= tZ) ? 0 : (tY <= tZ ? 1 : 2);
+                        const float tn = pick3(ax, tX, tY, tZ);
+                        const int na = pick3(ax, vx + sx, vy + sy, vz + sz);
+                        if (tn > tx_ || na < 0 || na >= N) {
+                            end = true;
+                            more = false;
+                        } else {
+                            t = tn;
+                            vx = ax == 0 ? na : vx;
+                            vy = ax == 1 ? na : vy;
+                            vz = ax == 2 ? na : vz;
+                            if ((vx >> 3) != bx || (vy >> 3) != by || (vz >> 3) != bz) more = false;
+                        }
+                    }
+                }
+                uint32_t dd[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    dd[j] = valid[j] ? brick[(px[j] & 7) + 8 * (py[j] & 7) + 64 * (pz[j] & 7)] : 0u;
+                bool stop = false;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (valid[j] && !stop) {
+                        ++nv;
+                        if (dd[j]) {
+                            const uint32_t p = s_pal[dd[j]];
+                            const float k = 1.0f / 255.0f;
+                            float cr = (float)(p & 255u) * k, cg = (float)((p >> 8) & 255u) * k;
+                            float cb = (float)((p >> 16) & 255u) * k;
+                            const float ca = (float)(p >> 24) * k;
+                            cr = cr * ca;
+                            cg = cg * ca;
+                            cb = cb * ca;
+                            const float s1 = 1.0f - aa;
+                            ar = ar + cr * s1;
+                            ag = ag + cg * s1;
+                            ab = ab + cb * s1;
+                            aa = aa + ca * s1;
+                            if (first) {
+                                first = false;
+                                thit = pt[j];
+                                hx = px[j];
+                                hy = py[j];
+                                hz = pz[j];
+                            }
+                            if (aa > 0.99f) {
+                                stop = true;
+                                end = true;
+                            }
+                        }
+                    }
+                }
+                if (!more) inb = false;  // left the brick (or the ray ended)
+            }
+          }
         }
     }
     if (a.rgba)
