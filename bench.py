@@ -439,9 +439,30 @@ def cpu_baseline(scene, cams, w, h, mode, sun, nframes):
         o = orc.render(nodes, root, pos, d, fov=60.0, w=w, h=h, light_mode=mode, light=sun, threads=threads)
         rays += o["rays"]
     dt = time.perf_counter() - t0
+    # SimpleRenderer (1 thread) and TreadedRenderer's hard-coded 4 threads
+    # (cell/ppu_renderer.cpp:129) on one frame each
+    by_threads = {}
+    for th in (1, 4):
+        pos, d = cams[int(idx[0])]
+        t1 = time.perf_counter()
+        o = orc.render(nodes, root, pos, d, fov=60.0, w=w, h=h, light_mode=mode, light=sun, threads=th)
+        by_threads[str(th)] = round(o["rays"] / (time.perf_counter() - t1) / 1e6, 4)
+    by_threads[str(threads)] = round(rays / dt / 1e6, 4)
     return {"value": round(rays / dt / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{len(idx)} of the {len(cams)} orbit frames (frames {idx.tolist()}), full "
-                      f"{w}x{h}, {rays} rays, {dt:.1f} s wall on {threads} threads"}
+                      f"{w}x{h}, {rays} rays, {dt:.1f} s wall on {threads} threads; 1 and 4 threads on "
+                      f"frame {int(idx[0])}",
+            "by_threads": by_threads, "cpu": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 if __name__ == "__main__":
