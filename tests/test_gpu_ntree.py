@@ -51,11 +51,14 @@ def test_dense_build_and_cast(yx, oracle_mod, cuda, size, levels):
     assert np.array_equal(g_root, root) and np.array_equal(g_nodes, nodes) and np.array_equal(g_bricks, bricks)
     pal = make_palette(size)
     w, h = 67, 45
+    launches = [(True, 16), (False, 16), (True, 1), (True, 64)]
     for i, view in enumerate(views_for(yx, w, h, seed=size)):
-        print(f"n-tree {size}/{levels} view {i}", flush=True)
-        g = gpu_cast(cuda, t, view, w, h, pal)
         c = o.render(root, nodes, bricks, size, levels, view, pal, w, h)
-        assert_cast_equal(g, c, what=f"size {size} levels {levels} view {i}")
+        for persistent, refill in launches:
+            print(f"n-tree {size}/{levels} view {i} persistent {persistent} refill {refill}", flush=True)
+            t.set_launch(persistent, refill)
+            g = gpu_cast(cuda, t, view, w, h, pal)
+            assert_cast_equal(g, c, what=f"size {size} levels {levels} view {i} launch {persistent}/{refill}")
         assert (g["t"] >= 0).any()
 
 

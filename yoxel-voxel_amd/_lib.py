@@ -219,6 +219,7 @@ SIGNATURES = {
     "yx_ntree_get_info": (ctypes.c_int, [_P, ctypes.POINTER(NTreeInfo)]),
     "yx_ntree_download": (ctypes.c_int, [_P, _P, _P, _P]),
     "yx_ntree_free": (None, [_P]),
+    "yx_ntree_set_launch": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
     "yx_ntree_render": (ctypes.c_int, [_P, _F3, ctypes.c_int32, ctypes.c_int32, _P, ctypes.POINTER(NTreeOutputs),
                                        _P]),
     "yx_last_error": (ctypes.c_char_p, []),

@@ -31,6 +31,7 @@
 
 #include "internal.h"
 #include "trace_device.h"
+#include "trace_persist.h"
 
 namespace yx {
 
@@ -208,6 +209,11 @@ struct NtArgs {
     int32_t* voxel;
     uint32_t* n_nodes;
     uint32_t* n_voxels;
+    // persistent launch
+    uint32_t* counters;  // [0]: next ray id, [1]: retired waves
+    uint32_t total;      // rays in whole 8x8 tiles
+    uint32_t n_waves;
+    int refill_min;
 };
 
 __device__ __forceinline__ float nt_clamp_eps(float x) {
@@ -228,24 +234,26 @@ __device__ __forceinline__ int nt_follow(float o, float d, float t, float nf, in
     return min(max(q, lo), lo + cs - 1);
 }
 
+// State of one ray of the n-tree cast.
 template <int L>
-__global__ __launch_bounds__(256) void nt_trace_kernel(NtArgs a) {
-    __shared__ uint32_t s_root[kNtMaxRoot * kNtMaxRoot * kNtMaxRoot];
-    __shared__ uint32_t s_pal[256];
-    const int nroot = a.R * a.R * a.R;
-    for (int i = threadIdx.x; i < nroot; i += 256) s_root[i] = a.root[i];
-    s_pal[threadIdx.x] = a.palette[threadIdx.x];
-    __syncthreads();
+struct NtRay {
+    float ox, oy, oz, dx, dy, dz, ix, iy, iz, tx, t;
+    int vx, vy, vz, guard;
+    float ar, ag, ab, aa, thit;
+    int hx, hy, hz;
+    uint32_t nn, nv;
+    bool first, inb;
+    const uint8_t* brick;
+    int bx, by, bz;
+    bool nc_valid[L > 0 ? L : 1];
+    int nck[L > 0 ? L : 1][3];
+    uint32_t ncm_lo[L > 0 ? L : 1], ncm_hi[L > 0 ? L : 1], ncb[L > 0 ? L : 1];
+};
 
-    // wave -> 8x8 pixel tile (row-major tiles), lane -> pixel in the tile
-    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
-    const int lane = (int)(threadIdx.x & 63);
-    const int tiles_x = (a.w + 7) >> 3;
-    const int tx = (int)(wave % (uint32_t)tiles_x), ty = (int)(wave / (uint32_t)tiles_x);
-    const int x = tx * 8 + (lane & 7), y = ty * 8 + (lane >> 3);
-    if (y >= a.h || x >= a.w) return;
-    const size_t pix = (size_t)y * a.w + x;
-
+// Ray of pixel (x, y); returns false when it misses the unit cube (the
+// outputs then stay those of an empty ray).
+template <int L>
+__device__ __forceinline__ bool nt_setup(const NtArgs& a, int x, int y, NtRay<L>& r) {
     const float* b = a.view + 3;
     float dx = b[0] + b[3] * (float)x + b[6] * (float)y;
     float dy = b[1] + b[4] * (float)x + b[7] * (float)y;
@@ -255,199 +263,300 @@ __global__ __launch_bounds__(256) void nt_trace_kernel(NtArgs a) {
     n2 += dy * dy;
     n2 += dz * dz;
     const float nrm = sqrtf(n2);
-    dx = nt_clamp_eps(dx / nrm);
-    dy = nt_clamp_eps(dy / nrm);
-    dz = nt_clamp_eps(dz / nrm);
-    const float ox = a.view[0], oy = a.view[1], oz = a.view[2];
-    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-
-    uint32_t nn = 0, nv = 0;
-    float thit = -1.0f;
-    int hx = -1, hy = -1, hz = -1;
-    float ar = 0.0f, ag = 0.0f, ab = 0.0f, aa = 0.0f;
-
-    const float lox = (0.0f - ox) * ix, hix = (1.0f - ox) * ix;
-    const float loy = (0.0f - oy) * iy, hiy = (1.0f - oy) * iy;
-    const float loz = (0.0f - oz) * iz, hiz = (1.0f - oz) * iz;
+    r.dx = nt_clamp_eps(dx / nrm);
+    r.dy = nt_clamp_eps(dy / nrm);
+    r.dz = nt_clamp_eps(dz / nrm);
+    r.ox = a.view[0];
+    r.oy = a.view[1];
+    r.oz = a.view[2];
+    r.ix = 1.0f / r.dx;
+    r.iy = 1.0f / r.dy;
+    r.iz = 1.0f / r.dz;
+    r.nn = r.nv = 0;
+    r.thit = -1.0f;
+    r.hx = r.hy = r.hz = -1;
+    r.ar = r.ag = r.ab = r.aa = 0.0f;
+    r.first = true;
+    r.inb = false;
+    r.brick = a.bricks;
+    r.bx = r.by = r.bz = 0;
+    r.guard = 0;
+#pragma unroll
+    for (int l = 0; l < (L > 0 ? L : 1); ++l) r.nc_valid[l] = false;
+    const float lox = (0.0f - r.ox) * r.ix, hix = (1.0f - r.ox) * r.ix;
+    const float loy = (0.0f - r.oy) * r.iy, hiy = (1.0f - r.oy) * r.iy;
+    const float loz = (0.0f - r.oz) * r.iz, hiz = (1.0f - r.oz) * r.iz;
     float te = fmaxf(fmaxf(fmaxf(-INFINITY, fminf(lox, hix)), fminf(loy, hiy)), fminf(loz, hiz));
-    const float tx_ = fminf(fminf(fminf(INFINITY, fmaxf(lox, hix)), fmaxf(loy, hiy)), fmaxf(loz, hiz));
+    r.tx = fminf(fminf(fminf(INFINITY, fmaxf(lox, hix)), fmaxf(loy, hiy)), fmaxf(loz, hiz));
     te = fmaxf(te, 0.0f);
-    if (!(tx_ < 0.0f || te > tx_)) {
-        const int N = a.size, R = a.R;
-        const float nf = (float)N, invN = 1.0f / nf;
-        int vx = min(max((int)floorf((ox + te * dx) * nf), 0), N - 1);
-        int vy = min(max((int)floorf((oy + te * dy) * nf), 0), N - 1);
-        int vz = min(max((int)floorf((oz + te * dz) * nf), 0), N - 1);
-        const int sx = dx > 0.0f ? 1 : -1, sy = dy > 0.0f ? 1 : -1, sz = dz > 0.0f ? 1 : -1;
-        const int ux = sx > 0 ? 1 : 0, uy = sy > 0 ? 1 : 0, uz = sz > 0 ? 1 : 0;
-        constexpr int sr = 3 + 2 * L;  // log2 of the root cell side
-        float t = te;
-        bool first = true;
-        bool nc_valid[L > 0 ? L : 1];
-        int nck[L > 0 ? L : 1][3];
-        uint32_t ncm_lo[L > 0 ? L : 1], ncm_hi[L > 0 ? L : 1], ncb[L > 0 ? L : 1];
+    if (r.tx < 0.0f || te > r.tx) return false;
+    const int N = a.size;
+    const float nf = (float)N;
+    r.vx = min(max((int)floorf((r.ox + te * r.dx) * nf), 0), N - 1);
+    r.vy = min(max((int)floorf((r.oy + te * r.dy) * nf), 0), N - 1);
+    r.vz = min(max((int)floorf((r.oz + te * r.dz) * nf), 0), N - 1);
+    r.t = te;
+    return true;
+}
+
+// One iteration of the walk: a ray outside a brick looks its voxel up from
+// the root (node records re-read only when the voxel has left that level's
+// node) and either leaves an empty cell in one step or enters the brick; a
+// ray inside a brick crosses up to four of its voxels (a voxel step moves
+// only the exit axis; the four positions do not depend on the data, so their
+// loads are issued together and composited in order). Returns true when the
+// ray is finished.
+template <int L>
+__device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_root, const uint32_t* s_pal,
+                                           NtRay<L>& r) {
+    const int N = a.size, R = a.R;
+    const float nf = (float)N, invN = 1.0f / nf;
+    const int sx = r.dx > 0.0f ? 1 : -1, sy = r.dy > 0.0f ? 1 : -1, sz = r.dz > 0.0f ? 1 : -1;
+    const int ux = sx > 0 ? 1 : 0, uy = sy > 0 ? 1 : 0, uz = sz > 0 ? 1 : 0;
+    constexpr int sr = 3 + 2 * L;  // log2 of the root cell side
+    if (++r.guard > 4 * N) return true;
+    if (!r.inb) {
+        int lS = sr;  // log2 of the side of the cell reached
+        uint32_t idx = s_root[((r.vz >> sr) * R + (r.vy >> sr)) * R + (r.vx >> sr)];
 #pragma unroll
-        for (int l = 0; l < L; ++l) nc_valid[l] = false;
-        // One flat loop: a lane outside a brick looks its voxel up from the
-        // root (node records re-read only when the voxel has left that level's
-        // node) and either leaves an empty cell in one step or enters the
-        // brick; a lane inside a brick crosses up to four of its voxels.
-        bool inb = false;
-        const uint8_t* brick = a.bricks;
-        int bx = 0, by = 0, bz = 0;
-        bool end = false;
-        for (int guard = 0; guard < 4 * N && !end; ++guard) {
-          if (!inb) {
-            int lS = sr;  // log2 of the side of the cell reached
-            uint32_t idx = s_root[((vz >> sr) * R + (vy >> sr)) * R + (vx >> sr)];
-#pragma unroll
-            for (int l = 1; l <= L; ++l) {
-                if (idx != kNtEmpty) {
-                    const int sn = 3 + 2 * (L - l + 1);
-                    const int kx = vx >> sn, ky = vy >> sn, kz = vz >> sn;
-                    if (!(nc_valid[l - 1] && nck[l - 1][0] == kx && nck[l - 1][1] == ky && nck[l - 1][2] == kz)) {
-                        const uint4 nd = a.nodes[idx];
-                        ++nn;
-                        ncm_lo[l - 1] = nd.x;
-                        ncm_hi[l - 1] = nd.y;
-                        ncb[l - 1] = nd.z;
-                        nck[l - 1][0] = kx;
-                        nck[l - 1][1] = ky;
-                        nck[l - 1][2] = kz;
-                        nc_valid[l - 1] = true;
-                    }
-                    lS -= 2;
-                    const int s = sn - 2;
-                    const int c = ((((vz >> s) & 3) * 4) + ((vy >> s) & 3)) * 4 + ((vx >> s) & 3);
-                    const uint32_t mlo = ncm_lo[l - 1], mhi = ncm_hi[l - 1];
-                    const bool set = c < 32 ? ((mlo >> c) & 1u) : ((mhi >> (c - 32)) & 1u);
-                    if (!set) {
-                        idx = kNtEmpty;
-                    } else {
-                        const uint32_t below = c < 32 ? __builtin_popcount(mlo & ((1u << c) - 1u))
-                                                      : __builtin_popcount(mlo) +
-                                                            __builtin_popcount(mhi & ((1u << (c - 32)) - 1u));
-                        idx = ncb[l - 1] + below;
-                    }
+        for (int l = 1; l <= L; ++l) {
+            if (idx != kNtEmpty) {
+                const int sn = 3 + 2 * (L - l + 1);
+                const int kx = r.vx >> sn, ky = r.vy >> sn, kz = r.vz >> sn;
+                if (!(r.nc_valid[l - 1] && r.nck[l - 1][0] == kx && r.nck[l - 1][1] == ky && r.nck[l - 1][2] == kz)) {
+                    const uint4 nd = a.nodes[idx];
+                    ++r.nn;
+                    r.ncm_lo[l - 1] = nd.x;
+                    r.ncm_hi[l - 1] = nd.y;
+                    r.ncb[l - 1] = nd.z;
+                    r.nck[l - 1][0] = kx;
+                    r.nck[l - 1][1] = ky;
+                    r.nck[l - 1][2] = kz;
+                    r.nc_valid[l - 1] = true;
+                }
+                lS -= 2;
+                const int s = sn - 2;
+                const int c = ((((r.vz >> s) & 3) * 4) + ((r.vy >> s) & 3)) * 4 + ((r.vx >> s) & 3);
+                const uint32_t mlo = r.ncm_lo[l - 1], mhi = r.ncm_hi[l - 1];
+                const bool set = c < 32 ? ((mlo >> c) & 1u) : ((mhi >> (c - 32)) & 1u);
+                if (!set) {
+                    idx = kNtEmpty;
+                } else {
+                    const uint32_t below = c < 32 ? __builtin_popcount(mlo & ((1u << c) - 1u))
+                                                  : __builtin_popcount(mlo) +
+                                                        __builtin_popcount(mhi & ((1u << (c - 32)) - 1u));
+                    idx = r.ncb[l - 1] + below;
                 }
             }
-            if (idx == kNtEmpty) {
-                // leave the empty cell of side 2^lS in one step
-                const int cx = vx >> lS, cy = vy >> lS, cz = vz >> lS;
-                const float tX = ((float)((cx + ux) << lS) * invN - ox) * ix;
-                const float tY = ((float)((cy + uy) << lS) * invN - oy) * iy;
-                const float tZ = ((float)((cz + uz) << lS) * invN - oz) * iz;
-                const int ax = (tX <= tY && tX <= tZ) ? 0 : (tY <= tZ ? 1 : 2);
-                t = pick3(ax, tX, tY, tZ);
-                if (t > tx_) break;
-                const int cs = 1 << lS;
-                const int nx = ax == 0 ? (sx > 0 ? (cx + 1) << lS : (cx << lS) - 1)
-                                       : nt_follow(ox, dx, t, nf, vx, sx, cx << lS, cs);
-                const int ny = ax == 1 ? (sy > 0 ? (cy + 1) << lS : (cy << lS) - 1)
-                                       : nt_follow(oy, dy, t, nf, vy, sy, cy << lS, cs);
-                const int nz = ax == 2 ? (sz > 0 ? (cz + 1) << lS : (cz << lS) - 1)
-                                       : nt_follow(oz, dz, t, nf, vz, sz, cz << lS, cs);
-                const int na = pick3(ax, nx, ny, nz);
-                if (na < 0 || na >= N) break;
-                vx = nx;
-                vy = ny;
-                vz = nz;
+        }
+        if (idx == kNtEmpty) {
+            // leave the empty cell of side 2^lS in one step
+            const int cx = r.vx >> lS, cy = r.vy >> lS, cz = r.vz >> lS;
+            const float tX = ((float)((cx + ux) << lS) * invN - r.ox) * r.ix;
+            const float tY = ((float)((cy + uy) << lS) * invN - r.oy) * r.iy;
+            const float tZ = ((float)((cz + uz) << lS) * invN - r.oz) * r.iz;
+            const int ax = (tX <= tY && tX <= tZ) ? 0 : (tY <= tZ ? 1 : 2);
+            const float tn = pick3(ax, tX, tY, tZ);
+            if (tn > r.tx) return true;
+            const int cs = 1 << lS;
+            const int nx = ax == 0 ? (sx > 0 ? (cx + 1) << lS : (cx << lS) - 1)
+                                   : nt_follow(r.ox, r.dx, tn, nf, r.vx, sx, cx << lS, cs);
+            const int ny = ax == 1 ? (sy > 0 ? (cy + 1) << lS : (cy << lS) - 1)
+                                   : nt_follow(r.oy, r.dy, tn, nf, r.vy, sy, cy << lS, cs);
+            const int nz = ax == 2 ? (sz > 0 ? (cz + 1) << lS : (cz << lS) - 1)
+                                   : nt_follow(r.oz, r.dz, tn, nf, r.vz, sz, cz << lS, cs);
+            const int na = pick3(ax, nx, ny, nz);
+            if (na < 0 || na >= N) return true;
+            r.t = tn;
+            r.vx = nx;
+            r.vy = ny;
+            r.vz = nz;
+            return false;
+        }
+        r.inb = true;
+        r.brick = a.bricks + (size_t)idx * 512u;
+        r.bx = r.vx >> 3;
+        r.by = r.vy >> 3;
+        r.bz = r.vz >> 3;
+    }
+    int px[4], py[4], pz[4];
+    float pt[4];
+    bool valid[4];
+    bool more = true, end = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        valid[j] = more;
+        px[j] = r.vx;
+        py[j] = r.vy;
+        pz[j] = r.vz;
+        pt[j] = r.t;
+        if (more) {
+            const float tX = ((float)(r.vx + ux) * invN - r.ox) * r.ix;
+            const float tY = ((float)(r.vy + uy) * invN - r.oy) * r.iy;
+            const float tZ = ((float)(r.vz + uz) * invN - r.oz) * r.iz;
+            const int ax = (tX <= tY && tX <= tZ) ? 0 : (tY <= tZ ? 1 : 2);
+            const float tn = pick3(ax, tX, tY, tZ);
+            const int na = pick3(ax, r.vx + sx, r.vy + sy, r.vz + sz);
+            if (tn > r.tx || na < 0 || na >= N) {
+                end = true;
+                more = false;
             } else {
-                inb = true;
-                brick = a.bricks + (size_t)idx * 512u;
-                bx = vx >> 3;
-                by = vy >> 3;
-                bz = vz >> 3;
+                r.t = tn;
+                r.vx = ax == 0 ? na : r.vx;
+                r.vy = ax == 1 ? na : r.vy;
+                r.vz = ax == 2 ? na : r.vz;
+                if ((r.vx >> 3) != r.bx || (r.vy >> 3) != r.by || (r.vz >> 3) != r.bz) more = false;
             }
-          }
-          if (inb) {
-            // cross up to four voxels of the brick (a voxel step moves only the
-            // exit axis); the positions do not depend on the data, so the four
-            // loads are issued together and composited in order
-            {
-                int px[4], py[4], pz[4];
-                float pt[4];
-                bool valid[4];
-                bool more = true;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    valid[j] = more;
-                    px[j] = vx;
-                    py[j] = vy;
-                    pz[j] = vz;
-                    pt[j] = t;
-                    if (more) {
-                        const float tX = ((float)(vx + ux) * invN - ox) * ix;
-                        const float tY = ((float)(vy + uy) * invN - oy) * iy;
-                        const float tZ = ((float)(vz + uz) * invN - oz) * iz;
-                        const int ax = (tX <= tY && tX <= tZ) ? 0 : (tY <= tZ ? 1 : 2);
-                        const float tn = pick3(ax, tX, tY, tZ);
-                        const int na = pick3(ax, vx + sx, vy + sy, vz + sz);
-                        if (tn > tx_ || na < 0 || na >= N) {
-                            end = true;
-                            more = false;
-                        } else {
-                            t = tn;
-                            vx = ax == 0 ? na : vx;
-                            vy = ax == 1 ? na : vy;
-                            vz = ax == 2 ? na : vz;
-                            if ((vx >> 3) != bx || (vy >> 3) != by || (vz >> 3) != bz) more = false;
-                        }
-                    }
-                }
-                uint32_t dd[4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    dd[j] = valid[j] ? brick[(px[j] & 7) + 8 * (py[j] & 7) + 64 * (pz[j] & 7)] : 0u;
-                bool stop = false;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    if (valid[j] && !stop) {
-                        ++nv;
-                        if (dd[j]) {
-                            const uint32_t p = s_pal[dd[j]];
-                            const float k = 1.0f / 255.0f;
-                            float cr = (float)(p & 255u) * k, cg = (float)((p >> 8) & 255u) * k;
-                            float cb = (float)((p >> 16) & 255u) * k;
-                            const float ca = (float)(p >> 24) * k;
-                            cr = cr * ca;
-                            cg = cg * ca;
-                            cb = cb * ca;
-                            const float s1 = 1.0f - aa;
-                            ar = ar + cr * s1;
-                            ag = ag + cg * s1;
-                            ab = ab + cb * s1;
-                            aa = aa + ca * s1;
-                            if (first) {
-                                first = false;
-                                thit = pt[j];
-                                hx = px[j];
-                                hy = py[j];
-                                hz = pz[j];
-                            }
-                            if (aa > 0.99f) {
-                                stop = true;
-                                end = true;
-                            }
-                        }
-                    }
-                }
-                if (!more) inb = false;  // left the brick (or the ray ended)
-            }
-          }
         }
     }
-    if (a.rgba)
-        a.rgba[pix] = (uint32_t)(nt_saturate(ar) * 255.0f) | ((uint32_t)(nt_saturate(ag) * 255.0f) << 8) |
-                      ((uint32_t)(nt_saturate(ab) * 255.0f) << 16) | ((uint32_t)(nt_saturate(aa) * 255.0f) << 24);
-    if (a.t) a.t[pix] = thit;
-    if (a.voxel) {
-        a.voxel[3 * pix] = hx;
-        a.voxel[3 * pix + 1] = hy;
-        a.voxel[3 * pix + 2] = hz;
+    uint32_t dd[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) dd[j] = valid[j] ? r.brick[(px[j] & 7) + 8 * (py[j] & 7) + 64 * (pz[j] & 7)] : 0u;
+    bool stop = false;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (valid[j] && !stop) {
+            ++r.nv;
+            if (dd[j]) {
+                const uint32_t p = s_pal[dd[j]];
+                const float k = 1.0f / 255.0f;
+                float cr = (float)(p & 255u) * k, cg = (float)((p >> 8) & 255u) * k;
+                float cb = (float)((p >> 16) & 255u) * k;
+                const float ca = (float)(p >> 24) * k;
+                cr = cr * ca;
+                cg = cg * ca;
+                cb = cb * ca;
+                const float s1 = 1.0f - r.aa;
+                r.ar = r.ar + cr * s1;
+                r.ag = r.ag + cg * s1;
+                r.ab = r.ab + cb * s1;
+                r.aa = r.aa + ca * s1;
+                if (r.first) {
+                    r.first = false;
+                    r.thit = pt[j];
+                    r.hx = px[j];
+                    r.hy = py[j];
+                    r.hz = pz[j];
+                }
+                if (r.aa > 0.99f) {
+                    stop = true;
+                    end = true;
+                }
+            }
+        }
     }
-    if (a.n_nodes) a.n_nodes[pix] = nn;
-    if (a.n_voxels) a.n_voxels[pix] = nv;
+    if (!more) r.inb = false;  // left the brick (or the ray ended)
+    return end;
+}
+
+template <int L>
+__device__ __forceinline__ void nt_write(const NtArgs& a, size_t pix, const NtRay<L>& r) {
+    if (a.rgba)
+        a.rgba[pix] = (uint32_t)(nt_saturate(r.ar) * 255.0f) | ((uint32_t)(nt_saturate(r.ag) * 255.0f) << 8) |
+                      ((uint32_t)(nt_saturate(r.ab) * 255.0f) << 16) |
+                      ((uint32_t)(nt_saturate(r.aa) * 255.0f) << 24);
+    if (a.t) a.t[pix] = r.thit;
+    if (a.voxel) {
+        a.voxel[3 * pix] = r.hx;
+        a.voxel[3 * pix + 1] = r.hy;
+        a.voxel[3 * pix + 2] = r.hz;
+    }
+    if (a.n_nodes) a.n_nodes[pix] = r.nn;
+    if (a.n_voxels) a.n_voxels[pix] = r.nv;
+}
+
+__device__ __forceinline__ void nt_stage(const NtArgs& a, uint32_t* s_root, uint32_t* s_pal) {
+    const int nroot = a.R * a.R * a.R;
+    for (int i = threadIdx.x; i < nroot; i += 256) s_root[i] = a.root[i];
+    s_pal[threadIdx.x] = a.palette[threadIdx.x];
+    __syncthreads();
+}
+
+// One lane per pixel; wave w covers 8x8 pixel tile w (row-major tiles).
+template <int L>
+__global__ __launch_bounds__(256) void nt_trace_kernel(NtArgs a) {
+    __shared__ uint32_t s_root[kNtMaxRoot * kNtMaxRoot * kNtMaxRoot];
+    __shared__ uint32_t s_pal[256];
+    nt_stage(a, s_root, s_pal);
+    const uint32_t wave = (blockIdx.x * 256u + threadIdx.x) >> 6;
+    const int lane = (int)(threadIdx.x & 63);
+    const int tiles_x = (a.w + 7) >> 3;
+    const int x = (int)(wave % (uint32_t)tiles_x) * 8 + (lane & 7);
+    const int y = (int)(wave / (uint32_t)tiles_x) * 8 + (lane >> 3);
+    if (y >= a.h || x >= a.w) return;
+    NtRay<L> r;
+    if (nt_setup(a, x, y, r))
+        while (!nt_iterate(a, s_root, s_pal, r)) {
+        }
+    nt_write(a, (size_t)y * a.w + x, r);
+}
+
+// Persistent waves over an atomic ray queue (ray ids in 8x8 pixel tiles).
+// A lane whose ray finished parks until at least refill_min lanes of its
+// wave wait (or none is walking); then the wave writes their outputs, takes
+// that many ray ids with one atomic and sets the new rays up together, so
+// short rays do not leave their lanes idle until the wave's longest ray ends.
+// The last wave to retire resets the queue for the next launch.
+template <int L>
+__global__ __launch_bounds__(256) void nt_trace_persistent(NtArgs a) {
+    __shared__ uint32_t s_root[kNtMaxRoot * kNtMaxRoot * kNtMaxRoot];
+    __shared__ uint32_t s_pal[256];
+    nt_stage(a, s_root, s_pal);
+    const int lane = (int)(threadIdx.x & 63);
+    const int tiles_x = (a.w + 7) >> 3;
+    enum : int { kIdle = 0, kWalk = 1, kDone = 2, kRetired = 3 };
+    int status = kIdle;
+    bool drained = false;  // wave-uniform
+    size_t pix = 0;
+    NtRay<L> r;
+    for (;;) {
+        const uint64_t walking = __ballot(status == kWalk);
+        const uint64_t waiting = __ballot(status == kIdle || status == kDone);
+        if (waiting != 0 && (walking == 0 || __popcll(waiting) >= a.refill_min)) {
+            if (status == kDone) {
+                nt_write(a, pix, r);
+                status = kIdle;
+            }
+            const uint64_t need = __ballot(status == kIdle);
+            if (need != 0) {
+                if (drained) {
+                    if (status == kIdle) status = kRetired;
+                } else {
+                    const uint32_t cnt = (uint32_t)__popcll(need);
+                    const int leader = __ffsll((unsigned long long)need) - 1;
+                    uint32_t got = 0;
+                    if (lane == leader) got = atomicAdd(&a.counters[0], cnt);
+                    got = __shfl(got, leader);
+                    if (got + cnt >= a.total) drained = true;
+                    if (status == kIdle) {
+                        const uint32_t my = got + lanes_below(need);
+                        if (my >= a.total) {
+                            status = kRetired;
+                        } else {
+                            const uint32_t tile = my >> 6, within = my & 63u;
+                            const int x = (int)(tile % (uint32_t)tiles_x) * 8 + (int)(within & 7u);
+                            const int y = (int)(tile / (uint32_t)tiles_x) * 8 + (int)(within >> 3);
+                            if (x < a.w && y < a.h) {
+                                pix = (size_t)y * a.w + x;
+                                status = nt_setup(a, x, y, r) ? kWalk : kDone;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        if (__ballot(status != kRetired) == 0) break;
+        if (status == kWalk && nt_iterate(a, s_root, s_pal, r)) status = kDone;
+    }
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(&a.counters[1], 1u);
+        if (done + 1 == a.n_waves) {
+            atomicExch(&a.counters[0], 0u);
+            atomicExch(&a.counters[1], 0u);
+            __threadfence();
+        }
+    }
 }
 
 }  // namespace yx
@@ -466,6 +575,10 @@ struct yx_ntree {
     uint4* d_nodes = nullptr;
     uint8_t* d_bricks = nullptr;
     uint64_t n_nodes = 0, n_bricks = 0;
+    uint32_t* d_counters = nullptr;  // ray queue of the persistent cast
+    int persistent = 1;              // 0: one lane per pixel
+    int refill_min = 16;
+    int num_cus = 0;
 };
 
 static int nt_shape(int size, int levels, int* R) {
@@ -579,10 +692,14 @@ static int nt_build(const NtSource& src, int size, int levels, yx_ntree** out, h
     t->R = R;
     t->n_nodes = total_nodes;
     t->n_bricks = total_bricks;
+    t->num_cus = cus;
     auto release = [&](int code) {
         yx_ntree_free(t);
         return code;
     };
+    if (hipMalloc((void**)&t->d_counters, 2 * sizeof(uint32_t)) != hipSuccess ||
+        hipMemsetAsync(t->d_counters, 0, 2 * sizeof(uint32_t), st) != hipSuccess)
+        return release(fail(YX_ERR_OOM, "hipMalloc for the n-tree ray queue"));
     if (hipMalloc((void**)&t->d_root, (size_t)R * R * R * 4) != hipSuccess ||
         hipMalloc((void**)&t->d_nodes, std::max<uint64_t>(1, total_nodes) * sizeof(uint4)) != hipSuccess ||
         hipMalloc((void**)&t->d_bricks, std::max<uint64_t>(1, total_bricks) * 512u) != hipSuccess)
@@ -663,7 +780,17 @@ extern "C" void yx_ntree_free(yx_ntree* t) {
     if (t->d_root) (void)hipFree(t->d_root);
     if (t->d_nodes) (void)hipFree(t->d_nodes);
     if (t->d_bricks) (void)hipFree(t->d_bricks);
+    if (t->d_counters) (void)hipFree(t->d_counters);
     delete t;
+}
+
+extern "C" int yx_ntree_set_launch(yx_ntree* t, int32_t persistent, int32_t refill_min) {
+    clear_error();
+    if (!t) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (refill_min < 1 || refill_min > 64) return fail(YX_ERR_INVALID_ARG, "refill_min must be 1..64");
+    t->persistent = persistent != 0;
+    t->refill_min = refill_min;
+    return YX_OK;
 }
 
 extern "C" int yx_ntree_get_info(const yx_ntree* t, yx_ntree_info* info) {
@@ -710,15 +837,38 @@ extern "C" int yx_ntree_render(const yx_ntree* t, const float view[12], int32_t 
     a.n_nodes = out->nodes;
     a.n_voxels = out->voxels;
     const uint32_t waves = (uint32_t)(((w + 7) / 8) * ((h + 7) / 8));
-    const dim3 grid((waves + 3) / 4), blk(256);
+    const dim3 blk(256);
     const hipStream_t st = (hipStream_t)stream;
-    switch (t->levels) {
-        case 0: hipLaunchKernelGGL(nt_trace_kernel<0>, grid, blk, 0, st, a); break;
-        case 1: hipLaunchKernelGGL(nt_trace_kernel<1>, grid, blk, 0, st, a); break;
-        case 2: hipLaunchKernelGGL(nt_trace_kernel<2>, grid, blk, 0, st, a); break;
-        case 3: hipLaunchKernelGGL(nt_trace_kernel<3>, grid, blk, 0, st, a); break;
-        default: hipLaunchKernelGGL(nt_trace_kernel<4>, grid, blk, 0, st, a); break;
+    if (!t->persistent) {
+        const dim3 grid((waves + 3) / 4);
+        switch (t->levels) {
+            case 0: hipLaunchKernelGGL(nt_trace_kernel<0>, grid, blk, 0, st, a); break;
+            case 1: hipLaunchKernelGGL(nt_trace_kernel<1>, grid, blk, 0, st, a); break;
+            case 2: hipLaunchKernelGGL(nt_trace_kernel<2>, grid, blk, 0, st, a); break;
+            case 3: hipLaunchKernelGGL(nt_trace_kernel<3>, grid, blk, 0, st, a); break;
+            default: hipLaunchKernelGGL(nt_trace_kernel<4>, grid, blk, 0, st, a); break;
+        }
+        YX_HIP(hipGetLastError());
+        return YX_OK;
     }
+    a.counters = t->d_counters;
+    a.total = waves * 64u;
+    a.refill_min = t->refill_min;
+    const void* fn = nullptr;
+    void (*kern)(NtArgs) = nullptr;
+    switch (t->levels) {
+        case 0: kern = nt_trace_persistent<0>; break;
+        case 1: kern = nt_trace_persistent<1>; break;
+        case 2: kern = nt_trace_persistent<2>; break;
+        case 3: kern = nt_trace_persistent<3>; break;
+        default: kern = nt_trace_persistent<4>; break;
+    }
+    fn = (const void*)kern;
+    int occ = 0;
+    YX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 256, 0));
+    const uint32_t grid = (uint32_t)(std::max(1, occ) * t->num_cus);
+    a.n_waves = grid * 4u;
+    hipLaunchKernelGGL(kern, dim3(grid), blk, 0, st, a);
     YX_HIP(hipGetLastError());
     return YX_OK;
 }

@@ -58,6 +58,11 @@ class BrickTree:
                                       bricks.ctypes.data if bricks.size else None))
         return root, nodes, bricks
 
+    def set_launch(self, persistent=True, refill_min=16):
+        """Persistent waves with ray refill (default) or one lane per pixel;
+        identical outputs."""
+        check(lib().yx_ntree_set_launch(self._h, 1 if persistent else 0, int(refill_min)))
+
     def render(self, view, width, height, palette, stream=None, **out):
         """Cast into caller-owned device buffers (rgba, t, voxel, nodes,
         voxels); `palette` is a device tensor of 256 RGBA8 (uint32/int32)."""
