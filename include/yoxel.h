@@ -338,9 +338,10 @@ typedef struct yx_ntree_outputs {
  * device pointer; entry 0 unused) until alpha > 0.99. Asynchronous. */
 int yx_ntree_render(const yx_ntree* t, const float view[12], int32_t width, int32_t height,
                     const uint32_t* palette, const yx_ntree_outputs* out, void* stream);
-/* Launch of the cast: persistent != 0 (default) runs persistent waves over an
- * atomic ray queue whose lanes take new rays once refill_min (1..64, default
- * 16) of them wait; 0 runs one lane per pixel. Both give identical outputs.
+/* Launch of the cast: persistent != 0 runs persistent waves over an atomic ray
+ * queue whose lanes take new rays once refill_min (1..64, default 16) of them
+ * wait; 0 (default) runs one lane per pixel, one 8x8 tile per wave. Both give
+ * identical outputs.
  * Casts on one tree must not overlap (the queue lives in the tree). No
  * reference counterpart (launch shape). */
 int yx_ntree_set_launch(yx_ntree* t, int32_t persistent, int32_t refill_min);

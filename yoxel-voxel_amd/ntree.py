@@ -58,8 +58,8 @@ class BrickTree:
                                       bricks.ctypes.data if bricks.size else None))
         return root, nodes, bricks
 
-    def set_launch(self, persistent=True, refill_min=16):
-        """Persistent waves with ray refill (default) or one lane per pixel;
+    def set_launch(self, persistent=False, refill_min=16):
+        """One lane per pixel (default) or persistent waves with ray refill;
         identical outputs."""
         check(lib().yx_ntree_set_launch(self._h, 1 if persistent else 0, int(refill_min)))
 
