@@ -349,12 +349,12 @@ static int plan_launch(yx_batch* b, const yx_outputs& out, BatchLaunch& L) {
     a.tiles_x = 1;  // per job in the table
     a.w = a.h = 1;
     const int win = t.window;
-    const int rows = std::max(1, b->levels - win);
+    const size_t lane_bytes = stack_lane_bytes(b->levels, win);
     int block = t.block;
-    size_t lds = (size_t)rows * 32u * (size_t)block;
+    size_t lds = lane_bytes * (size_t)block;
     while (lds > b->max_lds && block > 64) {
         block /= 2;
-        lds = (size_t)rows * 32u * (size_t)block;
+        lds = lane_bytes * (size_t)block;
     }
     if (lds > b->max_lds)
         return fail(YX_ERR_RANGE, "traversal stack rows need %zu bytes of LDS, device allows %zu", lds, b->max_lds);

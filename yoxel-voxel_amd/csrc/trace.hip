@@ -238,15 +238,14 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
     int kernel = t.kernel;
     int block = t.block;
     if (kernel == YX_KERNEL_PERSISTENT) {
-        // 32 bytes per LDS level row and lane (rows for levels older than the
-        // register window); shrink the workgroup, then fall back to the replay
-        // kernel (same results) when the rows do not fit
+        // LDS stack rows per lane (stack_lane_bytes); shrink the workgroup, then
+        // fall back to the replay kernel (same results) when they do not fit
         const int win = t.window <= 0 ? 0 : (t.window <= 2 ? 2 : (t.window <= 4 ? 4 : 6));
-        const int rows = std::max(1, a.levels - win);
-        size_t lds = (size_t)rows * 32u * (size_t)block;
+        const size_t lane_bytes = stack_lane_bytes(a.levels, win);
+        size_t lds = lane_bytes * (size_t)block;
         while (lds > r->max_lds && block > 64) {
             block /= 2;
-            lds = (size_t)rows * 32u * (size_t)block;
+            lds = lane_bytes * (size_t)block;
         }
         if (lds <= r->max_lds) {
             if (block == 64) {

@@ -94,6 +94,14 @@ __device__ __forceinline__ void write_final(const yx_outputs& o, uint32_t pix, u
     if (o.visits) o.visits[pix] = visits;
 }
 
+// LDS bytes per lane of stack_kernel<BLOCK, W, ...> when `levels` levels are
+// pushed: W == 0 keeps every node record (8 B) and the cells (24 B) of even
+// levels; W > 0 keeps 32-byte rows for the levels older than the window.
+inline size_t stack_lane_bytes(int levels, int W) {
+    if (W == 0) return (size_t)8 * (size_t)levels + (size_t)24 * (size_t)((levels + 1) / 2);
+    return (size_t)32 * (size_t)(levels - W > 1 ? levels - W : 1);
+}
+
 // lane states
 enum : int {
     sIdle = 0,     // needs a ray id
@@ -137,6 +145,38 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     float lam = 0.0f;
     WinEntry win[W > 0 ? W : 1];
 
+    // W == 0: every level's node record ([levels][BLOCK] uint2), and the cell
+    // of even levels only ([(levels + 1) / 2][BLOCK] as uint4 + uint2). A pop to
+    // an odd level rebuilds its cell from the level above by one exact replay
+    // step, so a lane needs 8 B + 12 B per level instead of 32 B.
+    uint64_t ffcpath = 0;  // FindFirstChild result per level (W == 0)
+    uint2* sh_node = reinterpret_cast<uint2*>(sh_stk);
+    uint4* sh_ca = reinterpret_cast<uint4*>(sh_node + (size_t)a.levels * BLOCK);
+    uint2* sh_cb = reinterpret_cast<uint2*>(sh_ca + (size_t)((a.levels + 1) >> 1) * BLOCK);
+    auto push0 = [&](int lv, const Cell& cc, uint2 nn) {
+        sh_node[(size_t)lv * BLOCK + tid] = nn;
+        if ((lv & 1) == 0) {
+            const size_t r = (size_t)(lv >> 1) * BLOCK + tid;
+            sh_ca[r] = make_uint4(__float_as_uint(cc.t1x), __float_as_uint(cc.t1y), __float_as_uint(cc.t1z),
+                                  __float_as_uint(cc.t2x));
+            sh_cb[r] = make_uint2(__float_as_uint(cc.t2y), __float_as_uint(cc.t2z));
+        }
+    };
+    auto pop0 = [&](int lv, Cell& cc, uint2& nn) {
+        nn = sh_node[(size_t)lv * BLOCK + tid];
+        const size_t r = (size_t)(lv >> 1) * BLOCK + tid;
+        const uint4 r0 = sh_ca[r];
+        const uint2 r1 = sh_cb[r];
+        cc.t1x = __uint_as_float(r0.x);
+        cc.t1y = __uint_as_float(r0.y);
+        cc.t1z = __uint_as_float(r0.z);
+        cc.t2x = __uint_as_float(r0.w);
+        cc.t2y = __uint_as_float(r1.x);
+        cc.t2z = __uint_as_float(r1.y);
+        // odd level: the row holds the cell of level lv - 1, i.e. this level's
+        // node interval; replay FindFirstChild / GoNext to the current child
+        if (lv & 1) replay_level(cc, bits3(ffcpath, lv), bits3(chpath, lv));
+    };
     auto lds_store = [&](int row, const Cell& cc, uint2 nn) {
         uint4* p = sh_stk + (size_t)(2 * row) * BLOCK + tid;
         p[0] = make_uint4(__float_as_uint(cc.t1x), __float_as_uint(cc.t1y), __float_as_uint(cc.t1z),
@@ -270,6 +310,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     level = 0;
                     adv = false;
                     ch = find_first_child(c);
+                    if constexpr (W == 0) ffcpath = set3(ffcpath, 0, (uint32_t)ch);
                     status = status == sStartP ? sTraceP : sTraceS;
                 } else if (status == sStartP) {
                     write_miss(a.out, pix, visits);
@@ -308,13 +349,14 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     win[0].c = c;
                     win[0].rec = node;
                 } else {
-                    lds_store(level, c, node);
+                    push0(level, c, node);
                 }
                 chpath = set3(chpath, level, (uint32_t)ch);
                 const uint32_t below = (cm | lm) & ((1u << real) - 1u);
                 node = recs[node.x + (uint32_t)__builtin_popcount(below)];
                 ++level;
                 ch = find_first_child(c);
+                if constexpr (W == 0) ffcpath = set3(ffcpath, level, (uint32_t)ch);
                 continue;
             }
             adv = true;
@@ -337,7 +379,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                 for (int k = 0; k + 1 < W; ++k) win[k] = win[k + 1];
                 if (level >= W) lds_load(level - W, win[W - 1].c, win[W - 1].rec);
             } else {
-                lds_load(level, c, node);
+                pop0(level, c, node);
             }
             ch = (int)bits3(chpath, level);
         }
