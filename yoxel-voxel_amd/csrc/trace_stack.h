@@ -325,10 +325,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
         if (status != sTraceP && status != sTraceS) continue;
 
         // ---- one traversal step ---------------------------------------------
-        // Test the current child: a leaf ends the ray (true), a non-empty child
-        // node in front of the origin is pushed and entered (true), anything
-        // else needs GoNext (false).
-        auto test_child = [&]() -> bool {
+        if (!adv) {
             const int real = ch ^ (int)flags;
             const uint32_t cm = node.y & 0xffu, lm = (node.y >> 8) & 0xffu;
             const float mt2 = min3(c.t2x, c.t2y, c.t2z);
@@ -340,7 +337,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     occluded = true;
                     status = sEndS;
                 }
-                return true;
+                continue;
             }
             if (!is_leaf && ((cm >> real) & 1u) && mt2 > 0) {
                 // push this level, descend into the child node
@@ -360,17 +357,12 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                 ++level;
                 ch = find_first_child(c);
                 if constexpr (W == 0) ffcpath = set3(ffcpath, level, (uint32_t)ch);
-                return true;
+                continue;
             }
-            return false;
-        };
-        if (!adv) {
-            if (test_child()) continue;
             adv = true;
         }
         if (go_next(ch, c)) {
-            // the sibling GoNext reached is tested in the same iteration
-            adv = !test_child();
+            adv = false;
         } else if (level == 0) {
             if (status == sTraceP) {
                 status = sMissP;
