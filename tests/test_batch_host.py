@@ -47,3 +47,19 @@ def test_batched_recipes(yx):
     assert len(jobs) == len(cams) == 416
     assert sum(j["width"] * j["height"] for j in jobs) == 348_521_472
     assert W.batched_recipes() == rec  # deterministic
+
+
+def test_ntree_argument_checks(yx):
+    """Shape and recipe checks of the n-tree entry points run before any
+    device call."""
+    import numpy as np
+    for size, levels, msg in [(100, 1, "power of two"), (64, 3, "8 \\* 4"), (4, 0, "power of two"),
+                              (64, 5, "levels"), (512, 0, "root grid")]:
+        with pytest.raises(yx.YoxelError, match=msg):
+            yx.BrickTree.build_fbm(size=size, levels=levels, octaves=4, frequency=4.0, threshold=0.3,
+                                   scale=100.0, seed=1, stream=0)
+    with pytest.raises(yx.YoxelError, match="octaves"):
+        yx.BrickTree.build_fbm(size=64, levels=1, octaves=0, frequency=4.0, threshold=0.3, scale=100.0, seed=1,
+                               stream=0)
+    with pytest.raises(yx.YoxelError, match="power of two"):
+        yx.BrickTree.from_dense(np.zeros((24, 24, 24), np.uint8), 0)

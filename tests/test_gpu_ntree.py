@@ -131,3 +131,55 @@ def test_brick_4096_full(yx, oracle_mod, cuda):
     hit = (g["t"] >= 0).mean()
     print(f"brick_4096 frame: {hit:.3f} of the rays reach a non-empty voxel", flush=True)
     assert 0.05 < hit
+
+
+def test_empty_and_edge_shapes(yx, oracle_mod, cuda):
+    """An all-empty volume (no bricks: every ray misses), the largest root grid
+    (16^3, 512^3 volume with one node level) from a sparse random volume, and a
+    four-level tree (2048^3 from the fBm) checked on sampled bricks and rows."""
+    o = oracle_mod.NTreeOracle()
+    pal = make_palette(9)
+    w, h = 48, 40
+    # empty
+    t = yx.BrickTree.from_dense(np.zeros((64, 64, 64), np.uint8), 1)
+    info = t.info()
+    assert info["num_bricks"] == 0 and info["num_nodes"] == 0
+    for view in views_for(yx, w, h, seed=3):
+        g = gpu_cast(cuda, t, view, w, h, pal)
+        assert np.all(g["rgba"] == 0) and np.all(g["t"] == -1) and np.all(g["voxel"] == -1)
+        assert np.all(g["nodes"] == 0) and np.all(g["voxels"] == 0)
+    # 16^3 root grid: 512^3 with one level of 4^3 nodes, sparse random blobs
+    rng = np.random.default_rng(5)
+    dense = np.zeros((512, 512, 512), np.uint8)
+    for _ in range(300):
+        c = rng.integers(0, 500, 3)
+        s = rng.integers(2, 12, 3)
+        dense[c[2]:c[2] + s[2], c[1]:c[1] + s[1], c[0]:c[0] + s[0]] = rng.integers(1, 256)
+    root, nodes, bricks = o.build_dense(dense, 1)
+    assert root.size == 16 ** 3
+    t = yx.BrickTree.from_dense(dense, 1)
+    g_root, g_nodes, g_bricks = t.download()
+    assert np.array_equal(g_root, root) and np.array_equal(g_nodes, nodes) and np.array_equal(g_bricks, bricks)
+    for i, view in enumerate(views_for(yx, w, h, seed=11)):
+        g = gpu_cast(cuda, t, view, w, h, pal)
+        c = o.render(root, nodes, bricks, 512, 1, view, pal, w, h)
+        assert_cast_equal(g, c, what=f"root 16^3 view {i}")
+    # four node levels: 2048 = 8 * 4^4 * 1
+    cfg = dict(size=2048, levels=4, octaves=4, frequency=6.0, threshold=0.55, scale=300.0, seed=19)
+    t = yx.BrickTree.build_fbm(**cfg)
+    root, nodes, bricks = t.download()
+    assert root.size == 1
+    d = o.desc(**cfg)
+    for k in range(200):
+        bx, by, bz = (int(v) for v in rng.integers(0, 256, 3))
+        ref = o.density_box(d, bx * 8, by * 8, bz * 8, 8, 8, 8)
+        idx = brick_index(root, nodes, 2048, 4, bx, by, bz)
+        assert (idx is not None) == bool(ref.any())
+        if idx is not None:
+            assert np.array_equal(bricks[idx].reshape(8, 8, 8), ref)
+    W, H = 320, 200
+    view = yx.composite.camera_views(yx.workloads.orbit_cameras(2, seed=4)[:1], 60.0, W, H)[0]
+    g = gpu_cast(cuda, t, view, W, H, pal)
+    c = o.render(root, nodes, bricks, 2048, 4, view, pal, W, H, ystep=4, threads=16)
+    assert_cast_equal(g, c, rows=np.arange(0, H, 4), what="four levels")
+    assert (g["t"] >= 0).any()

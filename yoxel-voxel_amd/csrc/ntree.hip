@@ -731,6 +731,11 @@ extern "C" int yx_ntree_build_fbm(const yx_ntree_desc* d, yx_ntree** out, void* 
     if (!d || !out) return fail(YX_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     if (d->octaves < 1 || d->octaves > 16) return fail(YX_ERR_INVALID_ARG, "octaves %d: 1..16", d->octaves);
+    if (!(d->scale >= 0.0f) || d->frequency != d->frequency || d->threshold != d->threshold)
+        return fail(YX_ERR_INVALID_ARG, "frequency, threshold and scale must be numbers, scale >= 0");
+    int R = 0;
+    const int shape = nt_shape(d->size, d->levels, &R);
+    if (shape != YX_OK) return shape;
     const hipStream_t st = (hipStream_t)stream;
     Perlin perlin(d->seed);
     int* d_perm = nullptr;

@@ -9,7 +9,11 @@
 // per iteration - test the current child and descend, or GoNext, or pop one
 // level - with the exact state the reference recursion keeps on its call stack
 // (RecTrace, cell/ppu_renderer.cpp:18-41): per level the cell (t1, t2) and the
-// node record, the W most recent levels in registers (a shift window) and the
+// node record. Without a register window (W = 0, the default) LDS holds every
+// level's node record and the cells of even levels; a pop to an odd level
+// rebuilds the cell from the level above with one exact replay step
+// (FindFirstChild / GoNext float operations, as in trace_persist.h). With
+// W > 0 the W most recent levels sit in registers (a shift window) and the
 // older ones in 32-byte LDS rows. Everything that happens once per ray - taking
 // ray ids, ray setup with its nine divisions, hit decoding, shading, the shadow
 // ray spawn, output writes - is parked and done for many lanes at once, only
