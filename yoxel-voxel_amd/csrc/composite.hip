@@ -52,6 +52,7 @@ struct CompArgs {
     uint32_t total_records;
     uint32_t n_waves;
     uint32_t* keys;
+    uint32_t* ids;  // record index (the sort's payload)
     float4* vals;
 };
 
@@ -266,6 +267,7 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
             const uint32_t pos = cursor + lanes_rank(mask);
             if (pos < a.total_records) {
                 a.keys[pos] = idx;
+                a.ids[pos] = pos;
                 a.vals[pos] = make_float4(dsig * sgm, wgt * gr, wgt * gg, wgt * gb);
             }
             T = tn;
@@ -299,13 +301,13 @@ __global__ void comp_zero_kernel(float4* g, size_t n) {
 // lane of each run piece writes. A run that leaves the wave is finished by
 // the wave holding its first record, reading on serially; waves that only
 // hold the continuation of an earlier run write nothing for it.
-__global__ __launch_bounds__(256) void comp_reduce_runs_kernel(const uint32_t* keys, const float4* vals, uint32_t n,
-                                                               float4* grad) {
+__global__ __launch_bounds__(256) void comp_reduce_runs_kernel(const uint32_t* keys, const uint32_t* ids,
+                                                               const float4* vals, uint32_t n, float4* grad) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = (int)(threadIdx.x & 63);
     const bool in = i < n;
     const uint32_t key = in ? keys[i] : 0xFFFFFFFFu;
-    float4 s = in ? vals[i] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    float4 s = in ? vals[ids[i]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     const uint32_t up = __shfl_up(key, 1u);
     const uint32_t before = lane > 0 ? up : (i > 0 && in ? keys[i - 1] : ~key);
     const bool head = in && before != key;
@@ -331,7 +333,7 @@ __global__ __launch_bounds__(256) void comp_reduce_runs_kernel(const uint32_t* k
     const bool owns = (heads >> start) & 1ull;  // the run's first record is in this wave
     if (!owns) return;
     for (uint32_t j = i + 1; j < n && keys[j] == key; ++j) {
-        const float4 v = vals[j];
+        const float4 v = vals[ids[j]];
         s.x += v.x;
         s.y += v.y;
         s.z += v.z;
@@ -511,14 +513,16 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     c->sorted_keys = nullptr;
     if (total == 0) return YX_OK;
 
-    // 2. workspace: keys, vals, sorted keys, sorted vals, library temporaries
+    // 2. workspace: keys, record ids, their sorted copies, values, library
+    //    temporaries. The sort moves (key, id) pairs only; the reduction
+    //    gathers each value once through the sorted ids.
     int key_bits = 1;
     while (key_bits < 32 && ((uint64_t)1 << key_bits) < (uint64_t)nvox) ++key_bits;
     size_t sort_tmp = 0;
     YX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                              (float4*)nullptr, (float4*)nullptr, (int)total, 0, key_bits, st));
+                                              (uint32_t*)nullptr, (uint32_t*)nullptr, (int)total, 0, key_bits, st));
     const size_t kb = align256(total * 4), vb = align256(total * 16);
-    const size_t need = off_bytes + 2 * kb + 2 * vb + 256 + align256(std::max(sort_tmp, scan_tmp));
+    const size_t need = off_bytes + 4 * kb + vb + 256 + align256(std::max(sort_tmp, scan_tmp));
     if (c->ws_bytes < need) {
         // grow, then recompute the offsets into the new block
         if (c->ws) YX_HIP(hipFree(c->ws));
@@ -537,9 +541,11 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     p += kb;
     uint32_t* keys_sorted = (uint32_t*)p;
     p += kb;
+    uint32_t* ids = (uint32_t*)p;
+    p += kb;
+    uint32_t* ids_sorted = (uint32_t*)p;
+    p += kb;
     float4* vals = (float4*)p;
-    p += vb;
-    float4* vals_sorted = (float4*)p;
     p += vb;
     uint32_t* counter = (uint32_t*)p;
     p += 256;
@@ -551,15 +557,17 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     a.offsets = offsets;
     a.total_records = (uint32_t)total;
     a.keys = keys;
+    a.ids = ids;
     a.vals = vals;
     hipLaunchKernelGGL(comp_backward_kernel, dim3((unsigned)((n_waves * 64 + 255) / 256)), dim3(256), 0, st, a);
     YX_HIP(hipGetLastError());
 
-    // 4. sort by voxel key (LSD radix, stable), 5. sum each key's run in order
-    YX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys_sorted, vals, vals_sorted, (int)total, 0,
+    // 4. sort (key, id) by voxel key (LSD radix, stable: a key's records stay
+    //    in emission order), 5. sum each key's run in that order
+    YX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys_sorted, ids, ids_sorted, (int)total, 0,
                                               key_bits, st));
     hipLaunchKernelGGL(comp_reduce_runs_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, keys_sorted,
-                       vals_sorted, (uint32_t)total, (float4*)grad_params);
+                       ids_sorted, vals, (uint32_t)total, (float4*)grad_params);
     YX_HIP(hipGetLastError());
     c->sorted_keys = keys_sorted;
     c->counter = counter;
