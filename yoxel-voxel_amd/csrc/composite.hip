@@ -112,7 +112,15 @@ struct SampleIter {
     int k;
     bool live;
 
-    __device__ __forceinline__ bool step(const CompArgs& a, uint32_t& idx, float4& q) {
+    int64_t cb;  // brick of the last table lookup (-1: none)
+    int cslot;   // its slot
+
+    // Advance by one step without loading the sample's parameters: returns
+    // true with the parameter index of a sample, false after a skip or at the
+    // end. The sequence does not depend on parameter values, so callers can
+    // take several steps and then load their parameters together.
+    __device__ __forceinline__ bool next(const CompArgs& a, uint32_t& idx) {
+        idx = 0;
         if (!live) return false;
         if (k >= r.kend) {
             live = false;
@@ -123,7 +131,12 @@ struct SampleIter {
         const float t = (float)k * inv_n;
         const int vx = vox(r.ox, r.dx, t, nf, a.n), vy = vox(r.oy, r.dy, t, nf, a.n), vz = vox(r.oz, r.dz, t, nf, a.n);
         const int bx = vx >> a.lb, by = vy >> a.lb, bz = vz >> a.lb;
-        const int slot = a.table[((size_t)bz * a.g + by) * a.g + bx];
+        const int64_t bl = ((int64_t)bz * a.g + by) * a.g + bx;
+        if (bl != cb) {  // consecutive samples mostly share a brick
+            cb = bl;
+            cslot = a.table[bl];
+        }
+        const int slot = cslot;
         if ((uint32_t)slot >= a.num_slots) {  // -1 (and anything out of range) is empty
             // jump to the last sample still in this brick (exact: the candidate
             // is checked and moved back while it left the brick)
@@ -146,11 +159,20 @@ struct SampleIter {
         }
         const uint32_t local = (uint32_t)((vx & bmask) + ((vy & bmask) << a.lb) + ((vz & bmask) << (2 * a.lb)));
         idx = ((uint32_t)slot << (3 * a.lb)) + local;
-        q = a.params[idx];
         ++k;
         return true;
     }
+
+    __device__ __forceinline__ void start(const CompArgs& a, int view, int x, int y, bool valid) {
+        live = valid && comp_setup(a, view, x, y, r);
+        k = r.k;
+        cb = -1;
+        cslot = -1;
+    }
 };
+
+// Steps taken per wave iteration; their parameter loads are issued together.
+constexpr int kCompBatch = 4;
 
 __device__ __forceinline__ float softplus(float x) { return x > 15.0f ? x : log1pf(expf(x)); }
 
@@ -180,24 +202,32 @@ __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
     uint32_t n = 0;
     SampleIter it;
-    it.live = comp_setup(a, view, x, y, it.r);
-    it.k = it.r.k;
-    float4 q;
-    uint32_t idx;
-    // one step per lane per iteration; the wave-wide vote keeps the loop flat
-    // (without it the skip steps get folded into an inner loop that holds the
-    // wave until every lane has found its next sample)
+    it.start(a, view, x, y, true);
+    // kCompBatch steps per lane per iteration, their loads issued together;
+    // the wave-wide vote keeps the loop flat (without it the skip steps get
+    // folded into an inner loop that holds the wave until every lane has found
+    // its next sample)
     for (;;) {
-        if (it.step(a, idx, q)) {
-            const float sig = softplus(q.x);
-            const float e = expf(-sig);
-            const float al = -expm1f(-sig);
-            const float wgt = T * al;
-            cr += wgt * q.y;
-            cg += wgt * q.z;
-            cb += wgt * q.w;
-            T = T * e;
-            ++n;
+        uint32_t ix[kCompBatch];
+        bool got[kCompBatch];
+#pragma unroll
+        for (int j = 0; j < kCompBatch; ++j) got[j] = it.next(a, ix[j]);
+        float4 q[kCompBatch];
+#pragma unroll
+        for (int j = 0; j < kCompBatch; ++j) q[j] = got[j] ? a.params[ix[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int j = 0; j < kCompBatch; ++j) {
+            if (got[j]) {
+                const float sig = softplus(q[j].x);
+                const float e = expf(-sig);
+                const float al = -expm1f(-sig);
+                const float wgt = T * al;
+                cr += wgt * q[j].y;
+                cg += wgt * q[j].z;
+                cb += wgt * q[j].w;
+                T = T * e;
+                ++n;
+            }
         }
         if (__ballot(it.live) == 0) break;
     }
@@ -233,8 +263,7 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
     size_t ray = 0;
     const bool valid = ray_of(a, t, view, x, y, ray);
     SampleIter it;
-    it.live = valid && comp_setup(a, view, x, y, it.r);
-    it.k = it.r.k;
+    it.start(a, view, x, y, valid);
     float gr = 0.0f, gg = 0.0f, gb = 0.0f, fr = 0.0f, fg = 0.0f, fb = 0.0f;
     if (valid) {
         gr = a.grad_rgb[3 * ray];
@@ -246,33 +275,43 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
     }
     uint32_t cursor = a.offsets[wave];
     float pr = 0.0f, pg = 0.0f, pb = 0.0f, T = 1.0f;
+    // kCompBatch steps per iteration as in the forward; the records of step j
+    // of every lane of the wave are appended (in lane order) before those of
+    // step j + 1, so the record order is fixed
     for (;;) {
-        float4 q;
-        uint32_t idx = 0;
-        const bool emit = it.step(a, idx, q);
-        const uint64_t mask = __ballot(emit);
-        if (mask == 0 && __ballot(it.live) == 0) break;
-        if (emit) {
-            const float sig = softplus(q.x);
-            const float e = expf(-sig);
-            const float al = -expm1f(-sig);
-            const float wgt = T * al;
-            pr += wgt * q.y;
-            pg += wgt * q.z;
-            pb += wgt * q.w;
-            const float tn = T * e;
-            const float sr = fr - pr, sg = fg - pg, sb = fb - pb;
-            const float dsig = gr * (tn * q.y - sr) + gg * (tn * q.z - sg) + gb * (tn * q.w - sb);
-            const float sgm = 1.0f / (1.0f + expf(-q.x));
-            const uint32_t pos = cursor + lanes_rank(mask);
-            if (pos < a.total_records) {
-                a.keys[pos] = idx;
-                a.ids[pos] = pos;
-                a.vals[pos] = make_float4(dsig * sgm, wgt * gr, wgt * gg, wgt * gb);
+        uint32_t ix[kCompBatch];
+        bool got[kCompBatch];
+#pragma unroll
+        for (int j = 0; j < kCompBatch; ++j) got[j] = it.next(a, ix[j]);
+        float4 q[kCompBatch];
+#pragma unroll
+        for (int j = 0; j < kCompBatch; ++j) q[j] = got[j] ? a.params[ix[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+#pragma unroll
+        for (int j = 0; j < kCompBatch; ++j) {
+            const uint64_t mask = __ballot(got[j]);
+            if (got[j]) {
+                const float sig = softplus(q[j].x);
+                const float e = expf(-sig);
+                const float al = -expm1f(-sig);
+                const float wgt = T * al;
+                pr += wgt * q[j].y;
+                pg += wgt * q[j].z;
+                pb += wgt * q[j].w;
+                const float tn = T * e;
+                const float sr = fr - pr, sg = fg - pg, sb = fb - pb;
+                const float dsig = gr * (tn * q[j].y - sr) + gg * (tn * q[j].z - sg) + gb * (tn * q[j].w - sb);
+                const float sgm = 1.0f / (1.0f + expf(-q[j].x));
+                const uint32_t pos = cursor + lanes_rank(mask);
+                if (pos < a.total_records) {
+                    a.keys[pos] = ix[j];
+                    a.ids[pos] = pos;
+                    a.vals[pos] = make_float4(dsig * sgm, wgt * gr, wgt * gg, wgt * gb);
+                }
+                T = tn;
             }
-            T = tn;
+            cursor += (uint32_t)__popcll(mask);
         }
-        cursor += (uint32_t)__popcll(mask);
+        if (__ballot(it.live) == 0) break;
     }
 }
 
@@ -293,8 +332,6 @@ __global__ void comp_zero_kernel(float4* g, size_t n) {
     if (i < n) g[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-// Records sorted by key: the first record of each key's run sums the run in
-// sorted order and writes that voxel's gradient (no atomics; deterministic).
 // Sum each run of equal keys of the sorted records into grad[key]. A wave
 // takes 64 consecutive records and does a segmented inclusive scan over them
 // (fixed shuffle order, so the result is the same on every call); the last
