@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--composite-steps", type=int, default=8)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch measurement")
     ap.add_argument("--no-brick", action="store_true", help="skip the brick n-tree measurement")
-    ap.add_argument("--only", choices=["composite", "batch", "brick"], default=None,
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain_512 measurement")
+    ap.add_argument("--only", choices=["composite", "batch", "brick", "plain"], default=None,
                     help="run one secondary workload alone and print its object (profiling)")
     ap.add_argument("--brick-frames", type=int, default=8)
     ap.add_argument("--autotune", action="store_true",
@@ -69,7 +70,9 @@ def main():
         dev = torch.device("cuda", 0)
         torch.cuda.set_device(dev)
         if args.only == "composite":
-            leg = composite_bench(yx, torch, dev, args.composite_steps)
+            leg = composite_bench(yx, torch, dev, args.composite_steps, not args.no_cpu_baseline)
+        elif args.only == "plain":
+            leg = plain_bench(yx, torch, not args.no_cpu_baseline)
         elif args.only == "batch":
             leg = batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
         else:
@@ -165,7 +168,9 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(scene, cams, w, h, mode, wl.SUN, args.cpu_frames)
-    fwd_bwd = None if args.no_composite else composite_bench(yx, torch, dev, args.composite_steps)
+    fwd_bwd = None if args.no_composite else composite_bench(yx, torch, dev, args.composite_steps,
+                                                            not args.no_cpu_baseline)
+    plain = None if args.no_plain else plain_bench(yx, torch, not args.no_cpu_baseline)
     batched = None if args.no_batch else batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
     brick = None if args.no_brick else brick_bench(yx, torch, args.brick_frames, not args.no_cpu_baseline)
 
@@ -191,7 +196,10 @@ def main():
                      "kernel": f"stack_kernel<{tuning['block']}, {tuning['window']}>",
                      "kernel_ms": round(avg_kernel_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": int(np.mean(bytes_f))},
+        "frame_ms": {"median": round(float(np.median(kern_ms)), 4), "min": round(float(np.min(kern_ms)), 4),
+                     "max": round(float(np.max(kern_ms)), 4)},
         "cpu_baseline": cpu,
+        "plain": plain,
         "fwd_bwd": fwd_bwd,
         "batched": batched,
         "brick": brick,
@@ -199,7 +207,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def composite_bench(yx, torch, dev, steps):
+def composite_bench(yx, torch, dev, steps, with_cpu=False):
     """fwd+bwd of the differentiable compositing workload composite_512: 512^3
     grid as 8^3 bricks (~10 % allocated), 16 views of 512x512 rays, L2 loss
     against targets rendered with raw densities perturbed by N(0, 0.1). One
@@ -246,17 +254,76 @@ def composite_bench(yx, torch, dev, steps):
     nvox = params.shape[0]
     # algorithmic bytes: forward 16 B per sample + 20 B per ray out; backward
     # 16 B per sample recomputed, 36 B per ray in (dL/dC, C, offset, count), a
-    # 20 B record per sample written, 3 radix passes reading and writing the
-    # 20 B records, the reduce (20 B in per record, 20 B out per key), the
-    # scatter (16 B per key) and the zeroed gradient (16 B per voxel)
+    # 24 B record per sample written (key, id, float4), 3 radix passes reading
+    # and writing the 8 B (key, id) pairs, the reduce (24 B in per record),
+    # 16 B out per key and the zeroed gradient (16 B per voxel)
     fwd_bytes = 16 * records + 20 * rays
-    bwd_bytes = 16 * records + 36 * rays + 20 * records + 3 * 40 * records + 20 * records + 36 * unique + 16 * nvox
-    return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "workload": "composite_512",
-            "rays_per_step": rays, "ms_per_step": round(dt * 1e3, 3), "fwd_ms": round(fwd_ms, 3),
-            "bwd_ms": round(bwd_ms, 3), "records": records, "unique_voxels": unique,
-            "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
-                         "fwd_achieved": round(fwd_bytes / (fwd_ms / 1e3) / 1e9, 1),
-                         "bwd_achieved": round(bwd_bytes / (bwd_ms / 1e3) / 1e9, 1)}}
+    bwd_bytes = 16 * records + 36 * rays + 24 * records + 3 * 16 * records + 24 * records + 16 * unique + 16 * nvox
+    res = {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "workload": "composite_512",
+           "rays_per_step": rays, "ms_per_step": round(dt * 1e3, 3), "fwd_ms": round(fwd_ms, 3),
+           "bwd_ms": round(bwd_ms, 3), "records": records, "unique_voxels": unique,
+           "roofline": {"bound": "hbm", "unit": "GB/s", "peak": HBM_PEAK_GBS,
+                        "fwd_achieved": round(fwd_bytes / (fwd_ms / 1e3) / 1e9, 1),
+                        "bwd_achieved": round(bwd_bytes / (bwd_ms / 1e3) / 1e9, 1)},
+           "cpu_baseline": None}
+    if with_cpu:
+        # the FP64 restatement's forward on one of the 16 views
+        co = entry.load_oracle().CompositeOracle()
+        threads = min(16, os.cpu_count() or 1)
+        v1 = comp_mod.orbit_views(nv, 60.0, w, h, seed=0)[:1]
+        t0 = time.perf_counter()
+        co.forward(vol.table, vol.params, grid, brick, v1, w, h, threads=threads)
+        cdt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(w * h / cdt / 1e6, 3), "unit": "Mrays/s (forward only, FP64)",
+                               "cores": threads, "kind": "port", "sample": f"view 0 of 16, {w}x{h} rays"}
+    return res
+
+
+def plain_bench(yx, torch, with_cpu, reps=8):
+    """plain_512: depth-8 octree (64 spheres), 8 orbit views of 512x512
+    primary rays, light at the eye; value = rays / mean kernel time of a view
+    (HIP events), CPU baseline = the restatement on all 8 views at 1, 4 and up
+    to 16 threads (SimpleRenderer / TreadedRenderer's 4 threads / all)."""
+    wl = yx.workloads
+    scene = yx.SVOData.build_synthetic(**wl.PLAIN_512)
+    cams = wl.orbit_cameras(8, seed=0)
+    r = yx.SVORenderer()
+    r.SetScene(scene)
+    r.SetFOV(60.0)
+    r.SetResolution(512, 512)
+    r.SetLight(wl.SUN, yx.YX_LIGHT_HEADLIGHT)
+    out = r.render_tensors(("hit_id", "voxel", "t", "rgba", "shadow"))
+    stream = torch.cuda.current_stream()
+    for pos, d in cams:
+        r.SetViewPos(pos)
+        r.SetViewDir(d)
+        r.Render(stream=stream, **out)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        for pos, d in cams:
+            r.SetViewPos(pos)
+            r.SetViewDir(d)
+            r.Render(stream=stream, **out)
+    b.record(stream)
+    torch.cuda.synchronize()
+    ms = a.elapsed_time(b) / (reps * len(cams))
+    res = {"value": round(512 * 512 / (ms / 1e3) / 1e6, 3), "unit": "Mrays/s", "workload": "plain_512",
+           "ms_per_view": round(ms, 4), "views": len(cams), "cpu_baseline": None}
+    if with_cpu:
+        orc = entry.load_oracle().Oracle()
+        nodes, root = scene.nodes(), scene.GetRoot()
+        by = {}
+        for th in (1, 4, min(16, os.cpu_count() or 1)):
+            t0 = time.perf_counter()
+            for pos, d in cams:
+                orc.render(nodes, root, pos, d, fov=60.0, w=512, h=512, threads=th)
+            by[str(th)] = round(len(cams) * 512 * 512 / (time.perf_counter() - t0) / 1e6, 4)
+        top = max(by, key=lambda k: int(k))
+        res["cpu_baseline"] = {"value": by[top], "unit": "Mrays/s", "cores": int(top), "kind": "port",
+                               "sample": "all 8 views, 512x512", "by_threads": by}
+    return res
 
 
 def batch_bench(yx, torch, frames, tune_cache):
