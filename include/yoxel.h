@@ -87,6 +87,11 @@ int yx_scene_save_vox(const yx_scene* scene, const char* path);
 int yx_scene_from_nodes(const uint32_t* nodes, uint32_t num_nodes, uint32_t root,
                         yx_scene** out);
 int yx_scene_build_synthetic(const yx_scene_desc* desc, yx_scene** out);
+/* The same scene built on the GPU (depth <= 12): the identical VoxNode array,
+ * root and counts as yx_scene_build_synthetic, returned as a host scene.
+ * SVO building is the step before the trace (DynamicSVO::BuildRange,
+ * ore/src/main.cpp:37-129); no GPU counterpart in the reference. */
+int yx_scene_build_synthetic_gpu(const yx_scene_desc* desc, yx_scene** out, void* stream);
 int yx_scene_get_info(const yx_scene* scene, yx_scene_info* info);
 /* SVOData::GetRoot / operator[]: borrowed view of the node array (valid until
  * yx_scene_free). */

@@ -63,3 +63,11 @@ def test_ntree_argument_checks(yx):
                                stream=0)
     with pytest.raises(yx.YoxelError, match="power of two"):
         yx.BrickTree.from_dense(np.zeros((24, 24, 24), np.uint8), 0)
+
+
+def test_gpu_scene_builder_argument_checks(yx):
+    for kw, msg in [(dict(depth=0), "depth"), (dict(depth=13), "depth"),
+                    (dict(depth=5, num_spheres=-1), "num_spheres"),
+                    (dict(depth=5, heightfield=True, hf_octaves=0), "hf_octaves")]:
+        with pytest.raises(yx.YoxelError, match=msg):
+            yx.SVOData.build_synthetic_gpu(stream=0, **kw)

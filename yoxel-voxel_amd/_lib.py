@@ -165,6 +165,7 @@ SIGNATURES = {
     "yx_scene_save_vox": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "yx_scene_from_nodes": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(_P)]),
     "yx_scene_build_synthetic": (ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(_P)]),
+    "yx_scene_build_synthetic_gpu": (ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.POINTER(_P), _P]),
     "yx_scene_get_info": (ctypes.c_int, [_P, ctypes.POINTER(SceneInfo)]),
     "yx_scene_nodes": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.POINTER(ctypes.c_uint32),
                                       ctypes.POINTER(ctypes.c_uint32)]),

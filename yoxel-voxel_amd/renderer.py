@@ -67,6 +67,19 @@ class SVOData:
         s._h = h
         return s
 
+    @classmethod
+    def build_synthetic_gpu(cls, depth, num_spheres=0, radius_min=0.0, radius_max=0.0, heightfield=False,
+                            hf_octaves=5, hf_amplitude=0.2, hf_base=0.3, seed=0, stream=None):
+        """The same scene as build_synthetic (identical nodes), built on the GPU."""
+        d = _lib.SceneDesc(depth, num_spheres, radius_min, radius_max, 1 if heightfield else 0,
+                           hf_octaves, hf_amplitude, hf_base, seed)
+        s = cls()
+        h = ctypes.c_void_p()
+        check(lib().yx_scene_build_synthetic_gpu(ctypes.byref(d), ctypes.byref(h),
+                                                 ctypes.c_void_p(_stream(stream))))
+        s._h = h
+        return s
+
     # -- accessors ---------------------------------------------------------
     def info(self):
         i = _lib.SceneInfo()
