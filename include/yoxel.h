@@ -168,6 +168,9 @@ typedef struct yx_tuning {
     int32_t window;       /* stack levels kept in registers: 0..6 (stack kernel
                              uses 0, 2, 4, 6; replay kernel 0, 1, 2)           */
     int32_t waves_per_cu; /* persistent launch size; 0 = occupancy limit        */
+    int32_t lds_levels;   /* stack kernel, single scene: node records of tree
+                             levels 0..lds_levels-1 staged in LDS per workgroup
+                             (0 = none; fewer levels when LDS is short)        */
 } yx_tuning;
 int yx_renderer_set_tuning(yx_renderer* r, const yx_tuning* tuning);
 int yx_renderer_get_tuning(const yx_renderer* r, yx_tuning* tuning);

@@ -23,6 +23,10 @@ TUNINGS = [
     dict(kernel=1, block=128, window=2, tile_w=16, tile_h=16, refill_min=1, queues=3),
     dict(kernel=1, block=64, window=6, tile_w=32, tile_h=2, refill_min=64, queues=64, waves_per_cu=2),
     dict(kernel=1, block=256, window=4, tile_w=4, tile_h=16, refill_min=20, queues=2),
+    # top tree levels staged in LDS (all levels asked: as many as fit)
+    dict(kernel=1, lds_levels=3),
+    dict(kernel=1, block=256, window=2, lds_levels=5, refill_min=16),
+    dict(kernel=1, block=128, lds_levels=16, tile_w=16, tile_h=4),
 ]
 
 

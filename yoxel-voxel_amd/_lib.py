@@ -87,6 +87,7 @@ class Tuning(ctypes.Structure):
         ("queues", ctypes.c_int32),
         ("window", ctypes.c_int32),
         ("waves_per_cu", ctypes.c_int32),
+        ("lds_levels", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -51,7 +51,7 @@ struct yx_batch {
     uint32_t* d_counters = nullptr;
     int device = -1, num_cus = 0;
     size_t max_lds = 0;
-    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 32, 1, 0, 0};
+    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 32, 1, 0, 0, 0};
     // graph of one frame
     bool use_graph = true;
     hipStream_t cap_stream = nullptr;
@@ -222,6 +222,7 @@ static int check_batch_tuning(const yx_tuning* t) {
     if (t->window != 0 && t->window != 2 && t->window != 4)
         return fail(YX_ERR_INVALID_ARG, "window must be 0, 2 or 4");
     if (t->waves_per_cu < 0 || t->waves_per_cu > 32) return fail(YX_ERR_INVALID_ARG, "waves_per_cu must be 0..32");
+    if (t->lds_levels != 0) return fail(YX_ERR_INVALID_ARG, "batches stage no tree levels in LDS (lds_levels 0)");
     return YX_OK;
 }
 

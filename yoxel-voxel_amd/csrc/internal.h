@@ -67,6 +67,9 @@ struct GpuTree {
     int depth = 0;               // deepest leaf level
     int node_levels = 0;         // deepest node level + 1 (root only: 1)
     bool empty = true;
+    // level_records[l]: records of levels 0..l (the root is level 0); the
+    // breadth-first order puts them first, so they are a prefix of `recs`
+    std::vector<uint32_t> level_records;
 };
 int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base = 0);
 
@@ -109,9 +112,11 @@ namespace yx {
 // stay fixed; sweeps repeat until one changes nothing (at most 3). A new value
 // must beat the current one by 1 % to replace it, so timing noise does not
 // move the result. Returns the best tuning found in *best.
+// With stage_levels the number of top tree levels staged in LDS is one more
+// axis (single-scene launches only).
 template <class TimeFn>
 int tune_coordinate_descent(const yx_tuning& start, TimeFn time_ms, yx_tune_trial* trials, int32_t cap,
-                            int32_t* n_trials, yx_tuning* best) {
+                            int32_t* n_trials, yx_tuning* best, bool stage_levels = false) {
     struct Seen {
         yx_tuning t;
         float ms;
@@ -121,7 +126,7 @@ int tune_coordinate_descent(const yx_tuning& start, TimeFn time_ms, yx_tune_tria
     auto same = [](const yx_tuning& a, const yx_tuning& b) {
         return a.kernel == b.kernel && a.block == b.block && a.tile_w == b.tile_w && a.tile_h == b.tile_h &&
                a.refill_min == b.refill_min && a.queues == b.queues && a.window == b.window &&
-               a.waves_per_cu == b.waves_per_cu;
+               a.waves_per_cu == b.waves_per_cu && a.lds_levels == b.lds_levels;
     };
     auto eval = [&](const yx_tuning& t, float* ms) -> int {
         for (const Seen& s : seen)
@@ -145,6 +150,7 @@ int tune_coordinate_descent(const yx_tuning& start, TimeFn time_ms, yx_tune_tria
     static const int windows[] = {0, 2, 4};
     static const int blocks[] = {64, 128, 256};
     static const int queues[] = {1, 8, 32};
+    static const int stages[] = {0, 2, 3, 4, 5};
     yx_tuning cur = start;
     cur.kernel = YX_KERNEL_PERSISTENT;
     float cur_ms = 0.0f;
@@ -183,6 +189,10 @@ int tune_coordinate_descent(const yx_tuning& start, TimeFn time_ms, yx_tune_tria
         moved += e;
         if ((e = try_axis([](yx_tuning& t, int i) { t.queues = queues[i]; }, 3)) < 0) return e;
         moved += e;
+        if (stage_levels) {
+            if ((e = try_axis([](yx_tuning& t, int i) { t.lds_levels = stages[i]; }, 5)) < 0) return e;
+            moved += e;
+        }
         if (moved == 0) break;
     }
     if (n_trials) *n_trials = n;

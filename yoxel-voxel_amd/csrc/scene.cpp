@@ -538,12 +538,16 @@ int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base) {
     q_lvl.push_back(0);
     seen[s.root] = 1;
     int depth = 0, max_node_level = 0;
+    out.level_records.assign(1, 1u);  // level 0: the root record
     for (size_t h = 0; h < q_id.size(); ++h) {
         const uint32_t id = q_id[h];
         const uint32_t* n = &s.nodes[(size_t)id * 10];
         const uint32_t leaf_mask = n[0] & 0xffu;
         uint32_t child_mask = 0;
         const int lvl = q_lvl[h];
+        // the first node of a level: every record of that level is appended
+        while ((int)out.level_records.size() <= lvl)
+            out.level_records.push_back((uint32_t)(out.recs.size() / 2));
         max_node_level = std::max(max_node_level, lvl);
         for (int c = 0; c < 8; ++c) {
             if (leaf_mask & (1u << c)) continue;
@@ -584,6 +588,7 @@ int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base) {
     out.depth = depth;
     out.node_levels = max_node_level + 1;
     out.empty = (depth == 0);
+    out.level_records.push_back((uint32_t)(out.recs.size() / 2));  // the deepest level's records
     return YX_OK;
 }
 

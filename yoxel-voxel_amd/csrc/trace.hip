@@ -120,6 +120,7 @@ struct yx_renderer {
     size_t n_recs = 0;
     int depth = 0;        // deepest leaf level
     int node_levels = 0;  // deepest node level + 1
+    std::vector<uint32_t> level_records;  // records of levels 0..l (a prefix)
     uint32_t* d_frame = nullptr;  // RenderFrame's device colour buffer
     std::vector<uint32_t> h_frame;
     uint32_t* d_counters = nullptr;  // ray queue heads + retired-wave count
@@ -128,7 +129,7 @@ struct yx_renderer {
     size_t max_lds = 0;  // dynamic LDS bytes a workgroup may use
     // measured best on MI355X for the 1080p depth-11 frame (stack kernel, one
     // wave per workgroup, no register window, refill at 32 idle lanes)
-    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 32, 1, 0, 0};
+    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 32, 1, 0, 0, 0};
 };
 
 #define YX_HIP(call)                                                                        \
@@ -247,6 +248,16 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
             block /= 2;
             lds = lane_bytes * (size_t)block;
         }
+        // top tree levels staged in LDS: the records of levels 0..lds_levels-1
+        // (a prefix of the breadth-first array), as many levels as fit
+        if (t.lds_levels > 0 && !r->level_records.empty()) {
+            int lv = std::min<int>(t.lds_levels, (int)r->level_records.size()) - 1;
+            while (lv >= 0 && lds + (size_t)r->level_records[(size_t)lv] * 8u > r->max_lds) --lv;
+            if (lv >= 0) {
+                a.lds_recs = r->level_records[(size_t)lv];
+                lds += ((size_t)a.lds_recs * 8u + 15u) & ~(size_t)15u;
+            }
+        }
         if (lds <= r->max_lds) {
             if (block == 64) {
                 if (win == 0) return YX_LAUNCH((stack_kernel<64, 0, false>), 64);
@@ -319,6 +330,8 @@ extern "C" int yx_renderer_set_tuning(yx_renderer* r, const yx_tuning* t) {
     if (t->queues < 1 || t->queues > kMaxQueues) return fail(YX_ERR_INVALID_ARG, "queues must be 1..%d", kMaxQueues);
     if (t->window < 0 || t->window > 6) return fail(YX_ERR_INVALID_ARG, "window must be 0..6");
     if (t->waves_per_cu < 0 || t->waves_per_cu > 32) return fail(YX_ERR_INVALID_ARG, "waves_per_cu must be 0..32");
+    if (t->lds_levels < 0 || t->lds_levels > kMaxDepth)
+        return fail(YX_ERR_INVALID_ARG, "lds_levels must be 0..%d", kMaxDepth);
     r->tune = *t;
     return YX_OK;
 }
@@ -359,6 +372,9 @@ extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
         r->n_recs = t.recs.size() / 2;
         r->depth = t.depth;
         r->node_levels = t.node_levels;
+        r->level_records = t.level_records;
+    } else {
+        r->level_records.clear();
     }
     r->has_scene = true;
     return YX_OK;
@@ -491,7 +507,7 @@ extern "C" int yx_renderer_autotune(yx_renderer* r, const yx_outputs* out, int32
         return time_frames([&]() { return launch_frame(r, *out, st); }, reps, st, ms);
     };
     yx_tuning best = keep;
-    const int rc = tune_coordinate_descent(keep, time_ms, trials, cap, n_trials, &best);
+    const int rc = tune_coordinate_descent(keep, time_ms, trials, cap, n_trials, &best, true);
     r->tune = rc == YX_OK ? best : keep;
     return rc;
 }

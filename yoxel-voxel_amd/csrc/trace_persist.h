@@ -64,6 +64,8 @@ struct PersistArgs {
     int tiles_x;
     int refill_min;
     int levels;  // LDS rows per lane (node levels of the tree)
+    // stack kernel: records [0, lds_recs) (the top tree levels) staged in LDS
+    uint32_t lds_recs;
     // batched launches only
     const BatchJob* jobs;
     const uint32_t* job_tiles;  // first tile of each job, n_jobs + 1 entries (last = all tiles)

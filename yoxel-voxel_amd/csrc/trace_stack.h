@@ -125,7 +125,11 @@ enum : int {
 // root record and output offset per job) instead of the single view in `a`.
 template <int BLOCK, int W, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
-    extern __shared__ uint4 sh_stk[];  // [levels - W][2][BLOCK]
+    // dynamic LDS: the staged top-level records (single scene), then the
+    // traversal stack rows
+    extern __shared__ uint4 sh_mem[];
+    uint2* sh_rec = reinterpret_cast<uint2*>(sh_mem);
+    uint4* sh_stk = sh_mem + ((a.lds_recs * 8u + 15u) >> 4);
     const int tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint2* __restrict__ recs = a.recs;
@@ -133,6 +137,24 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     QueueCursor qc;
     qc.q = (int)(blockIdx.x % (uint32_t)a.nq);
     qc.left = a.nq;
+
+    // records [0, lds_recs) are the top tree levels (breadth-first order)
+    if constexpr (!BATCH) {
+        for (uint32_t i = (uint32_t)tid; i < a.lds_recs; i += BLOCK) sh_rec[i] = recs[i];
+        __syncthreads();
+    }
+    auto rec_at = [&](uint32_t i) -> uint2 {
+        if constexpr (BATCH) {
+            return recs[i];
+        } else {
+            uint2 v;
+            if (i < a.lds_recs)
+                v = sh_rec[i];
+            else
+                v = recs[i];
+            return v;
+        }
+    };
 
     int status = sIdle;
     bool adv = false, occluded = false;
@@ -209,7 +231,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                 const int real = ch ^ (int)flags;
                 const uint32_t cm = node.y & 0xffu, lm = (node.y >> 8) & 0xffu;
                 const uint32_t below = (cm | lm) & ((1u << real) - 1u);
-                const uint2 leaf = recs[node.x + (uint32_t)__builtin_popcount(below)];
+                const uint2 leaf = rec_at(node.x + (uint32_t)__builtin_popcount(below));
                 int vx = 0, vy = 0, vz = 0;
                 for (int j = 0; j < level; ++j) {
                     const uint32_t s = bits3(chpath, j) ^ flags;
@@ -310,7 +332,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                                 min3(c.t2x, c.t2y, c.t2z) > 0;
                 if (ok) {
                     ++visits;
-                    node = recs[root];
+                    node = rec_at(root);
                     level = 0;
                     adv = false;
                     ch = find_first_child(c);
@@ -357,7 +379,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                 }
                 chpath = set3(chpath, level, (uint32_t)ch);
                 const uint32_t below = (cm | lm) & ((1u << real) - 1u);
-                node = recs[node.x + (uint32_t)__builtin_popcount(below)];
+                node = rec_at(node.x + (uint32_t)__builtin_popcount(below));
                 ++level;
                 ch = find_first_child(c);
                 if constexpr (W == 0) ffcpath = set3(ffcpath, level, (uint32_t)ch);
