@@ -248,14 +248,18 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
             block /= 2;
             lds = lane_bytes * (size_t)block;
         }
-        // top tree levels staged in LDS: the records of levels 0..lds_levels-1
-        // (a prefix of the breadth-first array), as many levels as fit
-        if (t.lds_levels > 0 && !r->level_records.empty()) {
+        // top tree levels staged in LDS (window 0 only): the records of levels
+        // 0..lds_levels-1 (a prefix of the breadth-first array), as many
+        // levels as fit
+        if (t.lds_levels > 0 && win == 0 && !r->level_records.empty()) {
             int lv = std::min<int>(t.lds_levels, (int)r->level_records.size()) - 1;
             while (lv >= 0 && lds + (size_t)r->level_records[(size_t)lv] * 8u > r->max_lds) --lv;
             if (lv >= 0) {
                 a.lds_recs = r->level_records[(size_t)lv];
                 lds += ((size_t)a.lds_recs * 8u + 15u) & ~(size_t)15u;
+                if (block == 64) return YX_LAUNCH((stack_kernel<64, 0, false, true>), 64);
+                if (block == 128) return YX_LAUNCH((stack_kernel<128, 0, false, true>), 128);
+                return YX_LAUNCH((stack_kernel<256, 0, false, true>), 256);
             }
         }
         if (lds <= r->max_lds) {

@@ -123,7 +123,8 @@ enum : int {
 // (a shift window); older levels live in LDS rows indexed by level.
 // BATCH: rays come from the (scene, camera) jobs of a.jobs (camera, light,
 // root record and output offset per job) instead of the single view in `a`.
-template <int BLOCK, int W, bool BATCH>
+// STAGE: node records [0, a.lds_recs) are read from an LDS copy (single scene).
+template <int BLOCK, int W, bool BATCH, bool STAGE = false>
 __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     // dynamic LDS: the staged top-level records (single scene), then the
     // traversal stack rows
@@ -139,12 +140,12 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     qc.left = a.nq;
 
     // records [0, lds_recs) are the top tree levels (breadth-first order)
-    if constexpr (!BATCH) {
+    if constexpr (STAGE) {
         for (uint32_t i = (uint32_t)tid; i < a.lds_recs; i += BLOCK) sh_rec[i] = recs[i];
         __syncthreads();
     }
     auto rec_at = [&](uint32_t i) -> uint2 {
-        if constexpr (BATCH) {
+        if constexpr (!STAGE) {
             return recs[i];
         } else {
             uint2 v;
