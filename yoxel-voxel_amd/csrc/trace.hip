@@ -171,8 +171,13 @@ static int launch_persistent(yx_renderer* r, const void* fn, void (*kern)(Persis
         YX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, block, lds));
         per_cu = std::max(1, occ);
     }
-    const uint32_t grid = (uint32_t)(per_cu * r->num_cus);
-    a.n_waves = grid * (uint32_t)(block / 64);
+    uint32_t grid = (uint32_t)(per_cu * r->num_cus);
+    // no more waves than 64-ray packets: a small image would otherwise start
+    // waves that find the queue empty
+    const uint32_t waves_per_block = (uint32_t)(block / 64);
+    const uint32_t useful = (a.total / 64u + waves_per_block - 1) / waves_per_block;
+    grid = std::max(1u, std::min(grid, useful));
+    a.n_waves = grid * waves_per_block;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(block), lds, stream, a);
     YX_HIP(hipGetLastError());
     return YX_OK;
