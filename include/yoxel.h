@@ -192,6 +192,62 @@ typedef struct yx_tune_trial {
 int yx_renderer_autotune(yx_renderer* r, const yx_outputs* out, int32_t reps, yx_tune_trial* trials,
                          int32_t cap, int32_t* n_trials, void* stream);
 
+/* ---- the reference GPU pipeline: LOD trace, depth blur, Phong shade -------
+ * SVORenderer::Render (demo/SVORenderer.cpp:95-149) runs Run_Trace with a
+ * level-of-detail stop, five depth-limited blurs of the z buffer (Run_BlurZ,
+ * "SSNA") and Run_ShadeSimple with point lights and normals taken from the z
+ * buffer. Its kernels are missing from the reference; the definitions this
+ * library implements:
+ *  - LOD stop: a node below the root whose cell edge 2^-level is smaller than
+ *    its entry distance times detail = coef * rad(fov/2) / width
+ *    (demo/SVORenderer.cpp:104), and which has a non-null child, ends the ray;
+ *    its colour is the mean RGBA8 of the leaves under it (VoxNode::data,
+ *    reaction/report/main.tex:61). coef 0 traces exactly.
+ *  - z = t / |dir0 + du x + dv y|, the view depth (+inf on a miss).
+ *  - blur pass i (i = 0..4, blurSize = 3 + 3i): pixels with z < 5 vox /
+ *    (pixelAng blurSize) (vox = 2^-depth, pixelAng = rad(fov) / width) become
+ *    the 5x5 Gaussian (SVORenderer::InitBlur) mean of the window pixels whose
+ *    z differs by less than 4 vox, renormalised by their weights.
+ *  - normals from z by one-sided differences of the smaller magnitude
+ *    (demo/dumps/ztools.py:23-42); Phong with ambient 0.1 and specular
+ *    exponent 10 (demo/SVORenderer.cpp:112-113) summed over the enabled lights,
+ *    each attenuated by 1 / (c0 + c1 d + c2 d^2).
+ * Primary rays only; no shadow rays. */
+#define YX_MAX_LIGHTS 4
+typedef struct yx_light_params { /* LightParams, demo/Demo.cpp:141-146 */
+    int32_t enabled;
+    float pos[3];         /* world position                                  */
+    float diffuse[3];     /* per-channel diffuse intensity                   */
+    float specular[3];    /* per-channel specular intensity                  */
+    float attenuation[3]; /* constant, linear, quadratic coefficients        */
+} yx_light_params;
+typedef struct yx_pipeline_outputs {
+    uint32_t* rgba;   /* shaded RGBA8 (r low byte); (0,0,0,0) on a miss     */
+    float* z;         /* view depth after the blur passes; +inf on a miss   */
+    float* t;         /* hit distance along the unit ray, -1 on a miss      */
+    uint32_t* albedo; /* leaf colour, or the node's mean colour on an LOD
+                         stop; 0 on a miss                                  */
+    uint8_t* level;   /* level of the hit cell (1 = child of the root); 0 on
+                         a miss                                             */
+    uint32_t* visits; /* node records read                                  */
+} yx_pipeline_outputs;
+/* SVORenderer::SetDetailCoef / GetDetailCoef (demo/SVORenderer.h:25-26);
+ * default 1. coef >= 0. */
+int yx_renderer_set_detail_coef(yx_renderer* r, float coef);
+int yx_renderer_get_detail_coef(const yx_renderer* r, float* coef);
+/* SVORenderer::SetSSNA / GetSSNA (demo/SVORenderer.h:28-29); default on. */
+int yx_renderer_set_ssna(yx_renderer* r, int32_t enable);
+int yx_renderer_get_ssna(const yx_renderer* r, int32_t* enable);
+/* SVORenderer::SetShowNormals (demo/SVORenderer.h:31): RGB = normal * 0.5 + 0.5. */
+int yx_renderer_set_show_normals(yx_renderer* r, int32_t enable);
+/* SVORenderer::SetLigth (demo/SVORenderer.h:34); i in 0..YX_MAX_LIGHTS-1, all
+ * disabled by default. */
+int yx_renderer_set_point_light(yx_renderer* r, int32_t i, const yx_light_params* light);
+/* SVORenderer::Render(void*) (demo/SVORenderer.cpp:95-149) into caller-owned
+ * device buffers on `stream` (asynchronous). Any output may be NULL; the
+ * renderer keeps its own z and colour work buffers. */
+int yx_render_pipeline(yx_renderer* r, const yx_pipeline_outputs* out, void* stream);
+
 /* ---- batched rendering: many scenes x many cameras in one launch ---------
  * Replaces a loop of per-scene SetScene / SetView / Render calls
  * (demo/Demo.cpp:138-174 drives one SVORenderer per scene and view) with one

@@ -335,3 +335,177 @@ class NTreeOracle:
 
     def plant(self):
         return self.lib.oracle_plant()
+
+
+LIGHT_DTYPE = np.dtype([("enabled", "<i4"), ("pos", "<f4", 3), ("diffuse", "<f4", 3), ("specular", "<f4", 3),
+                        ("attenuation", "<f4", 3)])  # 52 bytes, as o_light / yx_light_params
+MAX_LIGHTS = 4
+
+
+def make_lights(specs):
+    """specs: list of dicts with pos, diffuse, specular, attenuation (enabled
+    unless enabled=False); returns the MAX_LIGHTS-entry light table."""
+    lt = np.zeros(MAX_LIGHTS, LIGHT_DTYPE)
+    for i, s in enumerate(specs):
+        lt[i]["enabled"] = 1 if s.get("enabled", True) else 0
+        for k in ("pos", "diffuse", "specular", "attenuation"):
+            lt[i][k] = np.asarray(s[k], np.float32)
+    return lt
+
+
+class PipelineOracle:
+    """The reference GPU pipeline (SVORenderer::Render, demo/SVORenderer.cpp:
+    95-149) restated on the CPU: LOD colours, LOD trace (svo_oracle.c), BlurZ x5
+    and ShadeSimple (ssna_oracle.c)."""
+
+    def __init__(self, path=None):
+        path = pathlib.Path(path) if path else LIB
+        if not path.exists():
+            build(out=path)
+        l = ctypes.CDLL(str(path))
+        vp, ci, cf = ctypes.c_void_p, ctypes.c_int, ctypes.c_float
+        l.oracle_lod_colors.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp]
+        l.oracle_lod_colors.restype = ci
+        l.oracle_lod_render_rows.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, vp, vp, vp, ci, ci, ci, ci, cf,
+                                             vp, vp, vp, vp, vp]
+        l.oracle_lod_render_rows.restype = ctypes.c_longlong
+        l.oracle_lod_trace_rays.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32, ci, vp, vp, cf, vp, vp, vp, vp]
+        l.oracle_lod_trace_rays.restype = ci
+        l.oracle_blur_kernel.argtypes = [vp]
+        l.oracle_blur_kernel.restype = None
+        l.oracle_pipeline_params.argtypes = [cf, ci, ci, cf, vp]
+        l.oracle_pipeline_params.restype = None
+        l.oracle_blur_z.argtypes = [ci, ci, vp, vp, cf, cf, vp, ci, ci]
+        l.oracle_blur_z.restype = None
+        l.oracle_view_frame.argtypes = [vp, vp, vp]
+        l.oracle_view_frame.restype = None
+        l.oracle_shade_simple.argtypes = [ci, ci, vp, vp, vp, vp, vp, cf, vp, ci, cf, ci, vp, ci, ci]
+        l.oracle_shade_simple.restype = None
+        l.oracle_init_ray_dir.argtypes = [vp, vp, cf, ci, ci, vp]
+        l.oracle_init_ray_dir.restype = None
+        l.oracle_plant.restype = ci
+        self.lib = l
+
+    def lod_colors(self, nodes, root):
+        """Mean RGBA8 of the leaves under every node reachable from `root`,
+        indexed by VoxNode id (0 elsewhere)."""
+        nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
+        nn = nodes.size // 10
+        out = np.zeros(max(nn, 1), np.uint32)
+        scratch = np.zeros(5 * max(nn, 1), np.uint64)
+        stack = np.zeros(2 * max(nn, 1) + 2, np.uint32)
+        rc = self.lib.oracle_lod_colors(nodes.ctypes.data, nn, int(root), out.ctypes.data, scratch.ctypes.data,
+                                        stack.ctypes.data)
+        if rc < 0:
+            raise ValueError("bad child id or shared node")
+        return out
+
+    def params(self, fov, w, depth, coef):
+        out = np.zeros(10, np.float32)
+        self.lib.oracle_pipeline_params(float(fov), int(w), int(depth), float(coef), out.ctypes.data)
+        return {"detail": out[0], "pixel_ang": out[1], "pix": out[2], "vox": out[3], "edge": out[4],
+                "zlimits": out[5:10].copy()}
+
+    def basis(self, direction, up, fov, w, h):
+        d = np.asarray(direction, np.float32)
+        u = np.asarray(up, np.float32)
+        out = np.zeros(9, np.float32)
+        self.lib.oracle_init_ray_dir(d.ctypes.data, u.ctypes.data, float(fov), int(w), int(h), out.ctypes.data)
+        return out
+
+    def view_frame(self, direction, up):
+        d = np.asarray(direction, np.float32)
+        u = np.asarray(up, np.float32)
+        out = np.zeros(9, np.float32)
+        self.lib.oracle_view_frame(d.ctypes.data, u.ctypes.data, out.ctypes.data)
+        return out
+
+    def lod_trace_rays(self, nodes, root, pos, dirs, detail):
+        nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = dirs.shape[0]
+        pos = np.ascontiguousarray(np.broadcast_to(np.asarray(pos, np.float32).reshape(-1, 3), (n, 3)))
+        node = np.zeros(n, np.uint32)
+        child = np.zeros(n, np.int32)
+        t = np.zeros(n, np.float32)
+        lvl = np.zeros(n, np.int32)
+        rc = self.lib.oracle_lod_trace_rays(nodes.ctypes.data, nodes.size // 10, int(root), n, pos.ctypes.data,
+                                            dirs.ctypes.data, float(detail), node.ctypes.data, child.ctypes.data,
+                                            t.ctypes.data, lvl.ctypes.data)
+        if rc < 0:
+            raise ValueError("oracle followed a child id out of range")
+        return {"node": node, "child": child, "t": t, "level": lvl}
+
+    def lod_render(self, nodes, root, lod, pos, basis, w, h, detail, threads=8, rows=None):
+        nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
+        lod = np.ascontiguousarray(lod, np.uint32)
+        p = np.asarray(pos, np.float32)
+        b = np.asarray(basis, np.float32)
+        out = {"t": np.full((h, w), -1.0, np.float32), "z": np.full((h, w), np.inf, np.float32),
+               "albedo": np.zeros((h, w), np.uint32), "level": np.zeros((h, w), np.uint8),
+               "visits": np.zeros((h, w), np.uint32)}
+        y0, y1 = (0, h) if rows is None else rows
+        rays = []
+
+        def run(a, c):
+            rays.append(self.lib.oracle_lod_render_rows(
+                nodes.ctypes.data, nodes.size // 10, int(root), lod.ctypes.data, p.ctypes.data, b.ctypes.data, w, h,
+                a, c, float(detail), out["t"].ctypes.data, out["z"].ctypes.data, out["albedo"].ctypes.data,
+                out["level"].ctypes.data, out["visits"].ctypes.data))
+
+        _run_threads(run, [(y0 + a, y0 + c) for a, c in _bands(y1 - y0, threads)])
+        if any(r < 0 for r in rays):
+            raise ValueError("oracle followed a child id out of range")
+        return out
+
+    def blur_kernel(self):
+        k = np.zeros(25, np.float32)
+        self.lib.oracle_blur_kernel(k.ctypes.data)
+        return k
+
+    def blur_z(self, z, zlimit, edge, kern=None, threads=8):
+        z = np.ascontiguousarray(z, np.float32)
+        h, w = z.shape
+        kern = self.blur_kernel() if kern is None else np.ascontiguousarray(kern, np.float32)
+        out = np.empty_like(z)
+        _run_threads(lambda a, c: self.lib.oracle_blur_z(w, h, z.ctypes.data, out.ctypes.data, float(zlimit),
+                                                         float(edge), kern.ctypes.data, a, c), _bands(h, threads))
+        return out
+
+    def shade(self, z, albedo, basis, eye, frame, pix, lights, ambient=0.1, show_normals=False, threads=8):
+        z = np.ascontiguousarray(z, np.float32)
+        albedo = np.ascontiguousarray(albedo, np.uint32)
+        h, w = z.shape
+        b = np.ascontiguousarray(basis, np.float32)
+        e = np.ascontiguousarray(eye, np.float32)
+        f = np.ascontiguousarray(frame, np.float32)
+        lt = np.ascontiguousarray(lights, LIGHT_DTYPE)
+        out = np.zeros((h, w), np.uint32)
+        _run_threads(lambda a, c: self.lib.oracle_shade_simple(
+            w, h, z.ctypes.data, albedo.ctypes.data, b.ctypes.data, e.ctypes.data, f.ctypes.data, float(pix),
+            lt.ctypes.data, lt.size, float(ambient), 1 if show_normals else 0, out.ctypes.data, a, c),
+            _bands(h, threads))
+        return out
+
+    def pipeline(self, nodes, root, depth, pos, direction, up=(0, 0, 1), fov=70.0, w=64, h=64, detail_coef=1.0,
+                 ssna=True, lights=None, show_normals=False, threads=8, lod=None):
+        """SVORenderer::Render restated: returns t, z (raw), zb (blurred, or z
+        when ssna is off), albedo, level, visits, rgba."""
+        if lod is None:
+            lod = self.lod_colors(nodes, root)
+        prm = self.params(fov, w, depth, detail_coef)
+        basis = self.basis(direction, up, fov, w, h)
+        out = self.lod_render(nodes, root, lod, pos, basis, w, h, prm["detail"], threads)
+        zb = out["z"]
+        if ssna:
+            k = self.blur_kernel()
+            for zl in prm["zlimits"]:
+                zb = self.blur_z(zb, zl, prm["edge"], k, threads)
+        out["zb"] = zb
+        lights = make_lights([]) if lights is None else lights
+        out["rgba"] = self.shade(zb, out["albedo"], basis, pos, self.view_frame(direction, up), prm["pix"],
+                                 lights, 0.1, show_normals, threads)
+        return out
+
+    def plant(self):
+        return self.lib.oracle_plant()

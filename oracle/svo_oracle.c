@@ -202,15 +202,16 @@ typedef struct {
     uint32_t num_nodes;
     unsigned dir_flags;
     int occlusion;   /* DEFINED HERE: shadow rays skip leaf cells behind the origin */
+    float detail;    /* LOD stop coefficient (RenderParams::detailCoef); 0 = exact */
     uint32_t visits; /* node records read: RecTrace calls past the null/behind test */
     int bad_id;      /* a child id >= num_nodes was followed */
 } trace_ctx;
 
 typedef struct {
     uint32_t node;
-    int child;
+    int child; /* -1: the ray stopped on the node itself (LOD) */
     float t;
-    int level; /* leaf level: 1 for a child of the root */
+    int level; /* level of the hit cell: 1 for a child of the root */
     int vx, vy, vz;
     float t1[3]; /* entry times of the hit cell (entry-face normal) */
 } trace_hit;
@@ -219,6 +220,13 @@ typedef struct {
  * from the special-value rule of reaction/report/main.tex:52-55. */
 static int is_null(uint32_t id) { return (id & 0x80000000u) != 0; }
 static int leaf_flag(uint32_t flags, int i) { return (int)((flags >> i) & 1u); }
+
+/* a leaf or a non-null child id in some slot */
+static int has_child(const uint32_t* node) {
+    for (int i = 0; i < 8; ++i)
+        if (leaf_flag(node[0], i) || !is_null(node[2 + i])) return 1;
+    return 0;
+}
 
 static int rec_trace(trace_ctx* c, uint32_t node_id, const float* t1_in, const float* t2_in,
                      int level, int px, int py, int pz, trace_hit* res) {
@@ -231,6 +239,26 @@ static int rec_trace(trace_ctx* c, uint32_t node_id, const float* t1_in, const f
     }
     c->visits++;
     const uint32_t* node = c->nodes + (size_t)10 * node_id;
+    /* LOD stop, DEFINED HERE after RenderParams::detailCoef (demo/SVORenderer.cpp:
+     * 104) and RayData::endNodeChild < 0 (:171-179): a node below the root whose
+     * cell edge 2^-level is smaller than its entry distance times `detail`, and
+     * which has a non-null child, is the hit itself (its colour is node data). */
+#if ORACLE_PLANT == 9
+    float edge = ldexpf(1.0f, -(level + 1)); /* planted: half the cell edge */
+#else
+    float edge = ldexpf(1.0f, -level);
+#endif
+    if (level > 0 && has_child(node) && edge < max_coord(t1) * c->detail) {
+        res->node = node_id;
+        res->child = -1;
+        res->t = max_coord(t1);
+        res->level = level;
+        res->vx = px;
+        res->vy = py;
+        res->vz = pz;
+        memcpy(res->t1, t1, sizeof(t1));
+        return 1;
+    }
     int ch = find_first_child(t1, t2);
     for (;;) {
 #if ORACLE_PLANT == 3
@@ -270,8 +298,8 @@ static int entry_axis(const float* t1) {
 /* Trace one ray from `pos` along the (already normalized) `dir`. Returns 1 on a
  * hit. mode: 0 primary (reference RecTrace), 1 occlusion (shadow rays). */
 static int trace_ray(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, const float* pos,
-                     const float* dir_in, int occlusion, trace_hit* hit, uint32_t* visits,
-                     int* bad_id) {
+                     const float* dir_in, int occlusion, float detail, trace_hit* hit,
+                     uint32_t* visits, int* bad_id) {
     float d[3] = {adjust_coord(dir_in[0]), adjust_coord(dir_in[1]), adjust_coord(dir_in[2])};
     float t1[3], t2[3];
     unsigned flags;
@@ -281,6 +309,7 @@ static int trace_ray(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, c
     c.num_nodes = num_nodes;
     c.dir_flags = flags;
     c.occlusion = occlusion;
+    c.detail = detail;
     c.visits = 0;
     c.bad_id = 0;
     int r = rec_trace(&c, root, t1, t2, 0, 0, 0, 0, hit);
@@ -316,7 +345,7 @@ static void shade_pixel(const uint32_t* nodes, uint32_t num_nodes, uint32_t root
     dir[1] = adjust_coord(dir[1]);
     dir[2] = adjust_coord(dir[2]);
     trace_hit h;
-    if (!trace_ray(nodes, num_nodes, root, pos, dir, 0, &h, &o->visits, bad_id)) return;
+    if (!trace_ray(nodes, num_nodes, root, pos, dir, 0, 0.0f, &h, &o->visits, bad_id)) return;
     o->hit_id = h.node * 8u + (uint32_t)h.child;
     o->vox[0] = h.vx;
     o->vox[1] = h.vy;
@@ -342,7 +371,7 @@ static void shade_pixel(const uint32_t* nodes, uint32_t num_nodes, uint32_t root
         else
             p[e] = p[e] - half;
         trace_hit sh;
-        lit = !trace_ray(nodes, num_nodes, root, p, light, 1, &sh, &o->visits, bad_id);
+        lit = !trace_ray(nodes, num_nodes, root, p, light, 1, 0.0f, &sh, &o->visits, bad_id);
         o->shadow = lit ? 0 : 1;
     }
     float k = lit ? 0.1f + 0.9f * lam : 0.1f;
@@ -399,7 +428,7 @@ int oracle_trace_rays(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, 
     for (int i = 0; i < n; ++i) {
         trace_hit hh;
         uint32_t v = 0;
-        int r = trace_ray(nodes, num_nodes, root, pos + 3 * i, dir + 3 * i, occlusion, &hh, &v, &bad);
+        int r = trace_ray(nodes, num_nodes, root, pos + 3 * i, dir + 3 * i, occlusion, 0.0f, &hh, &v, &bad);
         if (out_visits) out_visits[i] = v;
         if (r) {
             out_hit[i] = hh.node * 8u + (uint32_t)hh.child;
@@ -440,4 +469,125 @@ void oracle_pixel_dir(const float* basis, int x, int y, float* out) {
     out[0] = d.x;
     out[1] = d.y;
     out[2] = d.z;
+}
+
+/* ---- LOD colours: VoxNode::data as the subtree mean ------------------------ */
+/* reaction/report/main.tex:61: a node's data is the arithmetic mean of the
+ * colours of every voxel in its subtree. DEFINED HERE for RGBA8 leaves: per
+ * channel (sum + count/2) / count in integers, 0 for a subtree without leaves.
+ * out[id] for every node reachable from the root (others untouched). Returns
+ * -1 on an id out of range or a node reached twice, else 0. */
+int oracle_lod_colors(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, uint32_t* out,
+                      uint64_t* scratch /* 5 per node */, uint32_t* stack /* 2 per node */) {
+    if (is_null(root)) return 0;
+    if (root >= num_nodes) return -1;
+    for (uint32_t i = 0; i < num_nodes; ++i) scratch[5 * (size_t)i + 4] = UINT64_MAX; /* unvisited */
+    /* iterative post order: stack entries (id, next slot) */
+    size_t sp = 0;
+    stack[0] = root;
+    stack[1] = 0;
+    scratch[5 * (size_t)root + 4] = 0;
+    for (int k = 0; k < 4; ++k) scratch[5 * (size_t)root + k] = 0;
+    sp = 1;
+    while (sp > 0) {
+        uint32_t id = stack[2 * (sp - 1)];
+        uint32_t slot = stack[2 * (sp - 1) + 1];
+        const uint32_t* n = nodes + (size_t)10 * id;
+        uint64_t* acc = scratch + 5 * (size_t)id;
+        if (slot < 8) {
+            stack[2 * (sp - 1) + 1] = slot + 1;
+            uint32_t ch = n[2 + slot];
+            if (leaf_flag(n[0], (int)slot)) {
+                for (int k = 0; k < 4; ++k) acc[k] += (ch >> (8 * k)) & 0xffu;
+                acc[4] += 1;
+            } else if (!is_null(ch)) {
+                if (ch >= num_nodes) return -1;
+                if (scratch[5 * (size_t)ch + 4] != UINT64_MAX) return -1;
+                for (int k = 0; k < 5; ++k) scratch[5 * (size_t)ch + k] = 0;
+                stack[2 * sp] = ch;
+                stack[2 * sp + 1] = 0;
+                ++sp;
+            }
+            continue;
+        }
+        /* done with id: its mean, then add its sums into the parent */
+        uint32_t mean = 0;
+        if (acc[4] > 0)
+            for (int k = 0; k < 4; ++k) mean |= (uint32_t)((acc[k] + acc[4] / 2) / acc[4]) << (8 * k);
+        out[id] = mean;
+        --sp;
+        if (sp > 0) {
+            uint64_t* pa = scratch + 5 * (size_t)stack[2 * (sp - 1)];
+            for (int k = 0; k < 5; ++k) pa[k] += acc[k];
+        }
+    }
+    return 0;
+}
+
+/* ---- LOD trace of the GPU pipeline (Run_Trace, demo/SVORenderer.cpp:125) ---- */
+/* Primary rays only. Per pixel: t = max(t1) of the hit cell (-1 on a miss);
+ * z = t / |dir0 + du x + dv y|, the view-space depth (+inf on a miss); albedo =
+ * the leaf's RGBA8, or the node's LOD colour on an LOD stop (0 on a miss);
+ * level of the hit cell (0 on a miss); visits. Returns the number of rays or -1
+ * on a bad child id. */
+long long oracle_lod_render_rows(const uint32_t* nodes, uint32_t num_nodes, uint32_t root,
+                                 const uint32_t* lod_colors, const float* pos, const float* basis,
+                                 int w, int h, int y0, int y1, float detail, float* t, float* z,
+                                 uint32_t* albedo, uint8_t* level, uint32_t* visits) {
+    long long rays = 0;
+    int bad = 0;
+    if (y1 > h) y1 = h;
+    for (int y = y0; y < y1; ++y) {
+        for (int x = 0; x < w; ++x) {
+            size_t offs = (size_t)y * (size_t)w + (size_t)x;
+            float xf = (float)x, yf = (float)y;
+            f3 q = f3_make(basis[0] + basis[3] * xf, basis[1] + basis[4] * xf, basis[2] + basis[5] * xf);
+            q = f3_make(q.x + basis[6] * yf, q.y + basis[7] * yf, q.z + basis[8] * yf);
+            float qn = sqrtf(f3_dot(q, q));
+            float dir[3] = {adjust_coord(q.x / qn), adjust_coord(q.y / qn), adjust_coord(q.z / qn)};
+            trace_hit hh;
+            uint32_t v = 0;
+            int r = trace_ray(nodes, num_nodes, root, pos, dir, 0, detail, &hh, &v, &bad);
+            ++rays;
+            if (visits) visits[offs] = v;
+            if (r) {
+                const uint32_t* n = nodes + (size_t)10 * hh.node;
+                if (t) t[offs] = hh.t;
+                if (z) z[offs] = hh.t / qn;
+                if (albedo) albedo[offs] = hh.child < 0 ? lod_colors[hh.node] : n[2 + hh.child];
+                if (level) level[offs] = (uint8_t)hh.level;
+            } else {
+                if (t) t[offs] = -1.0f;
+                if (z) z[offs] = INFINITY;
+                if (albedo) albedo[offs] = 0;
+                if (level) level[offs] = 0;
+            }
+        }
+    }
+    return bad ? -1 : rays;
+}
+
+/* Single rays with explicit origins and directions and an LOD coefficient
+ * (for the pins): hit node id (O_MISS on a miss), child slot (-1 on an LOD
+ * stop), t and level. */
+int oracle_lod_trace_rays(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, int n,
+                          const float* pos, const float* dir, float detail, uint32_t* out_node,
+                          int32_t* out_child, float* out_t, int32_t* out_level) {
+    int bad = 0;
+    for (int i = 0; i < n; ++i) {
+        trace_hit hh;
+        uint32_t v = 0;
+        if (trace_ray(nodes, num_nodes, root, pos + 3 * i, dir + 3 * i, 0, detail, &hh, &v, &bad)) {
+            out_node[i] = hh.node;
+            out_child[i] = hh.child;
+            out_t[i] = hh.t;
+            out_level[i] = hh.level;
+        } else {
+            out_node[i] = O_MISS;
+            out_child[i] = -2;
+            out_t[i] = -1.0f;
+            out_level[i] = 0;
+        }
+    }
+    return bad ? -1 : 0;
 }

@@ -98,6 +98,31 @@ class TuneTrial(ctypes.Structure):
     _fields_ = [("tuning", Tuning), ("ms", ctypes.c_float)]
 
 
+YX_MAX_LIGHTS = 4
+
+
+class LightParams(ctypes.Structure):
+    """yx_light_params (LightParams, demo/Demo.cpp:141-146)."""
+    _fields_ = [
+        ("enabled", ctypes.c_int32),
+        ("pos", ctypes.c_float * 3),
+        ("diffuse", ctypes.c_float * 3),
+        ("specular", ctypes.c_float * 3),
+        ("attenuation", ctypes.c_float * 3),
+    ]
+
+
+class PipelineOutputs(ctypes.Structure):
+    _fields_ = [
+        ("rgba", ctypes.c_void_p),
+        ("z", ctypes.c_void_p),
+        ("t", ctypes.c_void_p),
+        ("albedo", ctypes.c_void_p),
+        ("level", ctypes.c_void_p),
+        ("visits", ctypes.c_void_p),
+    ]
+
+
 class BatchJob(ctypes.Structure):
     _fields_ = [
         ("scene", ctypes.c_int32),
@@ -190,6 +215,13 @@ SIGNATURES = {
     "yx_renderer_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "yx_renderer_autotune": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), ctypes.c_int32, ctypes.POINTER(TuneTrial),
                                             ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _P]),
+    "yx_renderer_set_detail_coef": (ctypes.c_int, [_P, ctypes.c_float]),
+    "yx_renderer_get_detail_coef": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
+    "yx_renderer_set_ssna": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "yx_renderer_get_ssna": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int32)]),
+    "yx_renderer_set_show_normals": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "yx_renderer_set_point_light": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.POINTER(LightParams)]),
+    "yx_render_pipeline": (ctypes.c_int, [_P, ctypes.POINTER(PipelineOutputs), _P]),
     "yx_batch_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
     "yx_batch_destroy": (None, [_P]),
     "yx_batch_set_scenes": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int32]),

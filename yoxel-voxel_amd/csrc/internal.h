@@ -70,12 +70,17 @@ struct GpuTree {
     // level_records[l]: records of levels 0..l (the root is level 0); the
     // breadth-first order puts them first, so they are a prefix of `recs`
     std::vector<uint32_t> level_records;
+    // with_lod: per record, the mean RGBA8 of the leaves under a node record
+    // (VoxNode::data as the subtree mean); a leaf record holds its own colour
+    std::vector<uint32_t> lod;
 };
-int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base = 0);
+int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base = 0, bool with_lod = false);
 
 // Pixel-ray basis of a pinhole camera: dir0, du, dv as 9 floats; the ray of
 // pixel (x, y) is normalize(dir0 + du x + dv y).
 void camera_basis(const float dir[3], const float up[3], float fov, int w, int h, float basis[9]);
+// Camera frame: right, down (= -up'), forward as 9 floats.
+void camera_frame(const float dir[3], const float up[3], float frame[9]);
 
 // 3D Morton code, x in bit 0 of each triple (child slot convention).
 inline uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {

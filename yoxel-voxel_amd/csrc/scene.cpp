@@ -485,7 +485,8 @@ static int build_synthetic(const yx_scene_desc& d, yx_scene& out) {
 // right = normalize(fwd x up), up' = right x fwd, da = tan(fov/2) / w,
 // du = 2 right da, dv = -2 up' da, dir0 = fwd - du w/2 - dv h/2. Integer
 // factors are converted to float first, as the cg point operators do.
-void camera_basis(const float dir[3], const float up[3], float fov, int w, int h, float basis[9]) {
+// forward, right and up' of InitRayDir (cell/renderer_base.h:50-55)
+static void camera_vectors(const float dir[3], const float up[3], float fwd[3], float right[3], float upv[3]) {
     auto norm = [](float* v) {
         float n2 = 0.0f;
         n2 += v[0] * v[0];
@@ -501,12 +502,16 @@ void camera_basis(const float dir[3], const float up[3], float fov, int w, int h
         o[1] = a[2] * b[0] - a[0] * b[2];
         o[2] = a[0] * b[1] - a[1] * b[0];
     };
-    float fwd[3] = {dir[0], dir[1], dir[2]};
+    for (int i = 0; i < 3; ++i) fwd[i] = dir[i];
     norm(fwd);
-    float right[3], upv[3];
     cross(fwd, up, right);
     norm(right);
     cross(right, fwd, upv);
+}
+
+void camera_basis(const float dir[3], const float up[3], float fov, int w, int h, float basis[9]) {
+    float fwd[3], right[3], upv[3];
+    camera_vectors(dir, up, fwd, right, upv);
     const float deg2rad = (float)(3.14159265358979323846 / (double)180.0f);
     const float da = tanf((fov / 2.0f) * deg2rad) / (float)w;
     const float wf = (float)w, hf = (float)h;
@@ -519,8 +524,19 @@ void camera_basis(const float dir[3], const float up[3], float fov, int w, int h
     }
 }
 
-int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base) {
+void camera_frame(const float dir[3], const float up[3], float frame[9]) {
+    float fwd[3], right[3], upv[3];
+    camera_vectors(dir, up, fwd, right, upv);
+    for (int i = 0; i < 3; ++i) {
+        frame[i] = right[i];
+        frame[3 + i] = -upv[i];
+        frame[6 + i] = fwd[i];
+    }
+}
+
+int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base, bool with_lod) {
     out.recs.clear();
+    out.lod.clear();
     out.depth = 0;
     out.empty = true;
     const uint32_t nn = (uint32_t)(s.nodes.size() / 10);
@@ -589,6 +605,51 @@ int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base) {
     out.node_levels = max_node_level + 1;
     out.empty = (depth == 0);
     out.level_records.push_back((uint32_t)(out.recs.size() / 2));  // the deepest level's records
+    if (with_lod) {
+        // reaction/report/main.tex:61: a node's data is the mean of the voxels
+        // of its subtree. Per channel (sum + count / 2) / count in integers
+        // (0 without leaves). The queue in reverse visits children before
+        // their parents.
+        struct Sums {
+            uint64_t c[4];
+            uint64_t n;
+        };
+        std::vector<Sums> sums(nn);
+        out.lod.assign(out.recs.size() / 2, 0u);
+        for (size_t h = q_id.size(); h-- > 0;) {
+            const uint32_t id = q_id[h];
+            const uint32_t* n = &s.nodes[(size_t)id * 10];
+            Sums acc{{0, 0, 0, 0}, 0};
+            for (int c = 0; c < 8; ++c) {
+                const uint32_t ch = n[2 + c];
+                if (n[0] & (1u << c)) {
+                    for (int k = 0; k < 4; ++k) acc.c[k] += (ch >> (8 * k)) & 0xffu;
+                    acc.n += 1;
+                } else if (!(ch & 0x80000000u)) {
+                    const Sums& cs = sums[ch];
+                    for (int k = 0; k < 4; ++k) acc.c[k] += cs.c[k];
+                    acc.n += cs.n;
+                }
+            }
+            sums[id] = acc;
+            uint32_t mean = 0;
+            if (acc.n > 0)
+                for (int k = 0; k < 4; ++k) mean |= (uint32_t)((acc.c[k] + acc.n / 2) / acc.n) << (8 * k);
+            out.lod[q_rec[h]] = mean;
+        }
+        // leaf records: their own colour
+        for (size_t h = 0; h < q_id.size(); ++h) {
+            const uint32_t rec = q_rec[h];
+            const uint32_t first = out.recs[2 * (size_t)rec] - base;
+            const uint32_t masks = out.recs[2 * (size_t)rec + 1];
+            const uint32_t lm = (masks >> 8) & 0xffu, all = (masks & 0xffu) | lm;
+            for (int c = 0, k = 0; c < 8; ++c) {
+                if (!(all & (1u << c))) continue;
+                if (lm & (1u << c)) out.lod[first + k] = out.recs[2 * (size_t)(first + k)];
+                ++k;
+            }
+        }
+    }
     return YX_OK;
 }
 

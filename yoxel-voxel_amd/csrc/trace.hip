@@ -18,6 +18,7 @@
 #include "trace_device.h"
 #include "trace_persist.h"
 #include "trace_stack.h"
+#include "ssna.h"
 
 namespace yx {
 
@@ -130,6 +131,17 @@ struct yx_renderer {
     // measured best on MI355X for the 1080p depth-11 frame (stack kernel, one
     // wave per workgroup, no register window, refill at 32 idle lanes)
     yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 32, 1, 0, 0, 0};
+    // shading pipeline (SVORenderer::Render): per-record mean leaf colours,
+    // settings with SVORenderer's defaults (demo/SVORenderer.cpp:6-16), and the
+    // z / colour work buffers (m_zbuf[2], demo/SVORenderer.h:59)
+    uint32_t* d_lod = nullptr;
+    float detail_coef = 1.0f;
+    bool ssna = true;
+    bool show_normals = false;
+    yx_light_params lights[YX_MAX_LIGHTS] = {};
+    float* d_z[2] = {nullptr, nullptr};
+    uint32_t* d_albedo = nullptr;
+    size_t work_pixels = 0;
 };
 
 #define YX_HIP(call)                                                                        \
@@ -322,6 +334,10 @@ extern "C" void yx_renderer_destroy(yx_renderer* r) {
     if (r->d_recs) (void)hipFree(r->d_recs);
     if (r->d_frame) (void)hipFree(r->d_frame);
     if (r->d_counters) (void)hipFree(r->d_counters);
+    if (r->d_lod) (void)hipFree(r->d_lod);
+    for (float* z : r->d_z)
+        if (z) (void)hipFree(z);
+    if (r->d_albedo) (void)hipFree(r->d_albedo);
     delete r;
 }
 
@@ -358,7 +374,7 @@ extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
     GpuTree t;
     int rc;
     try {
-        rc = build_gpu_tree(*s, t);
+        rc = build_gpu_tree(*s, t, 0, true);
     } catch (const std::bad_alloc&) {
         return fail(YX_ERR_OOM, "out of host memory in the re-layout");
     }
@@ -366,6 +382,10 @@ extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
     if (r->d_recs) {
         YX_HIP(hipFree(r->d_recs));
         r->d_recs = nullptr;
+    }
+    if (r->d_lod) {
+        YX_HIP(hipFree(r->d_lod));
+        r->d_lod = nullptr;
     }
     r->n_recs = 0;
     r->depth = 0;
@@ -378,6 +398,15 @@ extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
             return fail(YX_ERR_OOM, "hipMalloc(%zu) for the scene: %s", bytes, hipGetErrorString(e));
         }
         YX_HIP(hipMemcpy(r->d_recs, t.recs.data(), bytes, hipMemcpyHostToDevice));
+        const size_t lod_bytes = t.lod.size() * sizeof(uint32_t);
+        e = hipMalloc((void**)&r->d_lod, lod_bytes);
+        if (e != hipSuccess) {
+            r->d_lod = nullptr;
+            (void)hipFree(r->d_recs);
+            r->d_recs = nullptr;
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) for the LOD colours: %s", lod_bytes, hipGetErrorString(e));
+        }
+        YX_HIP(hipMemcpy(r->d_lod, t.lod.data(), lod_bytes, hipMemcpyHostToDevice));
         r->n_recs = t.recs.size() / 2;
         r->depth = t.depth;
         r->node_levels = t.node_levels;
@@ -498,7 +527,7 @@ extern "C" int yx_render_frame(yx_renderer* r, const uint32_t** colors) {
 extern "C" int yx_renderer_scene_bytes(const yx_renderer* r, uint64_t* bytes) {
     clear_error();
     if (!r || !bytes) return fail(YX_ERR_INVALID_ARG, "null argument");
-    *bytes = (uint64_t)r->n_recs * 8u;
+    *bytes = (uint64_t)r->n_recs * 12u;  // 8-byte records + 4-byte LOD colours
     return YX_OK;
 }
 
@@ -519,4 +548,233 @@ extern "C" int yx_renderer_autotune(yx_renderer* r, const yx_outputs* out, int32
     const int rc = tune_coordinate_descent(keep, time_ms, trials, cap, n_trials, &best, true);
     r->tune = rc == YX_OK ? best : keep;
     return rc;
+}
+
+// ---- shading pipeline (SVORenderer::Render, demo/SVORenderer.cpp:95-149) ---
+
+extern "C" int yx_renderer_set_detail_coef(yx_renderer* r, float coef) {
+    clear_error();
+    if (!r) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (!(coef >= 0.0f) || !std::isfinite(coef)) return fail(YX_ERR_INVALID_ARG, "detail coef must be finite and >= 0");
+    r->detail_coef = coef;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_get_detail_coef(const yx_renderer* r, float* coef) {
+    clear_error();
+    if (!r || !coef) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *coef = r->detail_coef;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_ssna(yx_renderer* r, int32_t enable) {
+    clear_error();
+    if (!r) return fail(YX_ERR_INVALID_ARG, "null argument");
+    r->ssna = enable != 0;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_get_ssna(const yx_renderer* r, int32_t* enable) {
+    clear_error();
+    if (!r || !enable) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *enable = r->ssna ? 1 : 0;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_show_normals(yx_renderer* r, int32_t enable) {
+    clear_error();
+    if (!r) return fail(YX_ERR_INVALID_ARG, "null argument");
+    r->show_normals = enable != 0;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_point_light(yx_renderer* r, int32_t i, const yx_light_params* light) {
+    clear_error();
+    if (!r || !light) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (i < 0 || i >= YX_MAX_LIGHTS) return fail(YX_ERR_INVALID_ARG, "light index %d not in 0..%d", i, YX_MAX_LIGHTS - 1);
+    for (int k = 0; k < 3; ++k)
+        if (!std::isfinite(light->pos[k]) || !std::isfinite(light->diffuse[k]) || !std::isfinite(light->specular[k]) ||
+            !std::isfinite(light->attenuation[k]))
+            return fail(YX_ERR_INVALID_ARG, "light %d: non-finite parameter", i);
+    // the attenuation 1 / (c0 + c1 d + c2 d^2) stays finite and positive
+    if (!(light->attenuation[0] > 0.0f) || light->attenuation[1] < 0.0f || light->attenuation[2] < 0.0f)
+        return fail(YX_ERR_INVALID_ARG, "light %d: attenuation needs c0 > 0, c1 >= 0, c2 >= 0", i);
+    r->lights[i] = *light;
+    r->lights[i].enabled = light->enabled != 0;
+    return YX_OK;
+}
+
+// Frame constants of the pipeline (demo/SVORenderer.cpp:104-105,131-139).
+struct PipelineParams {
+    float detail, pixel_ang, pix, vox, edge, zlimit[kBlurPasses];
+};
+
+static PipelineParams pipeline_params(const yx_renderer* r) {
+    PipelineParams p;
+    const float deg2rad = (float)(3.14159265358979323846 / (double)180.0f);
+    const float w = (float)r->w;
+    p.detail = (r->detail_coef * ((r->fov / 2.0f) * deg2rad)) / w;
+    p.pixel_ang = (r->fov * deg2rad) / w;
+    const float da = tanf((r->fov / 2.0f) * deg2rad) / w;
+    p.pix = 2.0f * da;
+    p.vox = ldexpf(1.0f, -r->depth);
+    p.edge = 4.0f * p.vox;
+    float blur = 3.0f;
+    for (int i = 0; i < kBlurPasses; ++i) {
+        p.zlimit[i] = (5.0f * p.vox) / (p.pixel_ang * blur);
+        blur += 3.0f;
+    }
+    return p;
+}
+
+// SVORenderer::InitBlur (demo/SVORenderer.cpp:55-79): exp(-(tx^2 + ty^2)) with
+// tx = 2 (x - h) / h over the 5x5 window, normalised by the sum.
+static void blur_kernel_table(float* kern) {
+    const float h = (float)(kBlurK / 2);
+    const float scale = 2.0f;
+    float sum = 0.0f;
+    for (int y = 0; y < kBlurK; ++y)
+        for (int x = 0; x < kBlurK; ++x) {
+            float tx = scale * ((float)x - h) / h;
+            float ty = scale * ((float)y - h) / h;
+            tx = tx * tx;
+            ty = ty * ty;
+            const float v = expf(-(tx + ty));
+            kern[y * kBlurK + x] = v;
+            sum += v;
+        }
+    for (int i = 0; i < kBlurK * kBlurK; ++i) kern[i] /= sum;
+}
+
+static int ensure_work_buffers(yx_renderer* r, size_t npix) {
+    if (npix <= r->work_pixels && r->d_albedo) return YX_OK;
+    for (float*& z : r->d_z)
+        if (z) {
+            YX_HIP(hipFree(z));
+            z = nullptr;
+        }
+    if (r->d_albedo) {
+        YX_HIP(hipFree(r->d_albedo));
+        r->d_albedo = nullptr;
+    }
+    r->work_pixels = 0;
+    hipError_t e = hipMalloc((void**)&r->d_z[0], npix * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&r->d_z[1], npix * sizeof(float));
+    if (e == hipSuccess) e = hipMalloc((void**)&r->d_albedo, npix * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        for (float*& z : r->d_z)
+            if (z) {
+                (void)hipFree(z);
+                z = nullptr;
+            }
+        if (r->d_albedo) (void)hipFree(r->d_albedo);
+        r->d_albedo = nullptr;
+        return fail(YX_ERR_OOM, "hipMalloc for the pipeline's %zu-pixel work buffers: %s", npix, hipGetErrorString(e));
+    }
+    r->work_pixels = npix;
+    return YX_OK;
+}
+
+extern "C" int yx_render_pipeline(yx_renderer* r, const yx_pipeline_outputs* out, void* stream) {
+    clear_error();
+    if (!r || !out) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (!r->has_scene) return fail(YX_ERR_NO_SCENE, "no scene set");
+    int rc = ensure_device(r);
+    if (rc != YX_OK) return rc;
+    const size_t npix = (size_t)r->w * (size_t)r->h;
+    rc = ensure_work_buffers(r, npix);
+    if (rc != YX_OK) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    const PipelineParams pp = pipeline_params(r);
+    float basis[9];
+    init_ray_dir(r, basis);
+
+    // 1. LOD trace (Run_Trace): persistent stack kernel, primary rays only
+    PersistArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.recs = r->d_recs;  // null for an empty scene: every ray misses
+    for (int i = 0; i < 3; ++i) a.pos[i] = r->pos[i];
+    std::memcpy(a.basis, basis, sizeof(basis));
+    a.w = r->w;
+    a.h = r->h;
+    a.out.t = out->t;
+    a.out.visits = out->visits;
+    a.counters = r->d_counters;
+    const yx_tuning& t = r->tune;
+    a.tile_wl = __builtin_ctz((unsigned)t.tile_w);
+    a.tile_hl = __builtin_ctz((unsigned)t.tile_h);
+    a.tiles_x = (r->w + t.tile_w - 1) / t.tile_w;
+    const int tiles_y = (r->h + t.tile_h - 1) / t.tile_h;
+    const uint32_t tile = (uint32_t)(t.tile_w * t.tile_h);
+    a.total = (uint32_t)a.tiles_x * (uint32_t)tiles_y * tile;
+    a.nq = t.queues;
+    const uint32_t tiles_per_q = ((uint32_t)a.tiles_x * (uint32_t)tiles_y + t.queues - 1) / t.queues;
+    a.per_q = tiles_per_q * tile;
+    a.refill_min = t.refill_min;
+    a.levels = std::max(1, r->node_levels - 1);
+    a.lod = r->d_lod;
+    a.detail = pp.detail;
+    a.z = r->d_z[0];
+    a.albedo = out->albedo ? out->albedo : r->d_albedo;
+    a.hit_level = out->level;
+    int block = std::min(256, std::max(64, t.block));
+    const size_t lane_bytes = stack_lane_bytes(a.levels, 0);
+    size_t lds = lane_bytes * (size_t)block;
+    while (lds > r->max_lds && block > 64) {
+        block /= 2;
+        lds = lane_bytes * (size_t)block;
+    }
+    if (lds > r->max_lds)
+        return fail(YX_ERR_RANGE, "LOD trace stack needs %zu bytes of LDS per workgroup, device allows %zu", lds,
+                    r->max_lds);
+    if (block == 64)
+        rc = launch_persistent(r, (const void*)stack_kernel<64, 0, false, false, true>,
+                               stack_kernel<64, 0, false, false, true>, 64, a, lds, st);
+    else if (block == 128)
+        rc = launch_persistent(r, (const void*)stack_kernel<128, 0, false, false, true>,
+                               stack_kernel<128, 0, false, false, true>, 128, a, lds, st);
+    else
+        rc = launch_persistent(r, (const void*)stack_kernel<256, 0, false, false, true>,
+                               stack_kernel<256, 0, false, false, true>, 256, a, lds, st);
+    if (rc != YX_OK) return rc;
+
+    // 2. depth-limited blur passes (Run_BlurZ), ping-pong between the z buffers
+    int cur = 0;
+    if (r->ssna) {
+        BlurArgs b;
+        b.w = r->w;
+        b.h = r->h;
+        b.edge = pp.edge;
+        blur_kernel_table(b.kern);
+        const dim3 grid((unsigned)((r->w + kBlurTX - 1) / kBlurTX), (unsigned)((r->h + kBlurTY - 1) / kBlurTY));
+        for (int i = 0; i < kBlurPasses; ++i) {
+            b.zin = r->d_z[cur];
+            b.zout = r->d_z[1 - cur];
+            b.zlimit = pp.zlimit[i];
+            hipLaunchKernelGGL(blur_z_kernel, grid, dim3(kBlurTX * kBlurTY), 0, st, b);
+            YX_HIP(hipGetLastError());
+            cur = 1 - cur;
+        }
+    }
+
+    // 3. Phong shading with normals from z (Run_ShadeSimple)
+    ShadeArgs sa;
+    std::memset(&sa, 0, sizeof(sa));
+    sa.z = r->d_z[cur];
+    sa.albedo = a.albedo;
+    sa.rgba = out->rgba;
+    sa.zout = out->z;
+    sa.w = r->w;
+    sa.h = r->h;
+    std::memcpy(sa.basis, basis, sizeof(basis));
+    for (int i = 0; i < 3; ++i) sa.eye[i] = r->pos[i];
+    camera_frame(r->dir, r->up, sa.frame);
+    sa.pix = pp.pix;
+    sa.ambient = 0.1f;  // rp.ambient, demo/SVORenderer.cpp:113
+    sa.show_normals = r->show_normals ? 1 : 0;
+    for (int i = 0; i < YX_MAX_LIGHTS; ++i) sa.lights[i] = r->lights[i];
+    const dim3 sgrid((unsigned)((r->w + kShadeTX - 1) / kShadeTX), (unsigned)((r->h + kShadeTY - 1) / kShadeTY));
+    hipLaunchKernelGGL(shade_simple_kernel, sgrid, dim3(kShadeTX * kShadeTY), 0, st, sa);
+    YX_HIP(hipGetLastError());
+    return YX_OK;
 }

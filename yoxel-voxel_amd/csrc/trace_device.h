@@ -35,8 +35,9 @@ __device__ __forceinline__ float adjust_coord(float x) {
     return x;
 }
 
-// normalized(dir0 + du*x + dv*y), cell/ppu_renderer.cpp:56
-__device__ __forceinline__ Ray3 pixel_dir(const float* b, int x, int y) {
+// normalized(dir0 + du*x + dv*y), cell/ppu_renderer.cpp:56; `len` receives
+// |dir0 + du*x + dv*y| (the ray's view-depth scale: z = t / len)
+__device__ __forceinline__ Ray3 pixel_dir_n(const float* b, int x, int y, float& len) {
     float xf = (float)x, yf = (float)y;
     float dx = b[0] + b[3] * xf, dy = b[1] + b[4] * xf, dz = b[2] + b[5] * xf;
     dx = dx + b[6] * yf;
@@ -47,11 +48,17 @@ __device__ __forceinline__ Ray3 pixel_dir(const float* b, int x, int y) {
     n2 += dy * dy;
     n2 += dz * dz;
     float n = __builtin_sqrtf(n2);
+    len = n;
     Ray3 r;
     r.x = dx / n;
     r.y = dy / n;
     r.z = dz / n;
     return r;
+}
+
+__device__ __forceinline__ Ray3 pixel_dir(const float* b, int x, int y) {
+    float len;
+    return pixel_dir_n(b, x, y, len);
 }
 
 // Exact per-ray traversal state of one level.

@@ -70,6 +70,12 @@ struct PersistArgs {
     const BatchJob* jobs;
     const uint32_t* job_tiles;  // first tile of each job, n_jobs + 1 entries (last = all tiles)
     int n_jobs;
+    // LOD trace of the shading pipeline (stack kernel, LOD = true)
+    const uint32_t* lod;  // mean leaf colour per node record
+    float detail;         // stop when 2^-level < entry t * detail
+    float* z;             // view depth t / |dir0 + du x + dv y|, +inf on a miss
+    uint32_t* albedo;     // leaf or LOD colour, 0 on a miss
+    uint8_t* hit_level;   // level of the hit cell, 0 on a miss
 };
 
 struct WinEntry {

@@ -91,6 +91,14 @@ __device__ __forceinline__ void write_miss(const yx_outputs& o, uint32_t pix, ui
     if (o.visits) o.visits[pix] = visits;
 }
 
+__device__ __forceinline__ void write_miss_lod(const PersistArgs& a, uint32_t pix, uint32_t visits) {
+    if (a.out.t) a.out.t[pix] = -1.0f;
+    if (a.z) a.z[pix] = __builtin_inff();
+    if (a.albedo) a.albedo[pix] = 0u;
+    if (a.hit_level) a.hit_level[pix] = 0;
+    if (a.out.visits) a.out.visits[pix] = visits;
+}
+
 __device__ __forceinline__ void write_final(const yx_outputs& o, uint32_t pix, uint32_t color, float k,
                                             uint8_t shadow, uint32_t visits) {
     if (o.rgba) o.rgba[pix] = shade_rgba(color, k);
@@ -124,7 +132,11 @@ enum : int {
 // BATCH: rays come from the (scene, camera) jobs of a.jobs (camera, light,
 // root record and output offset per job) instead of the single view in `a`.
 // STAGE: node records [0, a.lds_recs) are read from an LDS copy (single scene).
-template <int BLOCK, int W, bool BATCH, bool STAGE = false>
+// LOD: the trace of the shading pipeline - primary rays only, a node below the
+// root whose cell edge 2^-level is smaller than its entry distance times
+// a.detail (and which has a non-null child) ends the ray with its mean colour
+// a.lod[record]; outputs t, z, albedo, hit level and visits.
+template <int BLOCK, int W, bool BATCH, bool STAGE = false, bool LOD = false>
 __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     // dynamic LDS: the staged top-level records (single scene), then the
     // traversal stack rows
@@ -171,6 +183,9 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     uint32_t visits = 0, color = 0;
     float lam = 0.0f;
     WinEntry win[W > 0 ? W : 1];
+    float qlen = 1.0f;      // LOD: |dir0 + du x + dv y| of the lane's pixel
+    bool lod_stop = false;  // LOD: the ray ended on a node
+    uint32_t lod_rec = 0;   // LOD: that node's record
 
     // W == 0: every level's node record ([levels][BLOCK] uint2), and the cell
     // of even levels only ([(levels + 1) / 2][BLOCK] as uint4 + uint2). A pop to
@@ -228,7 +243,29 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
         const uint64_t wait_mask = __ballot(!tracing && status != sRetired);
         if (wait_mask != 0 && (tr_mask == 0 || __popcll(wait_mask) >= a.refill_min)) {
             // ---- batched once-per-ray work ----------------------------------
-            if (status == sHitP) {
+            if (LOD && status == sHitP) {
+                const float t = max3(c.t1x, c.t1y, c.t1z);
+                uint32_t col;
+                int lv = level;
+                if (lod_stop) {
+                    col = a.lod[lod_rec];
+                } else {
+                    const int real = ch ^ (int)flags;
+                    const uint32_t cm = node.y & 0xffu, lm = (node.y >> 8) & 0xffu;
+                    const uint32_t below = (cm | lm) & ((1u << real) - 1u);
+                    col = rec_at(node.x + (uint32_t)__builtin_popcount(below)).x;
+                    lv = level + 1;
+                }
+                if (a.out.t) a.out.t[pix] = t;
+                if (a.z) a.z[pix] = t / qlen;
+                if (a.albedo) a.albedo[pix] = col;
+                if (a.hit_level) a.hit_level[pix] = (uint8_t)lv;
+                if (a.out.visits) a.out.visits[pix] = visits;
+                status = sIdle;
+            } else if (LOD && status == sMissP) {
+                write_miss_lod(a, pix, visits);
+                status = sIdle;
+            } else if (status == sHitP) {
                 const int real = ch ^ (int)flags;
                 const uint32_t cm = node.y & 0xffu, lm = (node.y >> 8) & 0xffu;
                 const uint32_t below = (cm | lm) & ((1u << real) - 1u);
@@ -309,7 +346,13 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                         } else {
                             pix = (uint32_t)y * (uint32_t)a.w + (uint32_t)x;
                         }
-                        const Ray3 d = pixel_dir(basis, x, y);
+                        Ray3 d;
+                        if constexpr (LOD) {
+                            d = pixel_dir_n(basis, x, y, qlen);
+                            lod_stop = false;
+                        } else {
+                            d = pixel_dir(basis, x, y);
+                        }
                         dx = adjust_coord(d.x);
                         dy = adjust_coord(d.y);
                         dz = adjust_coord(d.z);
@@ -340,7 +383,10 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     if constexpr (W == 0) ffcpath = set3(ffcpath, 0, (uint32_t)ch);
                     status = status == sStartP ? sTraceP : sTraceS;
                 } else if (status == sStartP) {
-                    write_miss(a.out, pix, visits);
+                    if constexpr (LOD)
+                        write_miss_lod(a, pix, visits);
+                    else
+                        write_miss(a.out, pix, visits);
                     status = sIdle;
                 } else {
                     write_final(a.out, pix, color, 0.1f + 0.9f * lam, 0, visits);  // lit
@@ -380,8 +426,19 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                 }
                 chpath = set3(chpath, level, (uint32_t)ch);
                 const uint32_t below = (cm | lm) & ((1u << real) - 1u);
-                node = rec_at(node.x + (uint32_t)__builtin_popcount(below));
+                const uint32_t crec = node.x + (uint32_t)__builtin_popcount(below);
+                node = rec_at(crec);
                 ++level;
+                if constexpr (LOD) {
+                    // c is this node's cell; 2^-level as a float is exact
+                    const float edge = __builtin_bit_cast(float, (uint32_t)(127 - level) << 23);
+                    if (node.y != 0u && edge < max3(c.t1x, c.t1y, c.t1z) * a.detail) {
+                        lod_stop = true;
+                        lod_rec = crec;
+                        status = sHitP;
+                        continue;
+                    }
+                }
                 ch = find_first_child(c);
                 if constexpr (W == 0) ffcpath = set3(ffcpath, level, (uint32_t)ch);
                 continue;

@@ -202,6 +202,55 @@ class SVORenderer:
         self.Render(stream=stream, **out)
         return out
 
+    # ---- the GPU pipeline of SVORenderer::Render (demo/SVORenderer.cpp:95-149)
+
+    def SetDetailCoef(self, coef):
+        check(lib().yx_renderer_set_detail_coef(self._h, float(coef)))
+
+    def GetDetailCoef(self):
+        c = ctypes.c_float()
+        check(lib().yx_renderer_get_detail_coef(self._h, ctypes.byref(c)))
+        return c.value
+
+    def SetSSNA(self, enable):
+        check(lib().yx_renderer_set_ssna(self._h, 1 if enable else 0))
+
+    def GetSSNA(self):
+        e = ctypes.c_int32()
+        check(lib().yx_renderer_get_ssna(self._h, ctypes.byref(e)))
+        return bool(e.value)
+
+    def SetShowNormals(self, enable):
+        check(lib().yx_renderer_set_show_normals(self._h, 1 if enable else 0))
+
+    def SetLigth(self, i, pos, diffuse=(0.7, 0.7, 0.7), specular=(0.3, 0.3, 0.3), attenuation=(1.0, 0.0, 0.5),
+                 enabled=True):
+        """SVORenderer::SetLigth(i, LightParams) (demo/SVORenderer.h:34), the
+        reference's spelling; defaults are the demo's eye light
+        (demo/Demo.cpp:141-146)."""
+        lp = _lib.LightParams(1 if enabled else 0, (ctypes.c_float * 3)(*pos), (ctypes.c_float * 3)(*diffuse),
+                              (ctypes.c_float * 3)(*specular), (ctypes.c_float * 3)(*attenuation))
+        check(lib().yx_renderer_set_point_light(self._h, int(i), ctypes.byref(lp)))
+
+    SetPointLight = SetLigth
+
+    def RenderPipeline(self, rgba=None, z=None, t=None, albedo=None, level=None, visits=None, stream=None):
+        """SVORenderer::Render(void*): LOD trace, SSNA depth blur, Phong shade,
+        into caller-owned device buffers, asynchronously on `stream`."""
+        o = _lib.PipelineOutputs(_addr(rgba), _addr(z), _addr(t), _addr(albedo), _addr(level), _addr(visits))
+        check(lib().yx_render_pipeline(self._h, ctypes.byref(o), ctypes.c_void_p(_stream(stream))))
+
+    def pipeline_tensors(self, outputs=("rgba", "z", "t", "albedo", "level"), stream=None):
+        """Allocate torch output tensors, run the pipeline, return them."""
+        import torch
+        w, h = self.GetResolution()
+        dev = torch.device("cuda", torch.cuda.current_device())
+        spec = {"rgba": torch.int32, "z": torch.float32, "t": torch.float32, "albedo": torch.int32,
+                "level": torch.uint8, "visits": torch.int32}
+        out = {k: torch.empty((h, w), dtype=spec[k], device=dev) for k in outputs}
+        self.RenderPipeline(stream=stream, **out)
+        return out
+
     def scene_bytes(self):
         b = ctypes.c_uint64()
         check(lib().yx_renderer_scene_bytes(self._h, ctypes.byref(b)))
