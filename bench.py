@@ -40,7 +40,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-launch measurement")
     ap.add_argument("--no-brick", action="store_true", help="skip the brick n-tree measurement")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain_512 measurement")
-    ap.add_argument("--only", choices=["composite", "batch", "brick", "plain"], default=None,
+    ap.add_argument("--no-pipeline", action="store_true",
+                    help="skip the shading-pipeline measurement (LOD trace, SSNA blur, Phong shade)")
+    ap.add_argument("--only", choices=["composite", "batch", "brick", "plain", "pipeline"], default=None,
                     help="run one secondary workload alone and print its object (profiling)")
     ap.add_argument("--brick-frames", type=int, default=8)
     ap.add_argument("--autotune", action="store_true",
@@ -75,6 +77,9 @@ def main():
             leg = plain_bench(yx, torch, not args.no_cpu_baseline)
         elif args.only == "batch":
             leg = batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
+        elif args.only == "pipeline":
+            leg = pipeline_bench(yx, torch, yx.SVOData.build_synthetic(**wl.LARGE_1080P), wl.orbit_cameras(32, seed=0),
+                                 not args.no_cpu_baseline)
         else:
             leg = brick_bench(yx, torch, args.brick_frames, not args.no_cpu_baseline)
         print(json.dumps(leg), flush=True)
@@ -173,6 +178,8 @@ def main():
     plain = None if args.no_plain else plain_bench(yx, torch, not args.no_cpu_baseline)
     batched = None if args.no_batch else batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
     brick = None if args.no_brick else brick_bench(yx, torch, args.brick_frames, not args.no_cpu_baseline)
+    pipeline = (None if args.no_pipeline or args.workload != "large_1080p"
+                else pipeline_bench(yx, torch, scene, cams, not args.no_cpu_baseline))
 
     line = {
         "metric": BASELINE_METRIC,
@@ -203,6 +210,7 @@ def main():
         "fwd_bwd": fwd_bwd,
         "batched": batched,
         "brick": brick,
+        "pipeline": pipeline,
     }
     print(json.dumps(line), flush=True)
 
@@ -323,6 +331,113 @@ def plain_bench(yx, torch, with_cpu, reps=8):
         top = max(by, key=lambda k: int(k))
         res["cpu_baseline"] = {"value": by[top], "unit": "Mrays/s", "cores": int(top), "kind": "port",
                                "sample": "all 8 views, 512x512", "by_threads": by}
+    return res
+
+
+def pipeline_bench(yx, torch, scene, cams, with_cpu, reps=2, cpu_frames=4):
+    """SVORenderer::Render's GPU pipeline (demo/SVORenderer.cpp:95-149) on
+    large_1080p: the 32 orbit frames at 1920x1080, fov 60, detail coef 1 and
+    SSNA on (SVORenderer's defaults), one point light at the eye with the
+    demo's parameters (demo/Demo.cpp:141-146). A frame is the LOD trace, five
+    blur passes and the shade; value = primary rays / mean frame time (HIP
+    events on the stream). The blur's share is the difference to the same
+    frames with SSNA off. CPU baseline: the restatement on `cpu_frames` frames."""
+    w, h = 1920, 1080
+    depth = scene.info()["depth"]
+    r = yx.SVORenderer()
+    r.SetScene(scene)
+    r.SetFOV(60.0)
+    r.SetResolution(w, h)
+    r.SetDetailCoef(1.0)
+    out = r.pipeline_tensors(("rgba", "z"))
+    extra = {"visits": torch.empty((h, w), dtype=torch.int32, device=out["rgba"].device),
+             "level": torch.empty((h, w), dtype=torch.uint8, device=out["rgba"].device)}
+    stream = torch.cuda.current_stream()
+
+    def frame(k, **o):
+        pos, d = cams[k % len(cams)]
+        r.SetViewPos(pos)
+        r.SetViewDir(d)
+        r.SetLigth(0, tuple(float(v) for v in pos))
+        r.RenderPipeline(stream=stream, **o)
+
+    # untimed counting pass: visits, hits and LOD stops of every frame
+    visits = hits = stops = 0
+    for k in range(len(cams)):
+        frame(k, **out, **extra)
+        torch.cuda.synchronize()
+        lv = extra["level"]
+        visits += int(extra["visits"].to(torch.int64).sum().item())
+        hits += int((lv > 0).sum().item())
+        stops += int(((lv > 0) & (lv < depth)).sum().item())
+    nf = len(cams)
+    res = {"unit": "Mrays/s", "workload": "large_1080p pipeline", "resolution": [w, h], "frames": nf,
+           "detail_coef": 1.0, "lod_stop_fraction_of_hits": round(stops / max(1, hits), 5)}
+    for ssna in (True, False):
+        r.SetSSNA(ssna)
+        for k in range(2):
+            frame(k, **out)
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for k in range(reps * nf):
+            frame(k, **out)
+        b.record(stream)
+        torch.cuda.synchronize()
+        res["ms_per_frame" if ssna else "ms_per_frame_no_ssna"] = round(a.elapsed_time(b) / (reps * nf), 4)
+    r.SetSSNA(True)
+    # a coarser level of detail (more LOD stops): coef 4
+    r.SetDetailCoef(4.0)
+    stops4 = hits4 = 0
+    for k in range(nf):
+        frame(k, **out, **extra)
+        torch.cuda.synchronize()
+        lv = extra["level"]
+        hits4 += int((lv > 0).sum().item())
+        stops4 += int(((lv > 0) & (lv < depth)).sum().item())
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for k in range(reps * nf):
+        frame(k, **out)
+    b.record(stream)
+    torch.cuda.synchronize()
+    res["coef4"] = {"ms_per_frame": round(a.elapsed_time(b) / (reps * nf), 4),
+                    "lod_stop_fraction_of_hits": round(stops4 / max(1, hits4), 5)}
+    r.SetDetailCoef(1.0)
+    ms = res["ms_per_frame"]
+    res["value"] = round(w * h / (ms / 1e3) / 1e6, 3)
+    blur_ms = max(1e-6, ms - res["ms_per_frame_no_ssna"])
+    res["blur_ms_5_passes"] = round(blur_ms, 4)
+    # per blur pass: z read + z written per pixel
+    pass_bytes = 8 * w * h
+    res["blur_roofline"] = {"bound": "hbm", "achieved": round(pass_bytes / (blur_ms / 5 / 1e3) / 1e9, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(pass_bytes / (blur_ms / 5 / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                            "bytes_per_pass": pass_bytes}
+    # algorithmic bytes of a frame: 8 B per node record read, 4 B per hit
+    # colour, trace z + colour writes 8 B, five passes 8 B, shade reads z and
+    # colour and writes RGBA8 and z 16 B per pixel
+    fb = (8 * visits + 4 * hits) / nf + (8 + 5 * 8 + 16) * w * h
+    res["algorithmic_bytes_per_frame"] = int(fb)
+    res["achieved_gbs"] = round(fb / (ms / 1e3) / 1e9, 1)
+    res["cpu_baseline"] = None
+    if with_cpu:
+        po = entry.load_oracle().PipelineOracle()
+        nodes, root = scene.nodes(), scene.GetRoot()
+        lod = po.lod_colors(nodes, root)
+        mk = entry.load_oracle().make_lights
+        th = min(16, os.cpu_count() or 1)
+        sel = list(range(0, nf, nf // cpu_frames))[:cpu_frames]
+        t0 = time.perf_counter()
+        for k in sel:
+            pos, d = cams[k]
+            lt = mk([dict(pos=pos, diffuse=(0.7,) * 3, specular=(0.3,) * 3, attenuation=(1.0, 0.0, 0.5))])
+            po.pipeline(nodes, root, depth, pos, d, fov=60.0, w=w, h=h, detail_coef=1.0, lights=lt, lod=lod,
+                        threads=th)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(len(sel) * w * h / dt / 1e6, 4), "unit": "Mrays/s", "cores": th,
+                               "kind": "port", "sample": f"orbit frames {sel}, full 1920x1080 pipeline "
+                                                         f"(trace, 5 blurs, shade), {dt:.1f} s"}
     return res
 
 

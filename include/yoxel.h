@@ -138,7 +138,9 @@ int yx_renderer_set_light(yx_renderer* r, const float dir[3], int32_t mode);
 int yx_renderer_ray_basis(const yx_renderer* r, float basis[9]);
 
 /* Render into caller-owned device buffers on `stream` (asynchronous). The GPU
- * variant of RenderFrame; demo/SVORenderer.cpp:159-213 Render(void*). */
+ * variant of RenderFrame with the device-buffer shape of SVORenderer::Render
+ * (demo/SVORenderer.cpp:95-149); the reference's full GPU pipeline is
+ * yx_render_pipeline below. */
 int yx_render(yx_renderer* r, const yx_outputs* out, void* stream);
 /* RenderFrame (cell/ppu_renderer.cpp:76-85): synchronous; returns a pointer to
  * a renderer-owned host RGBA8 buffer valid until the next call or resolution

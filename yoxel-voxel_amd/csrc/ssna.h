@@ -23,7 +23,9 @@ namespace yx {
 constexpr int kBlurK = 5;  // window side (BlurZKernSize)
 constexpr int kBlurR = kBlurK / 2;
 constexpr int kBlurPasses = 5;
-constexpr int kBlurTX = 64, kBlurTY = 4;  // one wave per tile row, 256 threads
+constexpr int kBlurTX = 64, kBlurTY = 16;  // 256 threads, 4 output rows each
+constexpr int kBlurRows = 4;
+constexpr int kBlurThreads = kBlurTX * kBlurTY / kBlurRows;
 constexpr int kBlurLX = kBlurTX + 2 * kBlurR, kBlurLY = kBlurTY + 2 * kBlurR;
 
 struct BlurArgs {
@@ -35,12 +37,16 @@ struct BlurArgs {
     float kern[kBlurK * kBlurK];
 };
 
-__global__ __launch_bounds__(kBlurTX* kBlurTY) void blur_z_kernel(BlurArgs a) {
+// A workgroup stages its 64x16 tile with a 2-pixel apron (68x20) in LDS; a
+// thread keeps the 8x5 window column it shares between its 4 vertically
+// adjacent outputs in registers. Each output sums its own 5x5 window in row
+// order, as the definition does.
+__global__ __launch_bounds__(kBlurThreads) void blur_z_kernel(BlurArgs a) {
     __shared__ float tile[kBlurLY * kBlurLX];
     const int x0 = (int)blockIdx.x * kBlurTX, y0 = (int)blockIdx.y * kBlurTY;
     // apron pixels outside the image read as +inf: they fail the depth test,
     // exactly as if they were left out of the window
-    for (int i = (int)threadIdx.x; i < kBlurLY * kBlurLX; i += kBlurTX * kBlurTY) {
+    for (int i = (int)threadIdx.x; i < kBlurLY * kBlurLX; i += kBlurThreads) {
         const int ly = i / kBlurLX, lx = i - ly * kBlurLX;
         const int gx = x0 + lx - kBlurR, gy = y0 + ly - kBlurR;
         float v = __builtin_inff();
@@ -48,27 +54,37 @@ __global__ __launch_bounds__(kBlurTX* kBlurTY) void blur_z_kernel(BlurArgs a) {
         tile[i] = v;
     }
     __syncthreads();
-    const int tx = (int)threadIdx.x % kBlurTX, ty = (int)threadIdx.x / kBlurTX;
-    const int x = x0 + tx, y = y0 + ty;
-    if (x >= a.w || y >= a.h) return;
-    const float zp = tile[(ty + kBlurR) * kBlurLX + tx + kBlurR];
-    float out = zp;
-    if (zp < a.zlimit) {
-        float num = 0.0f, den = 0.0f;
+    const int tx = (int)threadIdx.x % kBlurTX, ty = ((int)threadIdx.x / kBlurTX) * kBlurRows;
+    const int x = x0 + tx;
+    if (x >= a.w) return;
+    float win[kBlurRows + kBlurK - 1][kBlurK];
 #pragma unroll
-        for (int dy = 0; dy < kBlurK; ++dy)
+    for (int r = 0; r < kBlurRows + kBlurK - 1; ++r)
 #pragma unroll
-            for (int dx = 0; dx < kBlurK; ++dx) {
-                const float zq = tile[(ty + dy) * kBlurLX + tx + dx];
-                if (__builtin_fabsf(zq - zp) < a.edge) {
-                    const float wq = a.kern[dy * kBlurK + dx];
-                    num += wq * zq;
-                    den += wq;
+        for (int c = 0; c < kBlurK; ++c) win[r][c] = tile[(ty + r) * kBlurLX + tx + c];
+#pragma unroll
+    for (int j = 0; j < kBlurRows; ++j) {
+        const int y = y0 + ty + j;
+        if (y >= a.h) break;
+        const float zp = win[j + kBlurR][kBlurR];
+        float out = zp;
+        if (zp < a.zlimit) {
+            float num = 0.0f, den = 0.0f;
+#pragma unroll
+            for (int dy = 0; dy < kBlurK; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < kBlurK; ++dx) {
+                    const float zq = win[j + dy][dx];
+                    if (__builtin_fabsf(zq - zp) < a.edge) {
+                        const float wq = a.kern[dy * kBlurK + dx];
+                        num += wq * zq;
+                        den += wq;
+                    }
                 }
-            }
-        out = num / den;
+            out = num / den;
+        }
+        a.zout[(size_t)y * (size_t)a.w + (size_t)x] = out;
     }
-    a.zout[(size_t)y * (size_t)a.w + (size_t)x] = out;
 }
 
 constexpr int kShadeTX = 64, kShadeTY = 4;

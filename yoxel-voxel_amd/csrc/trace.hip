@@ -751,7 +751,7 @@ extern "C" int yx_render_pipeline(yx_renderer* r, const yx_pipeline_outputs* out
             b.zin = r->d_z[cur];
             b.zout = r->d_z[1 - cur];
             b.zlimit = pp.zlimit[i];
-            hipLaunchKernelGGL(blur_z_kernel, grid, dim3(kBlurTX * kBlurTY), 0, st, b);
+            hipLaunchKernelGGL(blur_z_kernel, grid, dim3(kBlurThreads), 0, st, b);
             YX_HIP(hipGetLastError());
             cur = 1 - cur;
         }
