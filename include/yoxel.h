@@ -93,6 +93,15 @@ int yx_scene_build_synthetic(const yx_scene_desc* desc, yx_scene** out);
  * ore/src/main.cpp:37-129); no GPU counterpart in the reference. */
 int yx_scene_build_synthetic_gpu(const yx_scene_desc* desc, yx_scene** out, void* stream);
 int yx_scene_get_info(const yx_scene* scene, yx_scene_info* info);
+/* Edits of the node array in place (the node pool of DynamicSVO / HomoStorage,
+ * reaction/report/main.tex:67-71): writes `count` VoxNode records (10 u32 each)
+ * from index `first` (at most the current size; the array grows as needed) and
+ * stamps their 256-node pages with a new version. The tree is validated when it
+ * is next laid out (yx_renderer_set_scene / yx_renderer_update_scene /
+ * yx_scene_get_info), so a sequence of writes may pass through invalid states.
+ * Invalidates pointers from yx_scene_nodes. */
+int yx_scene_write_nodes(yx_scene* scene, uint32_t first, const uint32_t* nodes, uint32_t count);
+int yx_scene_set_root(yx_scene* scene, uint32_t root);
 /* SVOData::GetRoot / operator[]: borrowed view of the node array (valid until
  * yx_scene_free). */
 int yx_scene_nodes(const yx_scene* scene, const uint32_t** nodes, uint32_t* num_nodes,
@@ -147,6 +156,25 @@ int yx_render(yx_renderer* r, const yx_outputs* out, void* stream);
  * change. Fails with YX_ERR_NO_SCENE when no scene is set (the reference
  * returns NULL). */
 int yx_render_frame(yx_renderer* r, const uint32_t** colors);
+
+/* Incremental scene sync (CudaSVO::Update, demo/SVORenderer.cpp:39-53,
+ * qtview.py:69-73): the renderer keeps a device copy of the scene's VoxNode
+ * array and uploads only the 256-node pages edited since its last sync (all of
+ * them for another scene object), then rebuilds its traversal records and LOD
+ * colours on the GPU. Results are identical to yx_renderer_set_scene. On a
+ * format error the previous records stay in place. */
+int yx_renderer_update_scene(yx_renderer* r, const yx_scene* scene, void* stream);
+/* DynamicSVO::CountChangedPages / CountTransfrerSize (ore/src/main.cpp:127-128):
+ * pages and bytes the next yx_renderer_update_scene would upload. */
+int yx_renderer_count_changed_pages(const yx_renderer* r, const yx_scene* scene, uint32_t* pages,
+                                    uint64_t* bytes);
+/* Pages and bytes uploaded by the last yx_renderer_update_scene. */
+int yx_renderer_last_update(const yx_renderer* r, uint32_t* pages, uint64_t* bytes);
+/* Copies the device traversal layout to host memory: 2 u32 per record (node:
+ * first child record, child mask | leaf mask << 8; leaf: RGBA8, parent id) and
+ * the LOD colour per record. Either array may be NULL; *n_recs is always set.
+ * No reference counterpart (inspection and tests of the re-layout). */
+int yx_renderer_layout(const yx_renderer* r, uint32_t* recs, uint32_t* lod, uint64_t cap, uint64_t* n_recs);
 
 /* Device memory used by the uploaded scene, in bytes. */
 int yx_renderer_scene_bytes(const yx_renderer* r, uint64_t* bytes);

@@ -215,6 +215,13 @@ SIGNATURES = {
     "yx_renderer_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "yx_renderer_autotune": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), ctypes.c_int32, ctypes.POINTER(TuneTrial),
                                             ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _P]),
+    "yx_scene_write_nodes": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
+    "yx_scene_set_root": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "yx_renderer_update_scene": (ctypes.c_int, [_P, _P, _P]),
+    "yx_renderer_count_changed_pages": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_uint32),
+                                                       ctypes.POINTER(ctypes.c_uint64)]),
+    "yx_renderer_last_update": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint64)]),
+    "yx_renderer_layout": (ctypes.c_int, [_P, _P, _P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
     "yx_renderer_set_detail_coef": (ctypes.c_int, [_P, ctypes.c_float]),
     "yx_renderer_get_detail_coef": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float)]),
     "yx_renderer_set_ssna": (ctypes.c_int, [_P, ctypes.c_int32]),

@@ -10,11 +10,23 @@
 
 // Host copy of an octree in the reference's VoxNode layout
 // (reaction/report/main.tex:36-56): 10 u32 per node, canonical ids are indices.
+// Edits (yx_scene_write_nodes / yx_scene_set_root) bump `version` and stamp
+// the 256-node pages they touch with it, so a device copy can be brought up to
+// date page by page (the page versions of HomoStorage / CudaSVO::Update,
+// reaction/report/main.tex:67-71).
+constexpr uint32_t kScenePageNodes = 256;
+uint64_t next_scene_uid();
+
 struct yx_scene {
     std::vector<uint32_t> nodes;
     uint32_t root = YX_EMPTY_NODE;
     uint64_t num_leaves = 0;
-    int depth = 0;
+    mutable int depth = 0;                 // deepest leaf level, computed on demand after edits
+    mutable bool depth_stale = false;      // edited since depth was computed
+    uint64_t uid = next_scene_uid();       // never reused by another scene object
+    uint64_t version = 1;                  // 1 at creation, +1 per edit
+    std::vector<uint64_t> page_version;    // per page; pages past the end: 1
+    uint64_t page_ver(size_t p) const { return p < page_version.size() ? page_version[p] : 1u; }
 };
 
 namespace yx {
@@ -75,6 +87,20 @@ struct GpuTree {
     std::vector<uint32_t> lod;
 };
 int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base = 0, bool with_lod = false);
+
+// The same records and LOD colours built on the GPU from a device copy of the
+// VoxNode array (10 u32 per node). recs (2 u32 per record) and lod are
+// hipMalloc'ed and owned by the caller on success; empty trees give none.
+struct GpuLayout {
+    uint32_t* recs = nullptr;
+    uint32_t* lod = nullptr;
+    size_t n_recs = 0;
+    int depth = 0;
+    int node_levels = 0;
+    std::vector<uint32_t> level_records;
+    bool empty = true;
+};
+int gpu_relayout(const uint32_t* d_vox, uint32_t nn, uint32_t root, void* stream, GpuLayout& out);
 
 // Pixel-ray basis of a pinhole camera: dir0, du, dv as 9 floats; the ray of
 // pixel (x, y) is normalize(dir0 + du x + dv y).

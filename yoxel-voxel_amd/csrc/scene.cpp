@@ -657,6 +657,11 @@ int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base, bool with_lod
 
 using namespace yx;
 
+uint64_t next_scene_uid() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1);
+}
+
 // ---- C ABI: scenes --------------------------------------------------------
 
 extern "C" int yx_scene_build_synthetic(const yx_scene_desc* desc, yx_scene** out) {
@@ -747,9 +752,61 @@ extern "C" int yx_scene_save_vox(const yx_scene* s, const char* path) {
     return ok ? YX_OK : fail(YX_ERR_IO, "write to %s failed", path);
 }
 
+extern "C" int yx_scene_write_nodes(yx_scene* s, uint32_t first, const uint32_t* nodes, uint32_t count) {
+    clear_error();
+    if (!s || (!nodes && count)) return fail(YX_ERR_INVALID_ARG, "null argument");
+    const size_t size = s->nodes.size() / 10;
+    if (first > size) return fail(YX_ERR_RANGE, "first node %u beyond the end (%zu nodes)", first, size);
+    const uint64_t end = (uint64_t)first + count;
+    if (end >= 0x80000000ull) return fail(YX_ERR_RANGE, "node ids must stay below 2^31");
+    if (count == 0) return YX_OK;
+    try {
+        if (end > size) s->nodes.resize((size_t)end * 10, 0u);
+        const size_t pages = (size_t)((end + kScenePageNodes - 1) / kScenePageNodes);
+        if (s->page_version.size() < pages) s->page_version.resize(pages, 1u);
+    } catch (const std::bad_alloc&) {
+        return fail(YX_ERR_OOM, "out of host memory growing the scene");
+    }
+    uint64_t leaves = s->num_leaves;
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t* dst = &s->nodes[((size_t)first + i) * 10];
+        if ((size_t)first + i < size) leaves -= (uint64_t)__builtin_popcount(dst[0] & 0xffu);
+        std::memcpy(dst, nodes + (size_t)i * 10, 40);
+        leaves += (uint64_t)__builtin_popcount(dst[0] & 0xffu);
+    }
+    s->num_leaves = leaves;
+    s->version += 1;
+    for (uint64_t p = first / kScenePageNodes; p <= (end - 1) / kScenePageNodes; ++p)
+        s->page_version[(size_t)p] = s->version;
+    s->depth_stale = true;
+    return YX_OK;
+}
+
+extern "C" int yx_scene_set_root(yx_scene* s, uint32_t root) {
+    clear_error();
+    if (!s) return fail(YX_ERR_INVALID_ARG, "null argument");
+    s->root = root;
+    s->version += 1;
+    s->depth_stale = true;
+    return YX_OK;
+}
+
 extern "C" int yx_scene_get_info(const yx_scene* s, yx_scene_info* info) {
     clear_error();
     if (!s || !info) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (s->depth_stale) {
+        // edited since the depth was known: validate and measure again
+        GpuTree t;
+        int rc;
+        try {
+            rc = build_gpu_tree(*s, t);
+        } catch (const std::bad_alloc&) {
+            return fail(YX_ERR_OOM, "out of host memory");
+        }
+        if (rc != YX_OK) return rc;
+        s->depth = t.depth;
+        s->depth_stale = false;
+    }
     info->root = s->root;
     info->num_nodes = (uint32_t)(s->nodes.size() / 10);
     info->num_leaves = s->num_leaves;

@@ -142,6 +142,13 @@ struct yx_renderer {
     float* d_z[2] = {nullptr, nullptr};
     uint32_t* d_albedo = nullptr;
     size_t work_pixels = 0;
+    // device copy of a scene's VoxNode array, kept in sync page by page
+    // (CudaSVO::Update): which scene object and version it holds
+    uint32_t* d_vox = nullptr;
+    size_t vox_cap = 0, vox_n = 0;  // nodes
+    uint64_t mirror_uid = 0, mirror_version = 0;
+    uint32_t last_pages = 0;
+    uint64_t last_bytes = 0;
 };
 
 #define YX_HIP(call)                                                                        \
@@ -338,6 +345,7 @@ extern "C" void yx_renderer_destroy(yx_renderer* r) {
     for (float* z : r->d_z)
         if (z) (void)hipFree(z);
     if (r->d_albedo) (void)hipFree(r->d_albedo);
+    if (r->d_vox) (void)hipFree(r->d_vox);
     delete r;
 }
 
@@ -776,5 +784,129 @@ extern "C" int yx_render_pipeline(yx_renderer* r, const yx_pipeline_outputs* out
     const dim3 sgrid((unsigned)((r->w + kShadeTX - 1) / kShadeTX), (unsigned)((r->h + kShadeTY - 1) / kShadeTY));
     hipLaunchKernelGGL(shade_simple_kernel, sgrid, dim3(kShadeTX * kShadeTY), 0, st, sa);
     YX_HIP(hipGetLastError());
+    return YX_OK;
+}
+
+// ---- incremental scene sync (CudaSVO::Update, reaction/report/main.tex:67-71) ----
+
+// Pages of `s` the renderer's device copy lacks: all of them for another scene
+// object, else those edited after the synced version and those past its end.
+static void dirty_pages(const yx_renderer* r, const yx_scene* s, std::vector<std::pair<size_t, size_t>>* runs,
+                        uint32_t* pages, uint64_t* bytes) {
+    const size_t nn = s->nodes.size() / 10;
+    const size_t np = (nn + kScenePageNodes - 1) / kScenePageNodes;
+    const bool all = r->mirror_uid != s->uid;
+    const size_t have = all ? 0 : r->vox_n;
+    uint32_t cnt = 0;
+    uint64_t b = 0;
+    size_t run_begin = 0;
+    bool in_run = false;
+    for (size_t p = 0; p <= np; ++p) {
+        bool d = false;
+        if (p < np) {
+            const size_t n0 = p * kScenePageNodes, n1 = std::min(nn, n0 + kScenePageNodes);
+            d = all || n1 > have || s->page_ver(p) > r->mirror_version;
+            if (d) {
+                ++cnt;
+                b += (uint64_t)(n1 - n0) * 40u;
+            }
+        }
+        if (d && !in_run) {
+            run_begin = p;
+            in_run = true;
+        } else if (!d && in_run) {
+            if (runs) runs->push_back({run_begin * kScenePageNodes, std::min(nn, p * kScenePageNodes)});
+            in_run = false;
+        }
+    }
+    if (pages) *pages = cnt;
+    if (bytes) *bytes = b;
+}
+
+extern "C" int yx_renderer_count_changed_pages(const yx_renderer* r, const yx_scene* s, uint32_t* pages,
+                                               uint64_t* bytes) {
+    clear_error();
+    if (!r || !s || !pages || !bytes) return fail(YX_ERR_INVALID_ARG, "null argument");
+    dirty_pages(r, s, nullptr, pages, bytes);
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_update_scene(yx_renderer* r, const yx_scene* s, void* stream) {
+    clear_error();
+    if (!r || !s) return fail(YX_ERR_INVALID_ARG, "null argument");
+    int rc = ensure_device(r);
+    if (rc != YX_OK) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    const size_t nn = s->nodes.size() / 10;
+    std::vector<std::pair<size_t, size_t>> runs;
+    uint32_t pages = 0;
+    uint64_t bytes = 0;
+    dirty_pages(r, s, &runs, &pages, &bytes);
+    const bool same = r->mirror_uid == s->uid;
+    if (nn > r->vox_cap) {
+        const size_t cap = std::max(nn, r->vox_cap + r->vox_cap / 2);
+        uint32_t* d = nullptr;
+        if (hipMalloc((void**)&d, cap * 40u) != hipSuccess)
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) for the scene's node copy", cap * 40u);
+        if (same && r->vox_n) {
+            const hipError_t e = hipMemcpyAsync(d, r->d_vox, r->vox_n * 40u, hipMemcpyDeviceToDevice, st);
+            if (e != hipSuccess) {
+                (void)hipFree(d);
+                return fail(YX_ERR_HIP, "copying the node copy: %s", hipGetErrorString(e));
+            }
+            YX_HIP(hipStreamSynchronize(st));
+        }
+        if (r->d_vox) YX_HIP(hipFree(r->d_vox));
+        r->d_vox = d;
+        r->vox_cap = cap;
+    }
+    // until the copy below completes the device copy is in no known state
+    r->mirror_uid = 0;
+    for (const auto& run : runs)
+        YX_HIP(hipMemcpyAsync(r->d_vox + run.first * 10, s->nodes.data() + run.first * 10,
+                              (run.second - run.first) * 40u, hipMemcpyHostToDevice, st));
+    // the copies read the scene's host memory: complete them before returning
+    YX_HIP(hipStreamSynchronize(st));
+    r->vox_n = nn;
+    r->mirror_uid = s->uid;
+    r->mirror_version = s->version;
+    r->last_pages = pages;
+    r->last_bytes = bytes;
+    GpuLayout L;
+    rc = gpu_relayout(r->d_vox, (uint32_t)nn, s->root, (void*)st, L);
+    if (rc != YX_OK) return rc;  // the previous records stay in place
+    if (r->d_recs) YX_HIP(hipFree(r->d_recs));
+    if (r->d_lod) YX_HIP(hipFree(r->d_lod));
+    r->d_recs = reinterpret_cast<uint2*>(L.recs);
+    r->d_lod = L.lod;
+    r->n_recs = L.empty ? 0 : L.n_recs;
+    r->depth = L.empty ? 0 : L.depth;
+    r->node_levels = L.empty ? 0 : L.node_levels;
+    if (L.empty)
+        r->level_records.clear();
+    else
+        r->level_records = L.level_records;
+    r->has_scene = true;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_last_update(const yx_renderer* r, uint32_t* pages, uint64_t* bytes) {
+    clear_error();
+    if (!r || !pages || !bytes) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *pages = r->last_pages;
+    *bytes = r->last_bytes;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_layout(const yx_renderer* r, uint32_t* recs, uint32_t* lod, uint64_t cap,
+                                  uint64_t* n_recs) {
+    clear_error();
+    if (!r || !n_recs) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *n_recs = r->n_recs;
+    if (!recs && !lod) return YX_OK;
+    if (cap < r->n_recs) return fail(YX_ERR_RANGE, "capacity %llu below %zu records", (unsigned long long)cap, r->n_recs);
+    if (r->n_recs == 0) return YX_OK;
+    if (recs) YX_HIP(hipMemcpy(recs, r->d_recs, r->n_recs * 8u, hipMemcpyDeviceToHost));
+    if (lod) YX_HIP(hipMemcpy(lod, r->d_lod, r->n_recs * 4u, hipMemcpyDeviceToHost));
     return YX_OK;
 }

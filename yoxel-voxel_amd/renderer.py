@@ -4,7 +4,10 @@ SVOData      - cell/svodata.h:22-55 (Load, GetRoot, operator[])
 SVORenderer  - ISVORenderer (cell/svorenderer.h:5-31): SetScene, SetViewPos,
                SetViewDir, SetViewUp, SetResolution, GetResolution, SetFOV,
                RenderFrame; plus the GPU renderer's Render(dst)
-               (demo/SVORenderer.h:8-39) and the light setter.
+               (demo/SVORenderer.h:8-39) and the light setter, the GPU
+               pipeline's settings and RenderPipeline
+               (demo/SVORenderer.cpp:95-149), and the incremental scene sync
+               UpdateScene / CountChangedPages (CudaSVO::Update).
 Error behaviour follows the reference: RenderFrame returns None when no scene
 is set (cell/ppu_renderer.cpp:78-79); other failures raise YoxelError.
 """
@@ -79,6 +82,16 @@ class SVOData:
                                                  ctypes.c_void_p(_stream(stream))))
         s._h = h
         return s
+
+    # -- edits (the node pool of DynamicSVO, reaction/report/main.tex:67-71) --
+    def write_nodes(self, first, nodes):
+        """Write VoxNode records (n x 10 u32) from index `first` (at most the
+        current size); their 256-node pages get a new version."""
+        a = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 10)
+        check(lib().yx_scene_write_nodes(self._h, int(first), a.ctypes.data_as(ctypes.c_void_p), a.shape[0]))
+
+    def set_root(self, root):
+        check(lib().yx_scene_set_root(self._h, int(root)))
 
     # -- accessors ---------------------------------------------------------
     def info(self):
@@ -201,6 +214,38 @@ class SVORenderer:
         out = {k: torch.empty(spec[k][0], dtype=spec[k][1], device=dev) for k in outputs}
         self.Render(stream=stream, **out)
         return out
+
+    # ---- incremental sync (CudaSVO::Update, qtview.py:69-73) ----------------
+
+    def UpdateScene(self, svo, stream=None):
+        """Upload the pages of `svo` edited since the last sync (all of them for
+        another scene) and rebuild the traversal layout on the GPU."""
+        check(lib().yx_renderer_update_scene(self._h, svo.handle, ctypes.c_void_p(_stream(stream))))
+
+    def CountChangedPages(self, svo):
+        """(pages, bytes) the next UpdateScene would upload
+        (DynamicSVO::CountChangedPages / CountTransfrerSize)."""
+        p = ctypes.c_uint32()
+        b = ctypes.c_uint64()
+        check(lib().yx_renderer_count_changed_pages(self._h, svo.handle, ctypes.byref(p), ctypes.byref(b)))
+        return p.value, b.value
+
+    def last_update(self):
+        p = ctypes.c_uint32()
+        b = ctypes.c_uint64()
+        check(lib().yx_renderer_last_update(self._h, ctypes.byref(p), ctypes.byref(b)))
+        return p.value, b.value
+
+    def layout(self):
+        """The device traversal layout: (records (n, 2) uint32, LOD colours (n,) uint32)."""
+        n = ctypes.c_uint64()
+        check(lib().yx_renderer_layout(self._h, None, None, 0, ctypes.byref(n)))
+        recs = np.zeros((n.value, 2), np.uint32)
+        lod = np.zeros(n.value, np.uint32)
+        if n.value:
+            check(lib().yx_renderer_layout(self._h, recs.ctypes.data_as(ctypes.c_void_p),
+                                           lod.ctypes.data_as(ctypes.c_void_p), n.value, ctypes.byref(n)))
+        return recs, lod
 
     # ---- the GPU pipeline of SVORenderer::Render (demo/SVORenderer.cpp:95-149)
 
