@@ -42,7 +42,8 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain_512 measurement")
     ap.add_argument("--no-pipeline", action="store_true",
                     help="skip the shading-pipeline measurement (LOD trace, SSNA blur, Phong shade)")
-    ap.add_argument("--only", choices=["composite", "batch", "brick", "plain", "pipeline"], default=None,
+    ap.add_argument("--no-update", action="store_true", help="skip the scene-sync measurement")
+    ap.add_argument("--only", choices=["composite", "batch", "brick", "plain", "pipeline", "update"], default=None,
                     help="run one secondary workload alone and print its object (profiling)")
     ap.add_argument("--brick-frames", type=int, default=8)
     ap.add_argument("--autotune", action="store_true",
@@ -77,6 +78,8 @@ def main():
             leg = plain_bench(yx, torch, not args.no_cpu_baseline)
         elif args.only == "batch":
             leg = batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
+        elif args.only == "update":
+            leg = update_bench(yx, torch, yx.SVOData.build_synthetic_gpu(**wl.LARGE_1080P))
         elif args.only == "pipeline":
             leg = pipeline_bench(yx, torch, yx.SVOData.build_synthetic(**wl.LARGE_1080P), wl.orbit_cameras(32, seed=0),
                                  not args.no_cpu_baseline)
@@ -180,6 +183,7 @@ def main():
     brick = None if args.no_brick else brick_bench(yx, torch, args.brick_frames, not args.no_cpu_baseline)
     pipeline = (None if args.no_pipeline or args.workload != "large_1080p"
                 else pipeline_bench(yx, torch, scene, cams, not args.no_cpu_baseline))
+    update = None if args.no_update or args.workload != "large_1080p" else update_bench(yx, torch, scene)
 
     line = {
         "metric": BASELINE_METRIC,
@@ -211,6 +215,7 @@ def main():
         "batched": batched,
         "brick": brick,
         "pipeline": pipeline,
+        "update": update,
     }
     print(json.dumps(line), flush=True)
 
@@ -439,6 +444,56 @@ def pipeline_bench(yx, torch, scene, cams, with_cpu, reps=2, cpu_frames=4):
                                "kind": "port", "sample": f"orbit frames {sel}, full 1920x1080 pipeline "
                                                          f"(trace, 5 blurs, shade), {dt:.1f} s"}
     return res
+
+
+def update_bench(yx, torch, scene, reps=3):
+    """Scene sync on large_1080p (14.2 M VoxNodes, 46.4 M records):
+    SetScene (host breadth-first re-layout + LOD colours + upload of the
+    records) against UpdateScene (CudaSVO::Update, demo/SVORenderer.cpp:39-53:
+    upload of the changed 256-node pages to the device node copy, re-layout
+    and LOD colours on the GPU). Wall time per call, synchronised; the edit
+    recolours the leaves of 64 nodes spread over 4 pages."""
+    nodes = scene.nodes()
+    cand = np.nonzero(nodes[:, 0] & 0xFF)[0]
+    pages = sorted({int(cand[int(k * len(cand) / 4)]) // 256 for k in range(4)})
+    idx = [int(i) for p in pages for i in cand[(cand // 256) == p][:16]]
+    orig = nodes[idx].copy()
+    edited = orig.copy()
+    for row in edited:
+        for sl in range(8):
+            if (int(row[0]) >> sl) & 1:
+                row[2 + sl] = 0xFF0000FF
+
+    def wall(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3
+
+    r_host = yx.SVORenderer()
+    r_gpu = yx.SVORenderer()
+    host_ms = min(wall(lambda: r_host.SetScene(scene)) for _ in range(reps))
+    full_ms = wall(lambda: r_gpu.UpdateScene(scene))
+    full_pages, full_bytes = r_gpu.last_update()
+    nop_ms = min(wall(lambda: r_gpu.UpdateScene(scene)) for _ in range(reps))
+    edit_ms = []
+    for k in range(reps):
+        for i, row in zip(idx, edited if k % 2 == 0 else orig):
+            scene.write_nodes(i, row)
+        edit_ms.append(wall(lambda: r_gpu.UpdateScene(scene)))
+        edit_pages, edit_bytes = r_gpu.last_update()
+    for i, row in zip(idx, orig):
+        scene.write_nodes(i, row)
+    r_gpu.UpdateScene(scene)
+    return {"workload": "large_1080p scene sync", "nodes": int(nodes.shape[0]),
+            "set_scene_host_ms": round(host_ms, 1),
+            "update_full_ms": round(full_ms, 1), "update_full_pages": full_pages,
+            "update_full_mb": round(full_bytes / 1e6, 1),
+            "update_unchanged_ms": round(nop_ms, 2),
+            "update_edit_ms": round(min(edit_ms), 2), "update_edit_pages": edit_pages,
+            "update_edit_kb": round(edit_bytes / 1e3, 1),
+            "speedup_edit_vs_set_scene": round(host_ms / min(edit_ms), 1)}
 
 
 def batch_bench(yx, torch, frames, tune_cache):

@@ -89,8 +89,8 @@ struct GpuTree {
 int build_gpu_tree(const yx_scene& s, GpuTree& out, uint32_t base = 0, bool with_lod = false);
 
 // The same records and LOD colours built on the GPU from a device copy of the
-// VoxNode array (10 u32 per node). recs (2 u32 per record) and lod are
-// hipMalloc'ed and owned by the caller on success; empty trees give none.
+// VoxNode array (10 u32 per node): recs (2 u32 per record) and lod on success;
+// empty trees give none.
 struct GpuLayout {
     uint32_t* recs = nullptr;
     uint32_t* lod = nullptr;
@@ -100,7 +100,14 @@ struct GpuLayout {
     std::vector<uint32_t> level_records;
     bool empty = true;
 };
-int gpu_relayout(const uint32_t* d_vox, uint32_t nn, uint32_t root, void* stream, GpuLayout& out);
+// The re-layout's device workspace, kept by its owner between calls.
+struct RelayoutWork;
+RelayoutWork* relayout_work_create();
+void relayout_work_free(RelayoutWork* w);
+// *recs_io / *lod_io (capacity *cap_io records) are reused when large enough,
+// else replaced; they are written only after the tree has been validated.
+int gpu_relayout(const uint32_t* d_vox, uint32_t nn, uint32_t root, void* stream, RelayoutWork* w,
+                 uint32_t** recs_io, uint32_t** lod_io, size_t* cap_io, GpuLayout& out);
 
 // Pixel-ray basis of a pinhole camera: dir0, du, dv as 9 floats; the ray of
 // pixel (x, y) is normalize(dir0 + du x + dv y).

@@ -17,6 +17,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <new>
 #include <vector>
 
 #include "internal.h"
@@ -36,7 +37,6 @@ constexpr uint32_t kNull = 0x80000000u;
 struct RlFlags {
     uint32_t bad_id;  // child id >= number of nodes (the largest found)
     uint32_t shared;  // a node reached twice (the largest such id)
-    uint32_t leaf;    // some node of the level has a leaf
     uint32_t any_bad;
 };
 
@@ -65,7 +65,6 @@ __global__ void rl_count_kernel(const uint32_t* __restrict__ vox, uint32_t nn, c
     }
     nrec[i] = (uint32_t)__popc(lm | cm);
     nnode[i] = (uint32_t)__popc(cm);
-    if (lm) atomicOr(&flags->leaf, 1u);
 }
 
 __global__ void rl_gather_kernel(const uint32_t* __restrict__ vox, const uint32_t* __restrict__ ids, uint32_t n,
@@ -150,108 +149,172 @@ __global__ void rl_lod_kernel(const uint32_t* __restrict__ vox, const uint32_t* 
     lod[rec[i]] = mean;
 }
 
-struct RlBufs {
-    std::vector<void*> ptrs;
-    ~RlBufs() {
-        for (void* p : ptrs) (void)hipFree(p);
-    }
-    template <class T>
-    int alloc(T** p, size_t n) {
-        *p = nullptr;
-        if (hipMalloc((void**)p, std::max<size_t>(1, n) * sizeof(T)) != hipSuccess) {
-            *p = nullptr;
-            return fail(YX_ERR_OOM, "hipMalloc(%zu) in the GPU re-layout", n * sizeof(T));
-        }
-        ptrs.push_back(*p);
-        return YX_OK;
-    }
-};
-
 struct RlLevel {
     uint32_t n = 0;           // nodes of the level
     uint32_t* ids = nullptr;  // canonical ids
     uint32_t* rec = nullptr;  // their record indices
-    uint32_t* roff = nullptr;
-    uint32_t* noff = nullptr;
+    uint32_t* roff = nullptr; // first child record, relative to begin_next
+    uint32_t* noff = nullptr; // position of the first node child in the next level
     uint32_t begin_next = 0;  // first record of the next level
 };
 
-inline dim3 rl_grid(uint32_t n) { return dim3((n + 255u) / 256u); }
+// one level's totals and the id checks, copied to the host in one transfer
+struct RlTail {
+    uint32_t any_bad, bad_id, shared, pad;
+    uint64_t child_recs, child_nodes;
+};
+
+__global__ void rl_tail_kernel(const uint32_t* __restrict__ roff, const uint32_t* __restrict__ nrec,
+                               const uint32_t* __restrict__ noff, const uint32_t* __restrict__ nnode, uint32_t n,
+                               RlFlags* flags, RlTail* tail) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    RlTail t;
+    t.any_bad = flags->any_bad;
+    t.bad_id = flags->bad_id;
+    t.shared = flags->shared;
+    t.pad = 0;
+    t.child_recs = (uint64_t)roff[n - 1] + nrec[n - 1];
+    t.child_nodes = (uint64_t)noff[n - 1] + nnode[n - 1];
+    *tail = t;
+    flags->any_bad = 0;
+    flags->bad_id = 0;
+    flags->shared = 0;
+}
+
+__global__ void rl_init_kernel(uint32_t* __restrict__ seen, uint32_t nn, uint32_t root, uint32_t* ids, uint32_t* rec,
+                               RlFlags* flags) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nn) seen[i] = (i == root) ? 1u : 0u;
+    if (i == 0) {
+        ids[0] = root;
+        rec[0] = 0;
+        flags->any_bad = 0;
+        flags->bad_id = 0;
+        flags->shared = 0;
+    }
+}
+
+inline dim3 rl_grid(uint64_t n) { return dim3((unsigned)((n + 255u) / 256u)); }
+
+template <class T>
+int rl_grow(T** p, size_t* cap, size_t need, const char* what) {
+    if (need <= *cap && *p) return YX_OK;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    if (hipMalloc((void**)p, std::max<size_t>(need, 1) * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return fail(YX_ERR_OOM, "hipMalloc(%zu) for %s", need * sizeof(T), what);
+    }
+    *cap = need;
+    return YX_OK;
+}
 
 }  // namespace
 
-int gpu_relayout(const uint32_t* d_vox, uint32_t nn, uint32_t root, void* stream, GpuLayout& out) {
+// Device workspace of the re-layout, kept between calls: node-sized arrays
+// (level lists laid end to end: level l at [off_l, off_l + n_l)), the subtree
+// sums of the widest level, scan scratch and a pinned host copy of RlTail.
+struct RelayoutWork {
+    size_t cap = 0, cap_wide = 0, scan_bytes = 0;
+    uint32_t *seen = nullptr, *ids = nullptr, *rec = nullptr, *roff = nullptr, *noff = nullptr;
+    uint32_t *nrec = nullptr, *nnode = nullptr;
+    uint64_t *sa = nullptr, *sb = nullptr;
+    void* scan_tmp = nullptr;
+    RlFlags* flags = nullptr;
+    RlTail* tail = nullptr;
+    RlTail* h_tail = nullptr;  // pinned
+    ~RelayoutWork() {
+        for (void* p : {(void*)seen, (void*)ids, (void*)rec, (void*)roff, (void*)noff, (void*)nrec, (void*)nnode,
+                        (void*)sa, (void*)sb, scan_tmp, (void*)flags, (void*)tail})
+            if (p) (void)hipFree(p);
+        if (h_tail) (void)hipHostFree(h_tail);
+    }
+};
+
+RelayoutWork* relayout_work_create() { return new (std::nothrow) RelayoutWork(); }
+void relayout_work_free(RelayoutWork* w) { delete w; }
+
+int gpu_relayout(const uint32_t* d_vox, uint32_t nn, uint32_t root, void* stream, RelayoutWork* w,
+                 uint32_t** recs_io, uint32_t** lod_io, size_t* cap_io, GpuLayout& out) {
     const hipStream_t st = (hipStream_t)stream;
     out = GpuLayout();
+    if (!w) return fail(YX_ERR_INVALID_ARG, "no re-layout workspace");
     if (root & kNull) return YX_OK;  // EmptyNode / FullNode root: nothing to hit
     if (root >= nn) return fail(YX_ERR_FORMAT, "root id %u out of range (%u nodes)", root, nn);
-    RlBufs B;
+    if (nn >= (1u << 29)) return fail(YX_ERR_RANGE, "at most 2^29 nodes");
     int rc;
-    uint32_t *d_seen = nullptr, *d_tmp_n = nullptr;
-    RlFlags* d_flags = nullptr;
-    if ((rc = B.alloc(&d_seen, nn)) != YX_OK || (rc = B.alloc(&d_flags, 1)) != YX_OK ||
-        (rc = B.alloc(&d_tmp_n, 2)) != YX_OK)
-        return rc;
-    RL_HIP(hipMemsetAsync(d_seen, 0, (size_t)nn * sizeof(uint32_t), st));
-    const uint32_t one = 1u, zero = 0u;
-    RL_HIP(hipMemcpyAsync(d_seen + root, &one, sizeof(uint32_t), hipMemcpyHostToDevice, st));
-
-    std::vector<RlLevel> lv(1);
-    if ((rc = B.alloc(&lv[0].ids, 1)) != YX_OK || (rc = B.alloc(&lv[0].rec, 1)) != YX_OK) return rc;
-    RL_HIP(hipMemcpyAsync(lv[0].ids, &root, sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    RL_HIP(hipMemcpyAsync(lv[0].rec, &zero, sizeof(uint32_t), hipMemcpyHostToDevice, st));
-    lv[0].n = 1;
-
-    uint64_t total = 1;  // records so far (the root's)
-    int depth = 0;
-    std::vector<uint32_t> level_records(1, 1u);
-    void* scan_tmp = nullptr;
-    size_t scan_bytes = 0;
-    uint32_t* d_nrec = nullptr;
-    uint32_t* d_nnode = nullptr;
-    uint32_t cap_n = 0;
-    for (size_t l = 0; lv[l].n > 0; ++l) {
-        const uint32_t n = lv[l].n;
-        if (l > 0) level_records.push_back((uint32_t)total);
-        if (n > cap_n) {
-            cap_n = std::max(n, cap_n * 2u);
-            if ((rc = B.alloc(&d_nrec, cap_n)) != YX_OK || (rc = B.alloc(&d_nnode, cap_n)) != YX_OK) return rc;
-            size_t bytes = 0;
-            RL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, d_nrec, d_nrec, (int)cap_n, st));
-            if (bytes > scan_bytes) {
-                if ((rc = B.alloc((uint8_t**)&scan_tmp, bytes)) != YX_OK) return rc;
-                scan_bytes = bytes;
+    if (w->cap < nn) {
+        size_t c = 0;
+        for (uint32_t** p : {&w->seen, &w->ids, &w->rec, &w->roff, &w->noff, &w->nrec, &w->nnode}) {
+            c = w->cap;
+            if ((rc = rl_grow(p, &c, nn, "the re-layout workspace")) != YX_OK) {
+                w->cap = 0;
+                return rc;
             }
         }
-        RL_HIP(hipMemsetAsync(d_flags, 0, sizeof(RlFlags), st));
-        hipLaunchKernelGGL(rl_count_kernel, rl_grid(n), dim3(256), 0, st, d_vox, nn, lv[l].ids, n, d_nrec, d_nnode,
-                           d_seen, d_flags);
+        w->cap = nn;
+        size_t bytes = 0;
+        RL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, w->nrec, w->roff, (int)nn, st));
+        if ((rc = rl_grow((uint8_t**)&w->scan_tmp, &w->scan_bytes, bytes, "scan scratch")) != YX_OK) return rc;
+    }
+    if (!w->flags) {
+        size_t c = 0;
+        if ((rc = rl_grow(&w->flags, &c, 1, "flags")) != YX_OK) return rc;
+        c = 0;
+        if ((rc = rl_grow(&w->tail, &c, 1, "flags")) != YX_OK) return rc;
+        if (hipHostMalloc((void**)&w->h_tail, sizeof(RlTail), hipHostMallocDefault) != hipSuccess) {
+            w->h_tail = nullptr;
+            return fail(YX_ERR_OOM, "pinned host memory for the re-layout");
+        }
+    }
+    hipLaunchKernelGGL(rl_init_kernel, rl_grid(nn), dim3(256), 0, st, w->seen, nn, root, w->ids, w->rec, w->flags);
+    RL_HIP(hipGetLastError());
+
+    std::vector<RlLevel> lv(1);
+    lv[0].n = 1;
+    lv[0].ids = w->ids;
+    lv[0].rec = w->rec;
+    lv[0].roff = w->roff;
+    lv[0].noff = w->noff;
+    size_t off = 0;  // first slot of the current level in the node-sized arrays
+    uint64_t total = 1;  // records so far (the root's)
+    int depth = 0;
+    uint32_t widest = 1;
+    std::vector<uint32_t> level_records(1, 1u);
+    for (size_t l = 0; lv[l].n > 0; ++l) {
+        const uint32_t n = lv[l].n;
+        widest = std::max(widest, n);
+        if (l > 0) level_records.push_back((uint32_t)total);
+        hipLaunchKernelGGL(rl_count_kernel, rl_grid(n), dim3(256), 0, st, d_vox, nn, lv[l].ids, n, w->nrec, w->nnode,
+                           w->seen, w->flags);
         RL_HIP(hipGetLastError());
-        if ((rc = B.alloc(&lv[l].roff, n)) != YX_OK || (rc = B.alloc(&lv[l].noff, n)) != YX_OK) return rc;
-        RL_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, d_nrec, lv[l].roff, (int)n, st));
-        RL_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp, scan_bytes, d_nnode, lv[l].noff, (int)n, st));
-        RlFlags h_flags;
-        uint32_t last[4];  // roff, nrec, noff, nnode of the last node
-        RL_HIP(hipMemcpyAsync(&h_flags, d_flags, sizeof(RlFlags), hipMemcpyDeviceToHost, st));
-        RL_HIP(hipMemcpyAsync(&last[0], lv[l].roff + (n - 1), 4, hipMemcpyDeviceToHost, st));
-        RL_HIP(hipMemcpyAsync(&last[1], d_nrec + (n - 1), 4, hipMemcpyDeviceToHost, st));
-        RL_HIP(hipMemcpyAsync(&last[2], lv[l].noff + (n - 1), 4, hipMemcpyDeviceToHost, st));
-        RL_HIP(hipMemcpyAsync(&last[3], d_nnode + (n - 1), 4, hipMemcpyDeviceToHost, st));
+        RL_HIP(hipcub::DeviceScan::ExclusiveSum(w->scan_tmp, w->scan_bytes, w->nrec, lv[l].roff, (int)n, st));
+        RL_HIP(hipcub::DeviceScan::ExclusiveSum(w->scan_tmp, w->scan_bytes, w->nnode, lv[l].noff, (int)n, st));
+        hipLaunchKernelGGL(rl_tail_kernel, dim3(1), dim3(64), 0, st, lv[l].roff, w->nrec, lv[l].noff, w->nnode, n,
+                           w->flags, w->tail);
+        RL_HIP(hipGetLastError());
+        RL_HIP(hipMemcpyAsync(w->h_tail, w->tail, sizeof(RlTail), hipMemcpyDeviceToHost, st));
         RL_HIP(hipStreamSynchronize(st));
-        if (h_flags.any_bad & 1u) return fail(YX_ERR_FORMAT, "child id %u out of range (%u nodes)", h_flags.bad_id, nn);
-        if (h_flags.any_bad & 2u) return fail(YX_ERR_FORMAT, "node %u reached twice (not a tree)", h_flags.shared);
-        const uint64_t child_recs = (uint64_t)last[0] + last[1];
-        const uint32_t child_nodes = last[2] + last[3];
-        if (child_recs > 0 && (int)l + 1 > kMaxDepth) return fail(YX_ERR_RANGE, "tree deeper than %d levels", kMaxDepth);
-        if (h_flags.leaf) depth = (int)l + 1;
+        const RlTail t = *w->h_tail;
+        if (t.any_bad & 1u) return fail(YX_ERR_FORMAT, "child id %u out of range (%u nodes)", t.bad_id, nn);
+        if (t.any_bad & 2u) return fail(YX_ERR_FORMAT, "node %u reached twice (not a tree)", t.shared);
+        if (t.child_recs > 0 && (int)l + 1 > kMaxDepth)
+            return fail(YX_ERR_RANGE, "tree deeper than %d levels", kMaxDepth);
+        if (t.child_recs > t.child_nodes) depth = (int)l + 1;  // some child is a leaf
         lv[l].begin_next = (uint32_t)total;
-        total += child_recs;
+        total += t.child_recs;
         if (total + 8 > 0xFFFFFFF0ull) return fail(YX_ERR_RANGE, "more than 2^32 records");
+        off += n;
         lv.emplace_back();
         RlLevel& nx = lv.back();
-        nx.n = child_nodes;
-        if (child_nodes > 0) {
-            if ((rc = B.alloc(&nx.ids, child_nodes)) != YX_OK || (rc = B.alloc(&nx.rec, child_nodes)) != YX_OK) return rc;
+        nx.n = (uint32_t)t.child_nodes;
+        // without shared nodes every node appears once: the lists fit in nn
+        nx.ids = w->ids + off;
+        nx.rec = w->rec + off;
+        nx.roff = w->roff + off;
+        nx.noff = w->noff + off;
+        if (nx.n > 0) {
             hipLaunchKernelGGL(rl_gather_kernel, rl_grid(n), dim3(256), 0, st, d_vox, lv[l].ids, n, lv[l].roff,
                                lv[l].noff, lv[l].begin_next, nx.ids, nx.rec);
             RL_HIP(hipGetLastError());
@@ -261,39 +324,49 @@ int gpu_relayout(const uint32_t* d_vox, uint32_t nn, uint32_t root, void* stream
     level_records.push_back((uint32_t)total);
     if (depth == 0) return YX_OK;  // no leaf anywhere: nothing to hit
 
-    uint2* d_recs = nullptr;
-    uint32_t* d_lod = nullptr;
-    if (hipMalloc((void**)&d_recs, total * sizeof(uint2)) != hipSuccess)
-        return fail(YX_ERR_OOM, "hipMalloc(%zu) for the records", (size_t)(total * sizeof(uint2)));
-    if (hipMalloc((void**)&d_lod, total * sizeof(uint32_t)) != hipSuccess) {
-        (void)hipFree(d_recs);
-        return fail(YX_ERR_OOM, "hipMalloc(%zu) for the LOD colours", (size_t)(total * sizeof(uint32_t)));
+    // the caller's record arrays, grown only when too small
+    if (*cap_io < total || !*recs_io || !*lod_io) {
+        uint32_t *nr = nullptr, *nl = nullptr;
+        if (hipMalloc((void**)&nr, total * 8u) != hipSuccess)
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) for the records", (size_t)(total * 8u));
+        if (hipMalloc((void**)&nl, total * 4u) != hipSuccess) {
+            (void)hipFree(nr);
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) for the LOD colours", (size_t)(total * 4u));
+        }
+        if (*recs_io) (void)hipFree(*recs_io);
+        if (*lod_io) (void)hipFree(*lod_io);
+        *recs_io = nr;
+        *lod_io = nl;
+        *cap_io = (size_t)total;
     }
-    auto give_up = [&](int code) {
-        (void)hipFree(d_recs);
-        (void)hipFree(d_lod);
-        return code;
-    };
+    uint2* d_recs = reinterpret_cast<uint2*>(*recs_io);
+    uint32_t* d_lod = *lod_io;
     for (size_t l = 0; l < levels; ++l) {
         hipLaunchKernelGGL(rl_write_kernel, rl_grid(lv[l].n), dim3(256), 0, st, d_vox, lv[l].ids, lv[l].rec, lv[l].n,
                            lv[l].roff, lv[l].begin_next, d_recs, d_lod);
-        if (hipGetLastError() != hipSuccess) return give_up(fail(YX_ERR_HIP, "re-layout write kernel launch"));
+        RL_HIP(hipGetLastError());
     }
     // bottom-up subtree sums: two buffers of 5 u64 per node of the widest level
-    uint32_t widest = 0;
-    for (size_t l = 0; l < levels; ++l) widest = std::max(widest, lv[l].n);
-    uint64_t *d_sa = nullptr, *d_sb = nullptr;
-    if ((rc = B.alloc(&d_sa, (size_t)widest * 5)) != YX_OK || (rc = B.alloc(&d_sb, (size_t)widest * 5)) != YX_OK)
-        return give_up(rc);
+    if (w->cap_wide < widest) {
+        size_t c = w->cap_wide * 5;
+        if ((rc = rl_grow(&w->sa, &c, (size_t)widest * 5, "subtree sums")) != YX_OK) return rc;
+        c = w->cap_wide * 5;
+        if ((rc = rl_grow(&w->sb, &c, (size_t)widest * 5, "subtree sums")) != YX_OK) {
+            w->cap_wide = 0;
+            return rc;
+        }
+        w->cap_wide = widest;
+    }
+    uint64_t *sa = w->sa, *sb = w->sb;
     for (size_t l = levels; l-- > 0;) {
         hipLaunchKernelGGL(rl_lod_kernel, rl_grid(lv[l].n), dim3(256), 0, st, d_vox, lv[l].ids, lv[l].rec, lv[l].n,
-                           lv[l].noff, (const uint64_t*)d_sb, d_sa, d_lod);
-        if (hipGetLastError() != hipSuccess) return give_up(fail(YX_ERR_HIP, "re-layout LOD kernel launch"));
-        std::swap(d_sa, d_sb);
+                           lv[l].noff, (const uint64_t*)sb, sa, d_lod);
+        RL_HIP(hipGetLastError());
+        std::swap(sa, sb);
     }
-    if (hipStreamSynchronize(st) != hipSuccess) return give_up(fail(YX_ERR_HIP, "re-layout"));
-    out.recs = reinterpret_cast<uint32_t*>(d_recs);
-    out.lod = d_lod;
+    RL_HIP(hipStreamSynchronize(st));
+    out.recs = *recs_io;
+    out.lod = *lod_io;
     out.n_recs = (size_t)total;
     out.depth = depth;
     out.node_levels = (int)levels;

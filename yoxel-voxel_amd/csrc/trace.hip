@@ -149,6 +149,8 @@ struct yx_renderer {
     uint64_t mirror_uid = 0, mirror_version = 0;
     uint32_t last_pages = 0;
     uint64_t last_bytes = 0;
+    RelayoutWork* rl_work = nullptr;  // GPU re-layout workspace
+    size_t recs_cap = 0;              // records d_recs / d_lod can hold
 };
 
 #define YX_HIP(call)                                                                        \
@@ -346,6 +348,7 @@ extern "C" void yx_renderer_destroy(yx_renderer* r) {
         if (z) (void)hipFree(z);
     if (r->d_albedo) (void)hipFree(r->d_albedo);
     if (r->d_vox) (void)hipFree(r->d_vox);
+    relayout_work_free(r->rl_work);
     delete r;
 }
 
@@ -416,6 +419,7 @@ extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
         }
         YX_HIP(hipMemcpy(r->d_lod, t.lod.data(), lod_bytes, hipMemcpyHostToDevice));
         r->n_recs = t.recs.size() / 2;
+        r->recs_cap = r->n_recs;
         r->depth = t.depth;
         r->node_levels = t.node_levels;
         r->level_records = t.level_records;
@@ -872,13 +876,24 @@ extern "C" int yx_renderer_update_scene(yx_renderer* r, const yx_scene* s, void*
     r->mirror_version = s->version;
     r->last_pages = pages;
     r->last_bytes = bytes;
+    if (!r->rl_work && !(r->rl_work = relayout_work_create())) return fail(YX_ERR_OOM, "out of host memory");
     GpuLayout L;
-    rc = gpu_relayout(r->d_vox, (uint32_t)nn, s->root, (void*)st, L);
-    if (rc != YX_OK) return rc;  // the previous records stay in place
-    if (r->d_recs) YX_HIP(hipFree(r->d_recs));
-    if (r->d_lod) YX_HIP(hipFree(r->d_lod));
-    r->d_recs = reinterpret_cast<uint2*>(L.recs);
-    r->d_lod = L.lod;
+    uint32_t* recs = reinterpret_cast<uint32_t*>(r->d_recs);
+    uint32_t* lod = r->d_lod;
+    size_t cap = r->d_recs ? r->recs_cap : 0;
+    rc = gpu_relayout(r->d_vox, (uint32_t)nn, s->root, (void*)st, r->rl_work, &recs, &lod, &cap, L);
+    r->d_recs = reinterpret_cast<uint2*>(recs);
+    r->d_lod = lod;
+    r->recs_cap = cap;
+    if (rc != YX_OK) return rc;  // format errors leave the previous records in place
+    if (L.empty) {
+        // nothing to hit: no records (the launchers treat a null record array as empty)
+        if (r->d_recs) YX_HIP(hipFree(r->d_recs));
+        if (r->d_lod) YX_HIP(hipFree(r->d_lod));
+        r->d_recs = nullptr;
+        r->d_lod = nullptr;
+        r->recs_cap = 0;
+    }
     r->n_recs = L.empty ? 0 : L.n_recs;
     r->depth = L.empty ? 0 : L.depth;
     r->node_levels = L.empty ? 0 : L.node_levels;
