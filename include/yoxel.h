@@ -386,6 +386,37 @@ int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol, const floa
 /* Records emitted by the last backward (sum of nsamp) and distinct keys. */
 int yx_compositor_last_stats(const yx_compositor* c, uint64_t* records, uint64_t* unique_keys);
 
+/* ---- brick pool manager (CuBrickPoolManager::allocMap,
+ * reaction/voxtree/brickpool/BrickPool.cpp:21-62; UpdateBrickPool,
+ * reaction/voxtree/density.cu:240-284) ---------------------------------------
+ * Allocates and frees parameter slots of a brick volume in place: the pool
+ * manages the caller's device table and params of `vol` (writing both) and
+ * keeps a used-slot map. Policy: the listed bricks that have no slot, in order
+ * of their first appearance in the list, get the lowest free slots in
+ * increasing order; bricks that have one keep it. The reference fills the
+ * least-used slices of a 3D texture pool; this pool is one linear array.
+ * Calls on one pool must not overlap. */
+typedef struct yx_brick_pool yx_brick_pool;
+/* The table may already map bricks to slots (YX_ERR_FORMAT if a slot is used
+ * twice or lies beyond num_slots). */
+int yx_brick_pool_create(const yx_volume_dev* vol, void* stream, yx_brick_pool** out);
+void yx_brick_pool_destroy(yx_brick_pool* p);
+/* bricks: device int32[n], linear brick indices x + g (y + g z). slots_out
+ * (device int32[n], may be NULL) receives every listed brick's slot. Fails
+ * with YX_ERR_RANGE and changes nothing when the free slots do not suffice or
+ * an index is out of range. */
+int yx_brick_pool_alloc(yx_brick_pool* p, const int32_t* bricks, uint32_t n, int32_t* slots_out,
+                        uint32_t* new_count, void* stream);
+/* Frees the listed bricks' slots (table entries become -1); bricks without a
+ * slot are skipped. */
+int yx_brick_pool_release(yx_brick_pool* p, const int32_t* bricks, uint32_t n, uint32_t* released, void* stream);
+/* UpdateBrickPool: copies packed brick contents (device float4[n][brick^3], in
+ * list order) into the listed bricks' slots; bricks without a slot are
+ * skipped, and a brick listed more than once takes its last entry. Fails with
+ * YX_ERR_RANGE (nothing copied) on an index out of range. */
+int yx_brick_pool_update(yx_brick_pool* p, const int32_t* bricks, uint32_t n, const float* packed, void* stream);
+int yx_brick_pool_stats(const yx_brick_pool* p, uint32_t* used, uint32_t* capacity);
+
 /* ---- brick n-tree volumes ---------------------------------------------------
  * The n-tree of ntree/ntree/nodes.h:13-100 (Grid nodes of 4^3 cells with the
  * ch2ofs cell order (z*4 + y)*4 + x, :99; Brick leaves, ntree/BrickManager.h

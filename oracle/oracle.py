@@ -509,3 +509,56 @@ class PipelineOracle:
 
     def plant(self):
         return self.lib.oracle_plant()
+
+
+class BrickPoolOracle:
+    """The brick pool's allocation policy restated in numpy (allocMap of
+    reaction/voxtree/brickpool/BrickPool.cpp:21-62 enumerates the mapped
+    bricks in request order; UpdateBrickPool of reaction/voxtree/density.cu:
+    240-284 copies packed bricks into their slots). Policy: the listed bricks
+    without a slot, in order of first appearance, get the lowest free slots in
+    increasing order. Arrays: table int32 [(grid/brick)^3], params float32
+    [slots * brick^3, 4]."""
+
+    def __init__(self, table, params, brick):
+        self.table = np.array(table, np.int32)
+        self.params = np.array(params, np.float32).reshape(-1, 4)
+        self.vox = brick ** 3
+        self.cap = self.params.shape[0] // self.vox
+        self.used = np.zeros(self.cap, bool)
+        live = self.table[self.table >= 0]
+        if live.size and (live.max() >= self.cap or np.unique(live).size != live.size):
+            raise ValueError("slot used twice or beyond capacity")
+        self.used[live] = True
+
+    def alloc(self, bricks):
+        bricks = np.asarray(bricks, np.int64)
+        if bricks.size and (bricks.min() < 0 or bricks.max() >= self.table.size):
+            raise ValueError("brick index out of range")
+        _, first = np.unique(bricks, return_index=True)
+        order = np.sort(first)
+        new = [int(bricks[i]) for i in order if self.table[bricks[i]] < 0]
+        free = np.nonzero(~self.used)[0]
+        if len(new) > free.size:
+            raise ValueError("not enough free slots")
+        for b, s in zip(new, free[:len(new)]):
+            self.table[b] = s
+            self.used[s] = True
+        return self.table[bricks].copy(), len(new)
+
+    def release(self, bricks):
+        n = 0
+        for b in np.unique(np.asarray(bricks, np.int64)):
+            s = self.table[b]
+            if s >= 0:
+                self.used[s] = False
+                self.table[b] = -1
+                n += 1
+        return n
+
+    def update(self, bricks, packed):
+        packed = np.asarray(packed, np.float32).reshape(-1, self.vox, 4)
+        for i, b in enumerate(np.asarray(bricks, np.int64)):
+            s = self.table[b]
+            if s >= 0:
+                self.params[s * self.vox:(s + 1) * self.vox] = packed[i]

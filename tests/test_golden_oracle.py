@@ -21,3 +21,19 @@ def test_oracle_matches_fixture(yx, orc):
                        light_mode=ent["light_mode"], light=gold["light"], threads=4)
         assert mg.digest(o) == ent["digest"], f"view {ent['view']} mode {ent['light_mode']}"
         assert o["rays"] == ent["rays"]
+
+
+def test_pipeline_oracle_matches_fixture(yx, oracle_mod):
+    """The committed pipeline fixture (tests/golden/pipeline_256.json) pins the
+    LOD trace, blur and shade restatement on the plain_512 scene."""
+    import golden.make_golden_pipeline as mp
+    gold = mp.load()
+    po = oracle_mod.PipelineOracle()
+    s = yx.SVOData.build_synthetic(**yx.workloads.PLAIN_512)
+    nodes, root = s.nodes(), s.GetRoot()
+    lod = po.lod_colors(nodes, root)
+    for ent in gold["views"]:
+        lights = oracle_mod.make_lights([dict(pos=ent["pos"], **gold["light"])])
+        o = po.pipeline(nodes, root, gold["depth"], ent["pos"], ent["dir"], fov=gold["fov"], w=gold["w"],
+                        h=gold["h"], detail_coef=ent["coef"], lights=lights, lod=lod, threads=4)
+        assert mp.digest(o) == ent["digest"], f"view {ent['view']} coef {ent['coef']}"

@@ -165,3 +165,18 @@ def test_null_outputs_and_repeat(yx, po, cuda):
     c = gpu_pipeline(yx, cuda, r, keys=("z", "level"))
     assert np.array_equal(a["rgba"], b["rgba"])
     assert np.array_equal(a["z"], c["z"]) and np.array_equal(a["level"], c["level"])
+
+
+def test_pipeline_golden_fixture(yx, cuda):
+    """GPU pipeline outputs hash to the committed oracle fixture."""
+    import golden.make_golden_pipeline as mp
+    gold = mp.load()
+    s = yx.SVOData.build_synthetic(**yx.workloads.PLAIN_512)
+    r = yx.SVORenderer()
+    for ent in gold["views"]:
+        lt = {"pos": tuple(ent["pos"]), **{k: tuple(v) for k, v in gold["light"].items()}}
+        setup_renderer(yx, s, ent["pos"], ent["dir"], gold["w"], gold["h"], fov=gold["fov"], coef=ent["coef"],
+                       lights=[lt], renderer=r)
+        g = gpu_pipeline(yx, cuda, r, keys=("rgba", "z", "t", "albedo", "level"))
+        g["zb"] = g.pop("z")
+        assert mp.digest(g) == ent["digest"], f"view {ent['view']} coef {ent['coef']}"

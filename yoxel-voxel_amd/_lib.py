@@ -215,6 +215,12 @@ SIGNATURES = {
     "yx_renderer_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "yx_renderer_autotune": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), ctypes.c_int32, ctypes.POINTER(TuneTrial),
                                             ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _P]),
+    "yx_brick_pool_create": (ctypes.c_int, [ctypes.POINTER(VolumeDev), _P, ctypes.POINTER(_P)]),
+    "yx_brick_pool_destroy": (None, [_P]),
+    "yx_brick_pool_alloc": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_uint32), _P]),
+    "yx_brick_pool_release": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _P]),
+    "yx_brick_pool_update": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, _P]),
+    "yx_brick_pool_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
     "yx_scene_write_nodes": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
     "yx_scene_set_root": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "yx_renderer_update_scene": (ctypes.c_int, [_P, _P, _P]),

@@ -137,4 +137,59 @@ def orbit_views(n, fov, w, h, seed=0):
     return camera_views(orbit_cameras(n, seed=seed), fov, w, h)
 
 
-__all__ = ["Volume", "Compositor", "composite_function", "camera_views", "orbit_views", "COMPOSITE_512"]
+__all__ = ["Volume", "Compositor", "composite_function", "camera_views", "orbit_views", "COMPOSITE_512",
+           "BrickPool"]
+
+
+class BrickPool:
+    """CuBrickPoolManager (reaction/voxtree/brickpool/BrickPool.cpp:21-62) over
+    caller-owned torch tensors: `table` int32 [(grid/brick)^3] and `params`
+    float32 [slots * brick^3, 4] on the GPU. alloc / release / update change
+    them in place (see yx_brick_pool_* in include/yoxel.h)."""
+
+    def __init__(self, table, params, grid, brick):
+        self.table, self.params, self.grid, self.brick = table, params, grid, brick
+        self._dev = _vol_dev(table, params, grid, brick)
+        self._h = ctypes.c_void_p()
+        check(lib().yx_brick_pool_create(ctypes.byref(self._dev), _stream(), ctypes.byref(self._h)))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _lib is not None and _lib._lib is not None:
+            _lib._lib.yx_brick_pool_destroy(h)
+            self._h = ctypes.c_void_p()
+
+    def alloc(self, bricks):
+        """Slots for the listed bricks (int32 device tensor of linear brick
+        indices); returns (slot of every listed brick, number of new slots)."""
+        import torch
+        b = bricks.to(torch.int32).contiguous()
+        slots = torch.empty_like(b)
+        n = ctypes.c_uint32()
+        check(lib().yx_brick_pool_alloc(self._h, ctypes.c_void_p(b.data_ptr()), b.numel(),
+                                        ctypes.c_void_p(slots.data_ptr()), ctypes.byref(n), _stream()))
+        return slots, n.value
+
+    def release(self, bricks):
+        import torch
+        b = bricks.to(torch.int32).contiguous()
+        n = ctypes.c_uint32()
+        check(lib().yx_brick_pool_release(self._h, ctypes.c_void_p(b.data_ptr()), b.numel(), ctypes.byref(n),
+                                          _stream()))
+        return n.value
+
+    def update(self, bricks, packed):
+        """UpdateBrickPool: packed float32 [n * brick^3, 4] into the listed bricks' slots."""
+        import torch
+        b = bricks.to(torch.int32).contiguous()
+        pk = packed.to(torch.float32).contiguous()
+        if pk.numel() != b.numel() * self.brick ** 3 * 4:
+            raise ValueError("packed must hold brick^3 float4 per listed brick")
+        check(lib().yx_brick_pool_update(self._h, ctypes.c_void_p(b.data_ptr()), b.numel(),
+                                         ctypes.c_void_p(pk.data_ptr()), _stream()))
+
+    def stats(self):
+        u = ctypes.c_uint32()
+        c = ctypes.c_uint32()
+        check(lib().yx_brick_pool_stats(self._h, ctypes.byref(u), ctypes.byref(c)))
+        return u.value, c.value
