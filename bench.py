@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--no-pipeline", action="store_true",
                     help="skip the shading-pipeline measurement (LOD trace, SSNA blur, Phong shade)")
     ap.add_argument("--no-update", action="store_true", help="skip the scene-sync measurement")
-    ap.add_argument("--only", choices=["composite", "batch", "brick", "plain", "pipeline", "update"], default=None,
+    ap.add_argument("--only", choices=["composite", "batch", "brick", "plain", "pipeline", "update", "pool"],
+                    default=None,
                     help="run one secondary workload alone and print its object (profiling)")
     ap.add_argument("--brick-frames", type=int, default=8)
     ap.add_argument("--autotune", action="store_true",
@@ -78,6 +79,8 @@ def main():
             leg = plain_bench(yx, torch, not args.no_cpu_baseline)
         elif args.only == "batch":
             leg = batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
+        elif args.only == "pool":
+            leg = pool_bench(yx, torch, dev)
         elif args.only == "update":
             leg = update_bench(yx, torch, yx.SVOData.build_synthetic_gpu(**wl.LARGE_1080P))
         elif args.only == "pipeline":
@@ -184,6 +187,7 @@ def main():
     pipeline = (None if args.no_pipeline or args.workload != "large_1080p"
                 else pipeline_bench(yx, torch, scene, cams, not args.no_cpu_baseline))
     update = None if args.no_update or args.workload != "large_1080p" else update_bench(yx, torch, scene)
+    pool = None if args.no_composite else pool_bench(yx, torch, dev)
 
     line = {
         "metric": BASELINE_METRIC,
@@ -216,6 +220,7 @@ def main():
         "brick": brick,
         "pipeline": pipeline,
         "update": update,
+        "brick_pool": pool,
     }
     print(json.dumps(line), flush=True)
 
@@ -494,6 +499,50 @@ def update_bench(yx, torch, scene, reps=3):
             "update_edit_ms": round(min(edit_ms), 2), "update_edit_pages": edit_pages,
             "update_edit_kb": round(edit_bytes / 1e3, 1),
             "speedup_edit_vs_set_scene": round(host_ms / min(edit_ms), 1)}
+
+
+def pool_bench(yx, torch, dev, reps=5):
+    """Brick pool churn on the composite_512 volume (26 k of 32 768 bricks
+    allocated, 8^3 float4 per brick): per round, release 10 % of the bricks,
+    allocate as many others and upload their packed contents
+    (CuBrickPoolManager::allocMap + UpdateBrickPool), against repacking the
+    whole volume on the host and uploading it. Wall time, synchronised."""
+    comp_mod = yx.composite
+    cfg = comp_mod.COMPOSITE_512
+    grid, brick = cfg["grid"], cfg["brick"]
+    vox = brick ** 3
+    vol = comp_mod.Volume(**cfg)
+    table, params = vol.to(dev)
+    spare = vol.num_slots // 5
+    params = torch.cat([params, torch.zeros((spare * vox, 4), dtype=torch.float32, device=dev)])
+    pool = comp_mod.BrickPool(table, params, grid, brick)
+    rng = np.random.default_rng(2)
+    live = np.nonzero(vol.table >= 0)[0].astype(np.int32)
+    empty = np.nonzero(vol.table < 0)[0].astype(np.int32)
+    k = live.size // 10
+    packed = torch.from_numpy(rng.standard_normal((k * vox, 4)).astype(np.float32)).to(dev)
+    times = []
+    for r in range(reps):
+        drop = torch.from_numpy(rng.choice(live, k, replace=False)).to(dev)
+        add = torch.from_numpy(rng.choice(empty, k, replace=False)).to(dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pool.release(drop)
+        pool.alloc(add)
+        pool.update(add, packed)
+        torch.cuda.synchronize()
+        times.append((time.perf_counter() - t0) * 1e3)
+        pool.release(add)  # back to the original occupancy (timing excluded)
+        pool.alloc(drop)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v2 = comp_mod.Volume(**cfg)
+    t2, p2 = v2.to(dev)
+    torch.cuda.synchronize()
+    rebuild_ms = (time.perf_counter() - t0) * 1e3
+    return {"workload": "composite_512 brick pool churn", "bricks": int(live.size), "churn_bricks": int(k),
+            "churn_ms": round(float(np.min(times)), 3), "churn_upload_mb": round(k * vox * 16 / 1e6, 1),
+            "host_rebuild_ms": round(rebuild_ms, 1), "speedup_vs_rebuild": round(rebuild_ms / np.min(times), 1)}
 
 
 def batch_bench(yx, torch, frames, tune_cache):
