@@ -451,7 +451,7 @@ def pipeline_bench(yx, torch, scene, cams, with_cpu, reps=2, cpu_frames=4):
     return res
 
 
-def update_bench(yx, torch, scene, reps=3):
+def update_bench(yx, torch, scene, reps=3, with_edit=True):
     """Scene sync on large_1080p (14.2 M VoxNodes, 46.4 M records):
     SetScene (host breadth-first re-layout + LOD colours + upload of the
     records) against UpdateScene (CudaSVO::Update, demo/SVORenderer.cpp:39-53:
@@ -491,6 +491,7 @@ def update_bench(yx, torch, scene, reps=3):
     for i, row in zip(idx, orig):
         scene.write_nodes(i, row)
     r_gpu.UpdateScene(scene)
+    edit = edit_bench(yx, torch, wall, host_ms) if with_edit else None
     return {"workload": "large_1080p scene sync", "nodes": int(nodes.shape[0]),
             "set_scene_host_ms": round(host_ms, 1),
             "update_full_ms": round(full_ms, 1), "update_full_pages": full_pages,
@@ -498,7 +499,58 @@ def update_bench(yx, torch, scene, reps=3):
             "update_unchanged_ms": round(nop_ms, 2),
             "update_edit_ms": round(min(edit_ms), 2), "update_edit_pages": edit_pages,
             "update_edit_kb": round(edit_bytes / 1e3, 1),
-            "speedup_edit_vs_set_scene": round(host_ms / min(edit_ms), 1)}
+            "speedup_edit_vs_set_scene": round(host_ms / min(edit_ms), 1), "sphere_edits": edit}
+
+
+def edit_bench(yx, torch, wall, set_scene_ms, reps=3):
+    """Demo::DoEdit on large_1080p (demo/Demo.cpp:82-114): 100 sphere edits of
+    radius 4 at level 11, centred on voxels that primary rays of an orbit view
+    hit, applied with BuildRange in place on the host, then UpdateScene
+    (CudaSVO::Update). Alternate batches GROW and CLEAR. Against: the same
+    batch applied by rebuilding the scene (on the host, or on the GPU) and
+    SetScene's full host re-layout. Wall time, synchronised."""
+    wl = yx.workloads
+    s = yx.SVOData.build_synthetic_gpu(**wl.LARGE_1080P)
+    r = yx.SVORenderer()
+    r.UpdateScene(s)
+    pos, d = wl.orbit_cameras(1, seed=0)[0]
+    r.SetViewPos(pos)
+    r.SetViewDir(d)
+    r.SetFOV(60.0)
+    r.SetResolution(480, 270)
+    out = r.render_tensors(("hit_id", "voxel"))
+    torch.cuda.synchronize()
+    vox = out["voxel"].reshape(-1, 3).cpu().numpy()
+    hit = out["hit_id"].reshape(-1).cpu().numpy() != -1
+    rng = np.random.default_rng(4)
+    cand = vox[hit]
+    batches = []
+    for k in range(reps + 1):
+        pick = cand[rng.choice(cand.shape[0], 100, replace=False)]
+        mode = k % 2
+        batches.append([(mode, 11, tuple(float(v) + 0.5 for v in p), 4.0, 0xFF80C0FF) for p in pick])
+    edit_ms, upd_ms, pages = [], [], []
+    for k in range(reps):
+        edit_ms.append(wall(lambda: s.BuildRanges(batches[k])))
+        upd_ms.append(wall(lambda: r.UpdateScene(s)))
+        pages.append(r.last_update()[0])
+    b = batches[reps]
+    gpu_rebuild_ms = wall(lambda: s.BuildRanges(b, how=yx.SVOData.REBUILD_GPU))
+    gpu_update_ms = wall(lambda: r.UpdateScene(s))
+    gpu_pages = r.last_update()[0]
+    cs = yx.SVOData.build_synthetic(**wl.LARGE_1080P)
+    cpu_rebuild_ms = wall(lambda: cs.BuildRanges(b, how=yx.SVOData.REBUILD))
+    k = int(np.argmin([a + u for a, u in zip(edit_ms, upd_ms)]))
+    total = edit_ms[k] + upd_ms[k]
+    return {"workload": "large_1080p, 100 sphere edits (radius 4, level 11) per batch",
+            "in_place_edit_ms": round(edit_ms[k], 2), "update_ms": round(upd_ms[k], 2),
+            "update_pages": pages[k], "in_place_total_ms": round(total, 2),
+            "gpu_rebuild_ms": round(gpu_rebuild_ms, 1), "gpu_rebuild_update_ms": round(gpu_update_ms, 1),
+            "gpu_rebuild_update_pages": gpu_pages,
+            "cpu_rebuild_ms": round(cpu_rebuild_ms, 1),
+            "cpu_rebuild_set_scene_ms": round(cpu_rebuild_ms + set_scene_ms, 1),
+            "speedup_vs_cpu_rebuild_set_scene": round((cpu_rebuild_ms + set_scene_ms) / total, 1),
+            "nodes": s.info()["num_nodes"]}
 
 
 def pool_bench(yx, torch, dev, reps=5):

@@ -102,6 +102,40 @@ int yx_scene_get_info(const yx_scene* scene, yx_scene_info* info);
  * Invalidates pointers from yx_scene_nodes. */
 int yx_scene_write_nodes(yx_scene* scene, uint32_t first, const uint32_t* nodes, uint32_t count);
 int yx_scene_set_root(yx_scene* scene, uint32_t root);
+
+/* DynamicSVO::BuildRange(level, pos, mode, SphereSource) (ore/src/main.cpp:
+ * 66-70 and 121-127 bind it; demo/Demo.cpp:82-114 and qtview.py:144-153 call
+ * it) on a synthetic scene. The scene keeps its recipe and edit list; the call
+ * appends `count` edits. The solid is the recipe's, with each edit applied in
+ * order: GROW unions the ball of voxel centres within `radius` of `pos`, CLEAR
+ * subtracts it; leaves are its surface voxels as for yx_scene_build_synthetic.
+ * A leaf's colour is that of the last edit whose ball contains it when that
+ * edit is a GROW, else the recipe's colour. `how`:
+ *   YX_RANGE_IN_PLACE     edit the node array in place on the host, one edit
+ *                         at a time: nodes outside an edit's ball (plus one
+ *                         voxel) keep their ids and bytes, emptied nodes are
+ *                         blanked (flags 0xFF00, no children) and their ids
+ *                         reused lowest first before the array grows;
+ *   YX_RANGE_REBUILD      rebuild the canonical breadth-first tree on the host;
+ *   YX_RANGE_REBUILD_GPU  the same tree, built on the GPU on `stream`.
+ * Every method gives the same voxels and colours; the two rebuilds give the
+ * same bytes. Pages whose bytes changed get a new version. YX_ERR_INVALID_ARG
+ * for scenes without a recipe (loaded, or edited with yx_scene_write_nodes),
+ * with nothing changed. */
+#define YX_BUILD_GROW  0
+#define YX_BUILD_CLEAR 1
+#define YX_RANGE_IN_PLACE    0
+#define YX_RANGE_REBUILD     1
+#define YX_RANGE_REBUILD_GPU 2
+typedef struct yx_sphere_edit {
+    int32_t mode;   /* YX_BUILD_GROW or YX_BUILD_CLEAR                        */
+    int32_t level;  /* grid level of pos and radius, 1..depth (BuildRange's)  */
+    float pos[3];   /* centre, in voxels of that level                         */
+    float radius;   /* in voxels of that level, > 0                            */
+    uint32_t color; /* RGBA8 of grown voxels (SphereSource's colour)           */
+} yx_sphere_edit;
+int yx_scene_build_range(yx_scene* scene, const yx_sphere_edit* edits, uint32_t count, int32_t how,
+                         void* stream);
 /* SVOData::GetRoot / operator[]: borrowed view of the node array (valid until
  * yx_scene_free). */
 int yx_scene_nodes(const yx_scene* scene, const uint32_t** nodes, uint32_t* num_nodes,

@@ -256,3 +256,59 @@ def morton3(x, y, z):
 
 def leaf_color(code, seed):
     return _mix64(int(code) ^ ((seed * 0x9E3779B97F4A7C15) & MASK64)) & 0xFFFFFFFF
+
+
+def edit_balls(depth, edits):
+    """Sphere edits (mode, level, pos, radius, color) in finest-level voxels:
+    pos and radius scale by 2^(depth - level) (float32 inputs, float64 maths)."""
+    out = []
+    for mode, level, pos, radius, color in edits:
+        sc = float(2 ** (depth - level))
+        c = [float(np.float32(p)) * sc for p in pos]
+        out.append((mode, c, float(np.float32(radius)) * sc, color))
+    return out
+
+
+def edited_inside(desc, edits):
+    """Dense inside grid after BuildRange edits in order: GROW (mode 0) is a
+    union with the ball, CLEAR (mode 1) a difference. A voxel is in a ball when
+    its centre is within the radius."""
+    inside = synthetic_inside(**desc)
+    n = inside.shape[0]
+    c = np.arange(n, dtype=np.float64) + 0.5
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    for mode, (cx, cy, cz), r, _ in edit_balls(desc["depth"], edits):
+        ball = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2 <= r * r
+        inside = inside | ball if mode == 0 else inside & ~ball
+    return inside
+
+
+def edited_color(x, y, z, desc, edits):
+    """Leaf colour after edits: the last ball holding the voxel decides (its
+    colour for a GROW, the recipe's hash for a CLEAR); else the hash."""
+    p = (x + 0.5, y + 0.5, z + 0.5)
+    for mode, c, r, color in reversed(edit_balls(desc["depth"], edits)):
+        if sum((p[a] - c[a]) ** 2 for a in range(3)) <= r * r:
+            return color if mode == 0 else leaf_color(int(morton3(x, y, z)), desc["seed"])
+    return leaf_color(int(morton3(x, y, z)), desc["seed"])
+
+
+def leaf_colors_np(codes, seed):
+    """leaf_color over an array of Morton codes (uint64 arithmetic wraps)."""
+    z = np.asarray(codes, np.uint64) ^ np.uint64((seed * 0x9E3779B97F4A7C15) & MASK64)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return ((z ^ (z >> np.uint64(31))) & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+
+
+def edited_colors_np(xs, ys, zs, desc, edits):
+    """edited_color over arrays of voxels: the last ball holding a voxel decides."""
+    out = leaf_colors_np(morton3(xs, ys, zs), desc["seed"])
+    p = [np.asarray(v, np.float64) + 0.5 for v in (xs, ys, zs)]
+    for mode, c, r, color in edit_balls(desc["depth"], edits):
+        inb = (p[0] - c[0]) ** 2 + (p[1] - c[1]) ** 2 + (p[2] - c[2]) ** 2 <= r * r
+        if mode == 0:
+            out[inb] = np.uint32(color)
+        else:
+            out[inb] = leaf_colors_np(morton3(xs[inb], ys[inb], zs[inb]), desc["seed"])
+    return out

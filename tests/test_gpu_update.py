@@ -144,3 +144,33 @@ def test_switching_scenes_uploads_everything(yx, cuda):
     assert r.last_update()[0] == -(-b.info()["num_nodes"] // 256)
     r = same_layout(yx, a, r, "a again")
     assert r.last_update()[0] == -(-a.info()["num_nodes"] // 256)
+
+
+def test_scattered_pages_in_one_staged_copy(yx, cuda):
+    """Many scattered dirty pages go through one pinned staging copy and a
+    scatter kernel; the device copy must equal a full upload."""
+    s = yx.SVOData.build_synthetic(depth=8, num_spheres=20, radius_min=4.0, radius_max=20.0, heightfield=True,
+                                   seed=5)
+    r = same_layout(yx, s, what="initial")
+    nodes = s.nodes()
+    n = nodes.shape[0]
+    cand = np.nonzero(nodes[:, 0] & 0xFF)[0]
+    rng = np.random.default_rng(12)
+    for rnd in range(3):
+        pages = sorted(rng.choice(-(-n // 256), 60, replace=False))
+        edited = set()
+        for p in pages:
+            in_page = cand[(cand // 256) == p]
+            if in_page.size == 0:
+                continue
+            i = int(in_page[rng.integers(in_page.size)])
+            row = s.nodes()[i].copy()
+            for sl in range(8):
+                if (int(row[0]) >> sl) & 1:
+                    row[2 + sl] = np.uint32(0xFF000000 | (rnd * 0x10203 + p))
+            s.write_nodes(i, row)
+            edited.add(p)
+        assert len(edited) >= 40
+        assert r.CountChangedPages(s)[0] == len(edited)
+        same_layout(yx, s, r, f"scattered round {rnd}")
+        assert r.last_update()[0] == len(edited)

@@ -66,3 +66,194 @@ def test_pages_to_upload_without_sync(yx):
     n = s.info()["num_nodes"]
     pages, nbytes = r.CountChangedPages(s)
     assert pages == -(-n // 256) and nbytes == 40 * n
+
+
+DESC = dict(depth=6, num_spheres=6, radius_min=3.0, radius_max=9.0, heightfield=True, hf_octaves=3,
+            hf_amplitude=0.2, hf_base=0.25, seed=11)
+
+# (mode, level, pos, radius, color): GROW 0 / CLEAR 1, pos and radius in voxels of `level`
+EDITS = [
+    (0, 6, (20.0, 30.5, 40.25), 9.5, 0xFF0000FF),   # grow a ball in the air
+    (1, 6, (12.0, 12.0, 8.0), 7.0, 0),              # carve into the ground
+    (0, 4, (2.0, 10.0, 5.5), 3.25, 0xFF00FF00),     # a coarser level: scales by 4
+    (0, 6, (62.0, 1.0, 63.0), 6.0, 0xFFFF0000),     # crosses the domain's edges
+    (1, 5, (10.5, 15.0, 20.0), 4.0, 0),             # clears part of the first ball
+    (0, 6, (20.0, 30.5, 40.25), 3.0, 0xFF123456),   # regrows inside the cleared part
+    (1, 6, (-40.0, 90.0, 10.0), 5.0, 0),            # entirely outside: no change
+]
+
+
+def check_edited(yx, s, desc, edits):
+    nodes, root = s.nodes(), s.GetRoot()
+    got = H.walk_leaves(nodes, root)
+    surf = H.surface(H.edited_inside(desc, edits))
+    xs, ys, zs = np.nonzero(surf)
+    depth = desc["depth"]
+    exp = {(depth, int(x), int(y), int(z)) for x, y, z in zip(xs, ys, zs)}
+    assert set(got) == exp
+    cols = H.edited_colors_np(xs, ys, zs, desc, edits)
+    assert [got[(depth, int(x), int(y), int(z))][0] for x, y, z in zip(xs, ys, zs)] == cols.tolist()
+    for x, y, z, c in list(zip(xs, ys, zs, cols))[::97]:  # the scalar restatement agrees
+        assert H.edited_color(int(x), int(y), int(z), desc, edits) == c
+    assert s.info()["num_leaves"] == len(exp)
+    check_pool(nodes, root)
+
+
+def leaf_colors(s):
+    return {k: v[0] for k, v in H.walk_leaves(s.nodes(), s.GetRoot()).items()}
+
+
+def check_pool(nodes, root):
+    """Reachable nodes are a tree whose null flags match their slots; every
+    other row is blank (flags 0xFF00, no children)."""
+    seen = np.zeros(nodes.shape[0], bool)
+    stack = [] if root == H.EMPTY else [int(root)]
+    while stack:
+        i = stack.pop()
+        assert not seen[i]
+        seen[i] = True
+        f = int(nodes[i, 0])
+        assert f & 0xFF or (f >> 8) & 0xFF != 0xFF  # no empty node is reachable
+        for c in range(8):
+            ch = int(nodes[i, 2 + c])
+            if (f >> c) & 1:
+                assert not (f >> (8 + c)) & 1
+            else:
+                assert bool((f >> (8 + c)) & 1) == (ch == H.EMPTY)
+                if ch != H.EMPTY:
+                    stack.append(ch)
+    blank = ~seen
+    assert np.all(nodes[blank, 0] == 0xFF00) and np.all(nodes[blank, 2:] == H.EMPTY)
+
+
+@pytest.mark.parametrize("how", ["IN_PLACE", "REBUILD"])
+def test_build_range_matches_dense_definition(yx, how):
+    """DynamicSVO::BuildRange(level, pos, BuildMode, SphereSource) (ore/src/main.cpp:121-127,
+    demo/Demo.cpp:82-114): every prefix of an edit sequence equals the dense
+    definition: the recipe's inside set, balls united (GROW) or removed (CLEAR)
+    in order, surface voxels, and the last ball holding a voxel giving its colour."""
+    how = getattr(yx.SVOData, how)
+    s = yx.SVOData.build_synthetic(**DESC)
+    for k in range(len(EDITS)):
+        mode, level, pos, radius, color = EDITS[k]
+        s.BuildRange(level, pos, mode, radius, color, how=how)
+        check_edited(yx, s, DESC, EDITS[:k + 1])
+    # a fresh scene given the same edits in one call ends identical
+    t = yx.SVOData.build_synthetic(**DESC)
+    t.BuildRanges(EDITS, how=how)
+    assert np.array_equal(s.nodes(), t.nodes()) and s.GetRoot() == t.GetRoot()
+    # the in-place tree holds the rebuilt tree's voxels and colours
+    r = yx.SVOData.build_synthetic(**DESC)
+    r.BuildRanges(EDITS, how=yx.SVOData.REBUILD)
+    assert leaf_colors(s) == leaf_colors(r)
+
+
+def node_cubes(nodes, root, depth):
+    """{node id: (level, origin voxel)} of the nodes reachable from root."""
+    out = {}
+    stack = [(int(root), 0, (0, 0, 0))]
+    while stack:
+        i, lvl, o = stack.pop()
+        out[i] = (lvl, o)
+        size = 1 << (depth - lvl - 1)
+        f = int(nodes[i, 0])
+        for c in range(8):
+            if not (f >> c) & 1 and nodes[i, 2 + c] != H.EMPTY:
+                stack.append((int(nodes[i, 2 + c]), lvl + 1,
+                              (o[0] + (c & 1) * size, o[1] + ((c >> 1) & 1) * size, o[2] + ((c >> 2) & 1) * size)))
+    return out
+
+
+def test_in_place_edit_touches_only_its_region(yx):
+    desc = dict(depth=8, num_spheres=30, radius_min=4.0, radius_max=24.0, heightfield=True, hf_octaves=4,
+                hf_amplitude=0.2, hf_base=0.3, seed=5)
+    s = yx.SVOData.build_synthetic(**desc)
+    rng = np.random.default_rng(3)
+    for k in range(6):
+        before, root0 = s.nodes(), s.GetRoot()
+        cubes = node_cubes(before, root0, 8)
+        mode = k % 2
+        c = rng.uniform(20, 236, 3)
+        rad = float(rng.uniform(2, 9))
+        s.BuildRange(8, tuple(c), mode, rad, 0xFF00A0F0)
+        after = s.nodes()
+        lo, hi = np.floor(c - rad) - 3, np.ceil(c + rad) + 3
+        changed = 0
+        for i, (lvl, o) in cubes.items():
+            size = 1 << (8 - lvl)
+            meets = all(o[a] <= hi[a] and o[a] + size - 1 >= lo[a] for a in range(3))
+            if not meets:
+                assert np.array_equal(before[i], after[i]), (k, i)
+            elif i < after.shape[0] and not np.array_equal(before[i], after[i]):
+                changed += 1
+        assert changed < 80, changed
+        check_pool(after, s.GetRoot())
+    r = yx.SVOData.build_synthetic(**desc)
+    rng = np.random.default_rng(3)
+    edits = []
+    for k in range(6):
+        c = rng.uniform(20, 236, 3)
+        rad = float(rng.uniform(2, 9))
+        edits.append((k % 2, 8, tuple(c), rad, 0xFF00A0F0))
+    r.BuildRanges(edits, how=yx.SVOData.REBUILD)
+    assert leaf_colors(s) == leaf_colors(r)
+
+
+def test_in_place_reuses_freed_nodes(yx):
+    s = yx.SVOData.build_synthetic(**DESC)
+    n0 = s.info()["num_nodes"]
+    sizes = []
+    for _ in range(4):
+        s.BuildRange(6, (40.0, 40.0, 50.0), 0, 8.0, 0xFF0000FF)   # grow a ball in the air
+        sizes.append(s.info()["num_nodes"])
+        s.BuildRange(6, (40.0, 40.0, 50.0), 1, 9.0)               # and clear it again
+    assert sizes[0] > n0 and sizes[1:] == sizes[:1] * 3
+    check_pool(s.nodes(), s.GetRoot())
+
+
+def test_build_range_on_empty_and_cleared_scenes(yx):
+    desc = dict(depth=5, num_spheres=0, radius_min=0.0, radius_max=0.0, heightfield=False, hf_octaves=5,
+                hf_amplitude=0.2, hf_base=0.3, seed=4)
+    for how in (yx.SVOData.IN_PLACE, yx.SVOData.REBUILD):
+        s = yx.SVOData.build_synthetic(**desc)
+        assert s.GetRoot() == H.EMPTY
+        edits = [(0, 5, (16.0, 16.0, 16.0), 6.0, 0xFF804020)]
+        s.BuildRange(5, (16.0, 16.0, 16.0), 0, 6.0, 0xFF804020, how=how)
+        check_edited(yx, s, desc, edits)
+        # clearing everything leaves an empty tree
+        s.BuildRange(1, (1.0, 1.0, 1.0), 1, 2.0, how=how)
+        assert s.GetRoot() == H.EMPTY and s.info()["num_leaves"] == 0
+        # a single voxel
+        s.BuildRange(5, (3.5, 4.5, 5.5), 0, 0.25, 0xFF010203, how=how)
+        got = H.walk_leaves(s.nodes(), s.GetRoot())
+        assert got == {(5, 3, 4, 5): got[(5, 3, 4, 5)]} and got[(5, 3, 4, 5)][0] == 0xFF010203
+        check_pool(s.nodes(), s.GetRoot())
+
+
+def test_build_range_errors(yx):
+    s = yx.SVOData.build_synthetic(**DESC)
+    before = s.nodes()
+    for args in ((0, (1.0, 1.0, 1.0), 0, 1.0),     # level 0
+                 (7, (1.0, 1.0, 1.0), 0, 1.0),     # deeper than the tree
+                 (6, (1.0, 1.0, 1.0), 2, 1.0),     # unknown mode
+                 (6, (1.0, 1.0, 1.0), 0, 0.0),     # radius 0
+                 (6, (1.0, 1.0, 1.0), 0, -2.0),
+                 (6, (float("nan"), 1.0, 1.0), 0, 1.0),
+                 (6, (1.0, 1.0, 1.0), 1, float("inf")),
+                 (6, (1.0, 3e7, 1.0), 0, 1.0),     # beyond 2^24 voxels
+                 (1, (1.0, 1.0, 1.0), 0, 1e6)):
+        with pytest.raises(yx.YoxelError):
+            s.BuildRange(*args)
+    with pytest.raises(yx.YoxelError):
+        s.BuildRange(6, (1.0, 1.0, 1.0), 0, 1.0, how=5)   # unknown method
+    # a bad edit in a batch rejects the whole batch
+    with pytest.raises(yx.YoxelError):
+        s.BuildRanges([(0, 6, (10.0, 10.0, 10.0), 3.0, 0xFF0000FF), (0, 9, (1.0, 1.0, 1.0), 1.0, 0)])
+    assert np.array_equal(s.nodes(), before)
+    # the recipe is gone once nodes were written by hand, or for a scene from nodes
+    s.write_nodes(0, before[0])
+    with pytest.raises(yx.YoxelError):
+        s.BuildRange(6, (1.0, 1.0, 1.0), 0, 1.0)
+    t = yx.SVOData.from_nodes(before, 0)
+    with pytest.raises(yx.YoxelError):
+        t.BuildRange(6, (1.0, 1.0, 1.0), 0, 1.0)

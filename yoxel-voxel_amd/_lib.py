@@ -99,6 +99,22 @@ class TuneTrial(ctypes.Structure):
 
 
 YX_MAX_LIGHTS = 4
+YX_BUILD_GROW = 0
+YX_BUILD_CLEAR = 1
+YX_RANGE_IN_PLACE = 0
+YX_RANGE_REBUILD = 1
+YX_RANGE_REBUILD_GPU = 2
+
+
+class SphereEdit(ctypes.Structure):
+    """yx_sphere_edit (BuildRange with a SphereSource)."""
+    _fields_ = [
+        ("mode", ctypes.c_int32),
+        ("level", ctypes.c_int32),
+        ("pos", ctypes.c_float * 3),
+        ("radius", ctypes.c_float),
+        ("color", ctypes.c_uint32),
+    ]
 
 
 class LightParams(ctypes.Structure):
@@ -221,6 +237,7 @@ SIGNATURES = {
     "yx_brick_pool_release": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _P]),
     "yx_brick_pool_update": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, _P]),
     "yx_brick_pool_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+    "yx_scene_build_range": (ctypes.c_int, [_P, ctypes.POINTER(SphereEdit), ctypes.c_uint32, ctypes.c_int32, _P]),
     "yx_scene_write_nodes": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
     "yx_scene_set_root": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "yx_renderer_update_scene": (ctypes.c_int, [_P, _P, _P]),

@@ -27,6 +27,13 @@ struct yx_scene {
     uint64_t version = 1;                  // 1 at creation, +1 per edit
     std::vector<uint64_t> page_version;    // per page; pages past the end: 1
     uint64_t page_ver(size_t p) const { return p < page_version.size() ? page_version[p] : 1u; }
+    // synthetic scenes: the recipe and the sphere edits applied so far
+    // (rebuilt by yx_scene_build_range); cleared by node writes
+    bool has_recipe = false;
+    yx_scene_desc recipe{};
+    std::vector<yx_sphere_edit> edits;
+    std::vector<uint32_t> free_nodes;      // blanked node ids, a min-heap (in-place edits)
+    std::vector<int32_t> hf_top;           // heightfield top per column, filled on demand
 };
 
 namespace yx {
@@ -34,6 +41,39 @@ namespace yx {
 // Deepest supported leaf level (root children are level 1). Bounds the
 // per-ray traversal stack.
 constexpr int kMaxDepth = 16;
+
+// A sphere edit in voxel units of the finest level (BuildRange with a
+// SphereSource): GROW unions the ball of voxel centres within r of the centre,
+// CLEAR subtracts it; `color` paints grown voxels.
+struct EditSphere {
+    int mode;
+    double cx, cy, cz, r;
+    uint32_t color;
+    int xlo, xhi, ylo, yhi;  // columns the edit can touch
+};
+// Edits in order and, per cell of 64 x 64 columns of the (x, y) plane, the ids
+// of the edits whose column range meets the cell, ascending (CSR form). A
+// column or voxel only ever needs its cell's edits.
+struct EditSet {
+    static constexpr int kCellShift = 6;
+    std::vector<EditSphere> edits;
+    int cells = 0;                   // cells per axis
+    std::vector<uint32_t> cell_off;  // cells * cells + 1
+    std::vector<uint32_t> cell_ids;
+    void index(int n);
+    bool empty() const { return edits.empty(); }
+    const uint32_t* begin(int i, int j) const {
+        return cell_ids.data() + cell_off[(size_t)(j >> kCellShift) * cells + (size_t)(i >> kCellShift)];
+    }
+    const uint32_t* end(int i, int j) const {
+        return cell_ids.data() + cell_off[(size_t)(j >> kCellShift) * cells + (size_t)(i >> kCellShift) + 1];
+    }
+};
+// Converts edits to the finest level of a depth-D grid and indexes them;
+// YX_ERR_INVALID_ARG for a bad mode, level or radius.
+int edit_spheres(const yx_sphere_edit* e, size_t n, int depth, EditSet& out);
+// The synthetic scene of a recipe with edits, built on the GPU (svo_build.hip).
+int gpu_build_synthetic(const yx_scene_desc& d, const EditSet& edits, yx_scene& out, void* stream);
 
 int fail(int code, const char* fmt, ...);
 void clear_error();
@@ -132,7 +172,7 @@ inline uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
 // Builds the canonical breadth-first VoxNode tree from sorted unique leaf
 // Morton codes at `depth`; payload(code) gives the leaf payload.
 void build_tree_from_codes(const std::vector<uint64_t>& codes, int depth, uint64_t color_seed,
-                           yx_scene& out);
+                           yx_scene& out, const EditSet* edits = nullptr);
 
 // Leaf colour of a synthetic scene: a hash of the Morton code, RGBA8.
 inline uint32_t leaf_color(uint64_t code, uint64_t seed) {

@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
+#include <stdexcept>
 #include <thread>
 
 #include "internal.h"
@@ -163,8 +165,42 @@ double Perlin::fbm3(double x, double y, double z, int octaves) const {
 
 // ---- tree from sorted leaf codes ------------------------------------------
 
+static inline uint32_t compact3(uint64_t v) {
+    v &= 0x1249249249249249ull;
+    v = (v ^ (v >> 2)) & 0x10c30c30c30c30c3ull;
+    v = (v ^ (v >> 4)) & 0x100f00f00f00f00full;
+    v = (v ^ (v >> 8)) & 0x1f0000ff0000ffull;
+    v = (v ^ (v >> 16)) & 0x1f00000000ffffull;
+    v = (v ^ (v >> 32)) & 0x1fffffull;
+    return (uint32_t)v;
+}
+
+static inline bool in_ball(const EditSphere& e, uint32_t i, uint32_t j, uint32_t k) {
+    double dx = ((double)i + 0.5) - e.cx;
+    double dy = ((double)j + 0.5) - e.cy;
+    double dz = ((double)k + 0.5) - e.cz;
+    return dx * dx + dy * dy + dz * dz <= e.r * e.r;
+}
+
+// Leaf colour: the last edit whose ball holds the voxel decides (a GROW's
+// colour, or after a CLEAR the recipe's); without one, the recipe's hash.
+// Edits with ids >= limit are not applied yet.
+static inline uint32_t edited_color(uint64_t code, uint64_t seed, const EditSet* edits, size_t limit) {
+    if (edits && !edits->empty()) {
+        const uint32_t i = compact3(code), j = compact3(code >> 1), k = compact3(code >> 2);
+        const uint32_t* b = edits->begin((int)i, (int)j);
+        for (const uint32_t* p = edits->end((int)i, (int)j); p-- != b;) {
+            if (*p >= limit) continue;
+            const EditSphere& e = edits->edits[*p];
+            if ((int)i < e.xlo || (int)i > e.xhi || (int)j < e.ylo || (int)j > e.yhi) continue;
+            if (in_ball(e, i, j, k)) return e.mode == YX_BUILD_GROW ? e.color : leaf_color(code, seed);
+        }
+    }
+    return leaf_color(code, seed);
+}
+
 void build_tree_from_codes(const std::vector<uint64_t>& codes, int depth, uint64_t color_seed,
-                           yx_scene& out) {
+                           yx_scene& out, const EditSet* edits) {
     out.nodes.clear();
     out.num_leaves = codes.size();
     out.depth = codes.empty() ? 0 : depth;
@@ -193,6 +229,8 @@ void build_tree_from_codes(const std::vector<uint64_t>& codes, int depth, uint64
     std::vector<uint64_t> base(depth + 1, 0);
     for (int L = 0; L < depth; ++L) base[L + 1] = base[L] + lv[L].size();
     const uint64_t total = base[depth];
+    // headroom for nodes that in-place edits append
+    out.nodes.reserve((total + total / 16 + 1024) * 10);
     out.nodes.assign(total * 10, 0);
     out.root = 0;
     // internal levels
@@ -235,7 +273,7 @@ void build_tree_from_codes(const std::vector<uint64_t>& codes, int depth, uint64
                 for (int s = 0; s < 8; ++s) n[2 + s] = YX_EMPTY_NODE;
                 while (j < codes.size() && (codes[j] >> 3) == cur[k]) {
                     int s = (int)(codes[j] & 7);
-                    n[2 + s] = leaf_color(codes[j], color_seed);
+                    n[2 + s] = edited_color(codes[j], color_seed, edits, edits ? edits->edits.size() : 0);
                     flags |= 1u << s;
                     ++j;
                 }
@@ -326,11 +364,138 @@ void intersect(std::vector<Interval>& a, const Interval* b, size_t nb) {
     a.swap(r);
 }
 
+// Surface voxels of column (i, j) with inside intervals I[0, cnt): inside, and
+// k - 1 or k + 1 outside the column, or voxel k of a side neighbour outside.
+// column(i, j, cnt) returns a column's intervals (none outside the domain).
+template <class ColumnFn, class Emit>
+void column_surface(int i, int j, const Interval* I, size_t cnt, ColumnFn&& column, std::vector<Interval>& interior,
+                    Emit&& emit) {
+    // erode along z inside the domain: k-1 and k+1 inside
+    interior.clear();
+    for (size_t q = 0; q < cnt; ++q)
+        if (I[q].lo + 1 <= I[q].hi - 1) interior.push_back({I[q].lo + 1, I[q].hi - 1});
+    const int nb[4][2] = {{i - 1, j}, {i + 1, j}, {i, j - 1}, {i, j + 1}};
+    for (int q = 0; q < 4 && !interior.empty(); ++q) {
+        size_t c2;
+        const Interval* J = column(nb[q][0], nb[q][1], c2);
+        intersect(interior, J, c2);
+    }
+    // surface = I \ interior, both sorted
+    size_t p = 0;
+    for (size_t q = 0; q < cnt; ++q) {
+        int k = I[q].lo;
+        while (k <= I[q].hi) {
+            while (p < interior.size() && interior[p].hi < k) ++p;
+            int stop = I[q].hi;
+            if (p < interior.size() && interior[p].lo <= k) {
+                k = interior[p].hi + 1;
+                continue;
+            }
+            if (p < interior.size()) stop = std::min(stop, interior[p].lo - 1);
+            for (; k <= stop; ++k) emit(k);
+        }
+    }
+}
+
 }  // namespace
 
-static int build_synthetic(const yx_scene_desc& d, yx_scene& out) {
-    const int D = d.depth;
-    const int n = 1 << D;
+// sorted, disjoint, non-touching interval lists
+static void interval_union(std::vector<Interval>& a, Interval v) {
+    size_t p = 0;
+    while (p < a.size() && a[p].lo <= v.lo) ++p;
+    a.insert(a.begin() + (ptrdiff_t)p, v);
+    std::vector<Interval> r;
+    r.reserve(a.size());
+    for (const Interval& x : a) {
+        if (!r.empty() && x.lo <= r.back().hi + 1)
+            r.back().hi = std::max(r.back().hi, x.hi);
+        else
+            r.push_back(x);
+    }
+    a.swap(r);
+}
+
+static void interval_subtract(std::vector<Interval>& a, Interval v) {
+    std::vector<Interval> r;
+    r.reserve(a.size() + 1);
+    for (const Interval& x : a) {
+        if (x.hi < v.lo || x.lo > v.hi) {
+            r.push_back(x);
+            continue;
+        }
+        if (x.lo < v.lo) r.push_back({x.lo, v.lo - 1});
+        if (x.hi > v.hi) r.push_back({v.hi + 1, x.hi});
+    }
+    a.swap(r);
+}
+
+void EditSet::index(int n) {
+    cells = std::max(1, (n + (1 << kCellShift) - 1) >> kCellShift);
+    cell_off.assign((size_t)cells * cells + 1, 0);
+    auto range = [&](const EditSphere& e, int& cx0, int& cx1, int& cy0, int& cy1) {
+        cx0 = e.xlo >> kCellShift;
+        cx1 = e.xhi >> kCellShift;
+        cy0 = e.ylo >> kCellShift;
+        cy1 = e.yhi >> kCellShift;
+        return e.xlo <= e.xhi && e.ylo <= e.yhi;
+    };
+    for (const EditSphere& e : edits) {
+        int cx0, cx1, cy0, cy1;
+        if (!range(e, cx0, cx1, cy0, cy1)) continue;
+        for (int cy = cy0; cy <= cy1; ++cy)
+            for (int cx = cx0; cx <= cx1; ++cx) cell_off[(size_t)cy * cells + cx + 1]++;
+    }
+    for (size_t c = 0; c + 1 < cell_off.size(); ++c) cell_off[c + 1] += cell_off[c];
+    cell_ids.assign(cell_off.back(), 0);
+    std::vector<uint32_t> fill(cell_off.begin(), cell_off.end() - 1);
+    for (size_t q = 0; q < edits.size(); ++q) {
+        int cx0, cx1, cy0, cy1;
+        if (!range(edits[q], cx0, cx1, cy0, cy1)) continue;
+        for (int cy = cy0; cy <= cy1; ++cy)
+            for (int cx = cx0; cx <= cx1; ++cx) cell_ids[fill[(size_t)cy * cells + cx]++] = (uint32_t)q;
+    }
+}
+
+int edit_spheres(const yx_sphere_edit* e, size_t n, int depth, EditSet& set) {
+    std::vector<EditSphere>& out = set.edits;
+    out.clear();
+    const int grid = 1 << depth;
+    for (size_t q = 0; q < n; ++q) {
+        const yx_sphere_edit& x = e[q];
+        if (x.mode != YX_BUILD_GROW && x.mode != YX_BUILD_CLEAR)
+            return fail(YX_ERR_INVALID_ARG, "edit %zu: unknown mode %d", q, x.mode);
+        if (x.level < 1 || x.level > depth)
+            return fail(YX_ERR_INVALID_ARG, "edit %zu: level %d outside 1..%d", q, x.level, depth);
+        if (!(x.radius > 0.0f) || !std::isfinite(x.radius) || !std::isfinite(x.pos[0]) || !std::isfinite(x.pos[1]) ||
+            !std::isfinite(x.pos[2]))
+            return fail(YX_ERR_INVALID_ARG, "edit %zu: radius must be positive and finite", q);
+        const double scale = std::ldexp(1.0, depth - x.level);
+        // keeps the column bounds of the ball within int range
+        const double lim = 16777216.0;
+        if (std::fabs((double)x.pos[0] * scale) > lim || std::fabs((double)x.pos[1] * scale) > lim ||
+            std::fabs((double)x.pos[2] * scale) > lim || (double)x.radius * scale > lim)
+            return fail(YX_ERR_INVALID_ARG, "edit %zu: centre or radius beyond 2^24 voxels", q);
+        EditSphere s;
+        s.mode = x.mode;
+        s.cx = (double)x.pos[0] * scale;
+        s.cy = (double)x.pos[1] * scale;
+        s.cz = (double)x.pos[2] * scale;
+        s.r = (double)x.radius * scale;
+        s.color = x.color;
+        s.xlo = std::max(0, (int)std::max(-1.0, std::floor(s.cx - s.r - 1.0)));
+        s.xhi = std::min(grid - 1, (int)std::min((double)grid, std::ceil(s.cx + s.r + 1.0)));
+        s.ylo = std::max(0, (int)std::max(-1.0, std::floor(s.cy - s.r - 1.0)));
+        s.yhi = std::min(grid - 1, (int)std::min((double)grid, std::ceil(s.cy + s.r + 1.0)));
+        out.push_back(s);
+    }
+    set.index(grid);
+    return YX_OK;
+}
+
+// The recipe's spheres in voxel units: centres uniform in the domain, radii
+// uniform in [radius_min, radius_max], drawn in that order from SplitMix64(seed).
+static std::vector<Sphere> recipe_spheres(const yx_scene_desc& d) {
+    const int n = 1 << d.depth;
     SplitMix64 rng(d.seed);
     std::vector<Sphere> spheres(d.num_spheres);
     for (auto& s : spheres) {
@@ -341,9 +506,20 @@ static int build_synthetic(const yx_scene_desc& d, yx_scene& out) {
         s.cz = cz * n;
         s.r = r;
     }
+    return spheres;
+}
+
+static int build_synthetic(const yx_scene_desc& d, yx_scene& out, const EditSet* edits = nullptr) {
+    const int D = d.depth;
+    const int n = 1 << D;
+    const std::vector<Sphere> spheres = recipe_spheres(d);
     Perlin noise(d.seed ^ 0xD1B54A32D192ED03ull);
     Heightfield hf{&noise, d.hf_octaves, (double)d.hf_base, (double)d.hf_amplitude, n};
 
+    if (d.heightfield)
+        out.hf_top.assign((size_t)n * n, 0);
+    else
+        out.hf_top.clear();
     // Pass 1: merged inside-interval lists per column, row bands in parallel.
     const int band = std::max(1, std::min(n, 16));
     const int nbands = (n + band - 1) / band;
@@ -372,6 +548,7 @@ static int build_synthetic(const yx_scene_desc& d, yx_scene& out) {
             for (int j = j0; j < j1; ++j)
                 for (int i = 0; i < n; ++i) {
                     int top = hf.top(i, j);
+                    out.hf_top[(size_t)j * n + i] = top;  // kept for in-place edits
                     if (top >= 0) es.push_back({(uint32_t)((j - j0) * n + i), Interval{0, top}});
                 }
         }
@@ -382,16 +559,32 @@ static int build_synthetic(const yx_scene_desc& d, yx_scene& out) {
         auto& ivs = band_iv[b];
         off.assign((size_t)(j1 - j0) * n + 1, 0);
         size_t e = 0;
+        std::vector<Interval> colv;
         for (uint32_t col = 0; col < (uint32_t)((j1 - j0) * n); ++col) {
             off[col] = (uint32_t)ivs.size();
+            colv.clear();
             while (e < es.size() && es[e].col == col) {
                 Interval iv = es[e].iv;
-                if (!ivs.empty() && off[col] < ivs.size() && iv.lo <= ivs.back().hi + 1)
-                    ivs.back().hi = std::max(ivs.back().hi, iv.hi);
+                if (!colv.empty() && iv.lo <= colv.back().hi + 1)
+                    colv.back().hi = std::max(colv.back().hi, iv.hi);
                 else
-                    ivs.push_back(iv);
+                    colv.push_back(iv);
                 ++e;
             }
+            // sphere edits in order: union (GROW) or difference (CLEAR)
+            const int i = (int)(col % (uint32_t)n), j = j0 + (int)(col / (uint32_t)n);
+            if (edits && !edits->empty())
+            for (const uint32_t* e = edits->begin(i, j); e != edits->end(i, j); ++e) {
+                const EditSphere& x = edits->edits[*e];
+                if (i < x.xlo || i > x.xhi || j < x.ylo || j > x.yhi) continue;
+                const Interval iv = sphere_column(Sphere{x.cx, x.cy, x.cz, x.r}, i, j, n);
+                if (iv.lo > iv.hi) continue;
+                if (x.mode == YX_BUILD_GROW)
+                    interval_union(colv, iv);
+                else
+                    interval_subtract(colv, iv);
+            }
+            ivs.insert(ivs.end(), colv.begin(), colv.end());
         }
         off[(size_t)(j1 - j0) * n] = (uint32_t)ivs.size();
     });
@@ -420,31 +613,7 @@ static int build_synthetic(const yx_scene_desc& d, yx_scene& out) {
                 size_t cnt;
                 const Interval* I = column(i, j, cnt);
                 if (cnt == 0) continue;
-                // erode along z inside the domain: k-1 and k+1 inside
-                interior.clear();
-                for (size_t q = 0; q < cnt; ++q)
-                    if (I[q].lo + 1 <= I[q].hi - 1) interior.push_back({I[q].lo + 1, I[q].hi - 1});
-                const int nb[4][2] = {{i - 1, j}, {i + 1, j}, {i, j - 1}, {i, j + 1}};
-                for (int q = 0; q < 4 && !interior.empty(); ++q) {
-                    size_t c2;
-                    const Interval* J = column(nb[q][0], nb[q][1], c2);
-                    intersect(interior, J, c2);
-                }
-                // surface = I \ interior, both sorted
-                size_t p = 0;
-                for (size_t q = 0; q < cnt; ++q) {
-                    int k = I[q].lo;
-                    while (k <= I[q].hi) {
-                        while (p < interior.size() && interior[p].hi < k) ++p;
-                        int stop = I[q].hi;
-                        if (p < interior.size() && interior[p].lo <= k) {
-                            k = interior[p].hi + 1;
-                            continue;
-                        }
-                        if (p < interior.size()) stop = std::min(stop, interior[p].lo - 1);
-                        for (; k <= stop; ++k) codes.push_back(morton3(i, j, k));
-                    }
-                }
+                column_surface(i, j, I, cnt, column, interior, [&](int k) { codes.push_back(morton3(i, j, k)); });
             }
     });
 
@@ -475,7 +644,220 @@ static int build_synthetic(const yx_scene_desc& d, yx_scene& out) {
     });
     parallel_for(nbuckets, [&](int k) { std::sort(codes.begin() + bstart[k], codes.begin() + bstart[k + 1]); });
 
-    build_tree_from_codes(codes, D, d.seed, out);
+    build_tree_from_codes(codes, D, d.seed, out, edits && !edits->empty() ? edits : nullptr);
+    return YX_OK;
+}
+
+// ---- sphere edits in place -----------------------------------------------------
+
+namespace {
+
+// Rewrites the nodes under the voxel box [lo, hi] of a scene whose leaves all
+// sit at level `depth`: leaves inside the box become exactly `codes` (sorted
+// Morton codes, with `colors`), nodes outside the box keep their ids and bytes.
+// Emptied nodes are blanked and their ids reused, lowest first, before the
+// array grows. Every written node's page gets `version`.
+struct Splice {
+    yx_scene& s;
+    int depth;
+    int lo[3], hi[3];
+    const std::vector<uint64_t>& codes;
+    const std::vector<uint32_t>& colors;
+    uint64_t version;
+    int64_t leaf_delta = 0;
+
+    void mark(uint32_t id) {
+        const size_t p = id / kScenePageNodes;
+        if (s.page_version.size() <= p) s.page_version.resize(p + 1, 1u);
+        s.page_version[p] = version;
+    }
+    uint32_t alloc() {
+        if (!s.free_nodes.empty()) {
+            std::pop_heap(s.free_nodes.begin(), s.free_nodes.end(), std::greater<uint32_t>());
+            const uint32_t id = s.free_nodes.back();
+            s.free_nodes.pop_back();
+            return id;
+        }
+        const size_t id = s.nodes.size() / 10;
+        if (id >= 0x7FFFFFFFull) throw std::length_error("node ids must stay below 2^31");
+        if (s.nodes.size() == s.nodes.capacity()) s.nodes.reserve(s.nodes.size() + s.nodes.size() / 4 + 10240);
+        s.nodes.resize((id + 1) * 10, 0u);
+        return (uint32_t)id;
+    }
+    void release(uint32_t id) {
+        uint32_t* r = &s.nodes[(size_t)id * 10];
+        r[0] = 0xFF00u;
+        r[1] = 0;
+        for (int c = 0; c < 8; ++c) r[2 + c] = YX_EMPTY_NODE;
+        mark(id);
+        s.free_nodes.push_back(id);
+        std::push_heap(s.free_nodes.begin(), s.free_nodes.end(), std::greater<uint32_t>());
+    }
+    bool meets_box(uint64_t cube, int level) const {
+        const int size = 1 << (depth - level);
+        const int o[3] = {(int)compact3(cube) * size, (int)compact3(cube >> 1) * size, (int)compact3(cube >> 2) * size};
+        for (int a = 0; a < 3; ++a)
+            if (o[a] > hi[a] || o[a] + size - 1 < lo[a]) return false;
+        return true;
+    }
+    // node `id` (or YX_EMPTY_NODE) covering cube `cube` of `level`; the new
+    // leaves in the cube are codes[b, e). Returns the node's id afterwards.
+    uint32_t visit(uint32_t id, int level, uint64_t cube, size_t b, size_t e) {
+        uint32_t old_row[10], row[10];
+        if (id != YX_EMPTY_NODE) {
+            std::memcpy(old_row, &s.nodes[(size_t)id * 10], 40);
+        } else {
+            old_row[0] = 0xFF00u;
+            old_row[1] = 0;
+            for (int c = 0; c < 8; ++c) old_row[2 + c] = YX_EMPTY_NODE;
+        }
+        std::memcpy(row, old_row, 40);
+        const int shift = 3 * (depth - level - 1);
+        size_t q = b;
+        for (int c = 0; c < 8; ++c) {
+            const uint64_t child = (cube << 3) | (uint64_t)c;
+            const size_t cb = q;
+            while (q < e && (codes[q] >> shift) == child) ++q;
+            if (!meets_box(child, level + 1)) continue;
+            const bool was_leaf = (old_row[0] >> c) & 1u;
+            if (level == depth - 1) {
+                const bool now_leaf = q > cb;
+                row[2 + c] = now_leaf ? colors[cb] : YX_EMPTY_NODE;
+                row[0] = (row[0] & ~((1u << c) | (1u << (8 + c)))) | (now_leaf ? 1u << c : 1u << (8 + c));
+                leaf_delta += (int64_t)now_leaf - (int64_t)was_leaf;
+            } else {
+                if (was_leaf) throw std::logic_error("a leaf above the finest level");
+                const uint32_t nc = visit(old_row[2 + c], level + 1, child, cb, q);
+                row[2 + c] = nc;
+                row[0] = (row[0] & ~(1u << (8 + c))) | (nc == YX_EMPTY_NODE ? 1u << (8 + c) : 0u);
+            }
+        }
+        if ((row[0] & 0xFFu) == 0 && (row[0] & 0xFF00u) == 0xFF00u) {
+            if (id != YX_EMPTY_NODE) release(id);
+            return YX_EMPTY_NODE;
+        }
+        if (id == YX_EMPTY_NODE) id = alloc();
+        else if (std::memcmp(row, old_row, 40) == 0) return id;
+        std::memcpy(&s.nodes[(size_t)id * 10], row, 40);
+        mark(id);
+        return id;
+    }
+};
+
+}  // namespace
+
+// Applies edits[first..] one at a time to the scene in place (the scene holds
+// the recipe's tree with edits[0, first) applied).
+static int edit_in_place(yx_scene& s, const EditSet& set, size_t first) {
+    const std::vector<EditSphere>& edits = set.edits;
+    const yx_scene_desc& d = s.recipe;
+    const int D = d.depth, n = 1 << D;
+    const std::vector<Sphere> spheres = recipe_spheres(d);
+    Perlin noise(d.seed ^ 0xD1B54A32D192ED03ull);
+    Heightfield hf{&noise, d.hf_octaves, (double)d.hf_base, (double)d.hf_amplitude, n};
+    constexpr int32_t kTopUnknown = INT32_MIN;
+    if (d.heightfield && s.hf_top.size() != (size_t)n * n) s.hf_top.assign((size_t)n * n, kTopUnknown);
+    for (size_t q = first; q < edits.size(); ++q) {
+        const EditSphere& e = edits[q];
+        // voxels the edit can change: the ball's voxel box (with a voxel of
+        // slack), then one more for the surface test of its neighbours
+        int lo[3], hi[3];
+        const double c[3] = {e.cx, e.cy, e.cz};
+        bool empty = false;
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = (int)std::max(-1.0, std::floor(c[a] - e.r - 0.5) - 1.0);
+            hi[a] = (int)std::min((double)n, std::ceil(c[a] + e.r - 0.5) + 1.0);
+            lo[a] = std::max(0, lo[a]);
+            hi[a] = std::min(n - 1, hi[a]);
+            if (lo[a] > hi[a]) empty = true;
+            lo[a] = std::max(0, lo[a] - 1);
+            hi[a] = std::min(n - 1, hi[a] + 1);
+        }
+        if (empty) continue;
+        // inside intervals of the columns of the box and its rim
+        const int x0 = std::max(0, lo[0] - 1), x1 = std::min(n - 1, hi[0] + 1);
+        const int y0 = std::max(0, lo[1] - 1), y1 = std::min(n - 1, hi[1] + 1);
+        const int nx = x1 - x0 + 1, ny = y1 - y0 + 1;
+        // only voxels k in [lo[2] - 1, hi[2] + 1] of these columns are ever
+        // tested, so spheres that miss that slab can be left out
+        const int z0 = lo[2] - 1, z1 = hi[2] + 1;
+        std::vector<Sphere> near;
+        for (const Sphere& sp : spheres)
+            if (sp.cx + sp.r + 1.0 >= x0 && sp.cx - sp.r - 1.0 <= x1 + 1 && sp.cy + sp.r + 1.0 >= y0 &&
+                sp.cy - sp.r - 1.0 <= y1 + 1 && sp.cz + sp.r + 1.0 >= z0 && sp.cz - sp.r - 1.0 <= z1 + 1)
+                near.push_back(sp);
+        std::vector<std::vector<Interval>> cols((size_t)nx * ny);
+        auto rows = [&](int jj) {
+            const int j = y0 + jj;
+            for (int i = x0; i <= x1; ++i) {
+                std::vector<Interval>& colv = cols[(size_t)jj * nx + (i - x0)];
+                std::vector<Interval> ivs;
+                for (const Sphere& sp : near) {
+                    const Interval iv = sphere_column(sp, i, j, n);
+                    if (iv.lo <= iv.hi) ivs.push_back(iv);
+                }
+                if (d.heightfield) {
+                    int32_t& top = s.hf_top[(size_t)j * n + i];
+                    if (top == kTopUnknown) top = hf.top(i, j);
+                    if (top >= 0) ivs.push_back(Interval{0, top});
+                }
+                std::sort(ivs.begin(), ivs.end(), [](const Interval& a, const Interval& b) { return a.lo < b.lo; });
+                for (const Interval& iv : ivs) {
+                    if (!colv.empty() && iv.lo <= colv.back().hi + 1)
+                        colv.back().hi = std::max(colv.back().hi, iv.hi);
+                    else
+                        colv.push_back(iv);
+                }
+                for (const uint32_t* ek = set.begin(i, j); ek != set.end(i, j) && *ek <= q; ++ek) {
+                    const EditSphere& x = edits[*ek];
+                    if (i < x.xlo || i > x.xhi || j < x.ylo || j > x.yhi) continue;
+                    const Interval iv = sphere_column(Sphere{x.cx, x.cy, x.cz, x.r}, i, j, n);
+                    if (iv.lo > iv.hi) continue;
+                    if (x.mode == YX_BUILD_GROW)
+                        interval_union(colv, iv);
+                    else
+                        interval_subtract(colv, iv);
+                }
+            }
+        };
+        // threads only pay off past a few thousand columns
+        if ((size_t)nx * ny >= 4096)
+            parallel_for(ny, rows);
+        else
+            for (int jj = 0; jj < ny; ++jj) rows(jj);
+        auto column = [&](int i, int j, size_t& cnt) -> const Interval* {
+            if (i < x0 || j < y0 || i > x1 || j > y1) {
+                cnt = 0;
+                return nullptr;
+            }
+            const std::vector<Interval>& v = cols[(size_t)(j - y0) * nx + (i - x0)];
+            cnt = v.size();
+            return v.data();
+        };
+        // the new leaves of the box
+        std::vector<uint64_t> codes;
+        std::vector<Interval> interior;
+        for (int j = lo[1]; j <= hi[1]; ++j)
+            for (int i = lo[0]; i <= hi[0]; ++i) {
+                size_t cnt;
+                const Interval* I = column(i, j, cnt);
+                if (cnt == 0) continue;
+                column_surface(i, j, I, cnt, column, interior, [&](int k) {
+                    if (k >= lo[2] && k <= hi[2]) codes.push_back(morton3(i, j, k));
+                });
+            }
+        std::sort(codes.begin(), codes.end());
+        std::vector<uint32_t> colors(codes.size());
+        for (size_t k = 0; k < codes.size(); ++k) colors[k] = edited_color(codes[k], d.seed, &set, q + 1);
+        s.version += 1;
+        Splice sp{s, D, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}, codes, colors, s.version};
+        s.root = sp.visit(s.root, 0, 0, 0, codes.size());
+        s.num_leaves = (uint64_t)((int64_t)s.num_leaves + sp.leaf_delta);
+    }
+    const size_t pages = (s.nodes.size() / 10 + kScenePageNodes - 1) / kScenePageNodes;
+    if (s.page_version.size() < pages) s.page_version.resize(pages, s.version);
+    s.depth = s.root == YX_EMPTY_NODE ? 0 : D;
+    s.depth_stale = false;
     return YX_OK;
 }
 
@@ -685,7 +1067,73 @@ extern "C" int yx_scene_build_synthetic(const yx_scene_desc* desc, yx_scene** ou
         delete s;
         return fail(YX_ERR_OOM, "out of host memory building the scene");
     }
+    s->has_recipe = true;
+    s->recipe = *desc;
     *out = s;
+    return YX_OK;
+}
+
+extern "C" int yx_scene_build_range(yx_scene* s, const yx_sphere_edit* edits, uint32_t count, int32_t how,
+                                    void* stream) {
+    clear_error();
+    if (!s || (!edits && count)) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (how != YX_RANGE_IN_PLACE && how != YX_RANGE_REBUILD && how != YX_RANGE_REBUILD_GPU)
+        return fail(YX_ERR_INVALID_ARG, "unknown build-range method %d", how);
+    if (!s->has_recipe)
+        return fail(YX_ERR_INVALID_ARG, "the scene has no synthetic recipe (loaded, or edited node by node)");
+    if (count == 0) return YX_OK;
+    std::vector<yx_sphere_edit> all;
+    EditSet es;
+    try {
+        all = s->edits;
+        all.insert(all.end(), edits, edits + count);
+    } catch (const std::bad_alloc&) {
+        return fail(YX_ERR_OOM, "out of host memory");
+    }
+    int rc = edit_spheres(all.data(), all.size(), s->recipe.depth, es);
+    if (rc != YX_OK) return rc;
+    if (how == YX_RANGE_IN_PLACE) {
+        try {
+            rc = edit_in_place(*s, es, s->edits.size());
+        } catch (const std::bad_alloc&) {
+            return fail(YX_ERR_OOM, "out of host memory editing the scene (the scene is partly edited)");
+        } catch (const std::exception& x) {
+            return fail(YX_ERR_RANGE, "%s (the scene is partly edited)", x.what());
+        }
+        if (rc == YX_OK) s->edits.swap(all);
+        return rc;
+    }
+    yx_scene t;
+    try {
+        rc = how == YX_RANGE_REBUILD_GPU ? gpu_build_synthetic(s->recipe, es, t, stream)
+                                         : build_synthetic(s->recipe, t, &es);
+    } catch (const std::bad_alloc&) {
+        return fail(YX_ERR_OOM, "out of host memory rebuilding the scene");
+    }
+    if (rc != YX_OK) return rc;
+    // new version on the pages whose bytes changed (and on new pages)
+    const size_t old_n = s->nodes.size() / 10, new_n = t.nodes.size() / 10;
+    const size_t pages = (new_n + kScenePageNodes - 1) / kScenePageNodes;
+    s->version += 1;
+    try {
+        s->page_version.resize(std::max(pages, s->page_version.size()), 1u);
+    } catch (const std::bad_alloc&) {
+        return fail(YX_ERR_OOM, "out of host memory");
+    }
+    for (size_t p = 0; p < pages; ++p) {
+        const size_t n0 = p * kScenePageNodes, n1 = std::min(new_n, n0 + kScenePageNodes);
+        const bool same = n1 <= old_n && std::memcmp(&s->nodes[n0 * 10], &t.nodes[n0 * 10], (n1 - n0) * 40) == 0;
+        if (!same) s->page_version[p] = s->version;
+    }
+    s->page_version.resize(pages);
+    s->nodes.swap(t.nodes);
+    s->root = t.root;
+    s->num_leaves = t.num_leaves;
+    s->depth = t.depth;
+    s->depth_stale = false;
+    s->free_nodes.clear();
+    if (!t.hf_top.empty()) s->hf_top.swap(t.hf_top);
+    s->edits.swap(all);
     return YX_OK;
 }
 
@@ -779,6 +1227,8 @@ extern "C" int yx_scene_write_nodes(yx_scene* s, uint32_t first, const uint32_t*
     for (uint64_t p = first / kScenePageNodes; p <= (end - 1) / kScenePageNodes; ++p)
         s->page_version[(size_t)p] = s->version;
     s->depth_stale = true;
+    s->has_recipe = false;  // the nodes no longer follow the recipe
+    s->free_nodes.clear();
     return YX_OK;
 }
 
@@ -788,6 +1238,7 @@ extern "C" int yx_scene_set_root(yx_scene* s, uint32_t root) {
     s->root = root;
     s->version += 1;
     s->depth_stale = true;
+    s->has_recipe = false;
     return YX_OK;
 }
 

@@ -51,7 +51,15 @@ struct SbArgs {
     SbInterval* iv;  // [n * n][kColMax]
     int* cnt;        // [n * n]
     int* overflow;
+    const EditSphere* edits;  // applied in order after the recipe's solid
+    const uint32_t* cell_off;   // edits per 64 x 64-column cell (EditSet's CSR)
+    const uint32_t* cell_ids;
+    int cells;
 };
+
+__device__ __forceinline__ size_t sb_cell(int cells, int i, int j) {
+    return (size_t)(j >> EditSet::kCellShift) * cells + (size_t)(i >> EditSet::kCellShift);
+}
 
 __device__ __forceinline__ bool sb_in_sphere(const SbSphere& s, int i, int j, int k) {
     const double dx = ((double)i + 0.5) - s.cx;
@@ -179,6 +187,71 @@ __global__ __launch_bounds__(256) void sb_columns_kernel(SbArgs a) {
         else
             out[m++] = v;
     }
+    // sphere edits in order: union (GROW) or difference (CLEAR); the list
+    // stays sorted, disjoint and non-touching
+    const size_t cell = a.edits ? sb_cell(a.cells, i, j) : 0;
+    const uint32_t e0 = a.edits ? a.cell_off[cell] : 0, e1 = a.edits ? a.cell_off[cell + 1] : 0;
+    for (uint32_t e = e0; e < e1 && !over; ++e) {
+        const EditSphere& x = a.edits[a.cell_ids[e]];
+        if (i < x.xlo || i > x.xhi || j < x.ylo || j > x.yhi) continue;
+        SbSphere sp;
+        sp.cx = x.cx;
+        sp.cy = x.cy;
+        sp.cz = x.cz;
+        sp.r = x.r;
+        sp.xlo = x.xlo;
+        sp.xhi = x.xhi;
+        sp.ylo = x.ylo;
+        sp.yhi = x.yhi;
+        const SbInterval v = sb_sphere_column(sp, i, j, n);
+        if (v.lo > v.hi) continue;
+        if (x.mode == YX_BUILD_GROW) {
+            if (m == kColMax) {
+                over = true;
+                break;
+            }
+            int p = m;
+            while (p > 0 && out[p - 1].lo > v.lo) {
+                out[p] = out[p - 1];
+                --p;
+            }
+            out[p] = v;
+            ++m;
+            int w = 0;
+            for (int q = 0; q < m; ++q) {
+                const SbInterval u = out[q];
+                if (w > 0 && u.lo <= out[w - 1].hi + 1)
+                    out[w - 1].hi = max(out[w - 1].hi, u.hi);
+                else
+                    out[w++] = u;
+            }
+            m = w;
+        } else {
+            // overlapping intervals are the run [p, q]
+            int p = 0;
+            while (p < m && out[p].hi < v.lo) ++p;
+            int q = p;
+            while (q < m && out[q].lo <= v.hi) ++q;
+            if (q == p) continue;  // nothing inside the ball
+            const SbInterval first = out[p], last = out[q - 1];
+            SbInterval keep[2];
+            int nk = 0;
+            if (first.lo < v.lo) keep[nk++] = SbInterval{first.lo, v.lo - 1};
+            if (last.hi > v.hi) keep[nk++] = SbInterval{v.hi + 1, last.hi};
+            const int removed = q - p, delta = nk - removed;
+            if (m + delta > kColMax) {
+                over = true;
+                break;
+            }
+            if (delta > 0) {
+                for (int r = m - 1; r >= q; --r) out[r + delta] = out[r];
+            } else if (delta < 0) {
+                for (int r = q; r < m; ++r) out[r + delta] = out[r];
+            }
+            for (int r = 0; r < nk; ++r) out[p + r] = keep[r];
+            m += delta;
+        }
+    }
     a.cnt[col] = m;
     if (over) atomicOr(a.overflow, 1);
 }
@@ -274,6 +347,32 @@ __device__ __forceinline__ uint32_t sb_leaf_color(uint64_t code, uint64_t seed) 
     return (uint32_t)(z ^ (z >> 31));
 }
 
+__device__ __forceinline__ uint32_t sb_compact3(uint64_t v) {
+    v &= 0x1249249249249249ull;
+    v = (v ^ (v >> 2)) & 0x10c30c30c30c30c3ull;
+    v = (v ^ (v >> 4)) & 0x100f00f00f00f00full;
+    v = (v ^ (v >> 8)) & 0x1f0000ff0000ffull;
+    v = (v ^ (v >> 16)) & 0x1f00000000ffffull;
+    v = (v ^ (v >> 32)) & 0x1fffffull;
+    return (uint32_t)v;
+}
+
+// the last edit whose ball holds the voxel decides its colour
+__device__ __forceinline__ uint32_t sb_edited_color(uint64_t code, uint64_t seed, const SbArgs& a) {
+    if (!a.edits) return sb_leaf_color(code, seed);
+    const uint32_t i = sb_compact3(code), j = sb_compact3(code >> 1), k = sb_compact3(code >> 2);
+    const size_t cell = sb_cell(a.cells, (int)i, (int)j);
+    for (uint32_t q = a.cell_off[cell + 1]; q-- > a.cell_off[cell];) {
+        const EditSphere& e = a.edits[a.cell_ids[q]];
+        if ((int)i < e.xlo || (int)i > e.xhi || (int)j < e.ylo || (int)j > e.yhi) continue;
+        const double dx = ((double)i + 0.5) - e.cx;
+        const double dy = ((double)j + 0.5) - e.cy;
+        const double dz = ((double)k + 0.5) - e.cz;
+        if (dx * dx + dy * dy + dz * dz <= e.r * e.r) return e.mode == YX_BUILD_GROW ? e.color : sb_leaf_color(code, seed);
+    }
+    return sb_leaf_color(code, seed);
+}
+
 struct Shr3 {
     __host__ __device__ uint64_t operator()(uint64_t v) const { return v >> 3; }
 };
@@ -286,7 +385,8 @@ struct ToU64 {
 // (the last node level, below = leaf codes).
 __global__ __launch_bounds__(256) void sb_nodes_kernel(const uint64_t* cur, uint64_t n_cur, const uint64_t* below,
                                                        uint64_t n_below, uint64_t node_base, uint64_t child_base,
-                                                       int leaf_level, uint64_t color_seed, uint32_t* nodes) {
+                                                       int leaf_level, uint64_t color_seed, SbArgs a,
+                                                       uint32_t* nodes) {
     const uint64_t k = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (k >= n_cur) return;
     const uint64_t code = cur[k];
@@ -306,7 +406,7 @@ __global__ __launch_bounds__(256) void sb_nodes_kernel(const uint64_t* cur, uint
     for (uint64_t q = lo; q < n_below && (below[q] >> 3) == code; ++q) {
         const int s = (int)(below[q] & 7u);
         if (leaf_level) {
-            child[s] = sb_leaf_color(below[q], color_seed);
+            child[s] = sb_edited_color(below[q], color_seed, a);
             flags |= 1u << s;
         } else {
             child[s] = (uint32_t)(child_base + q);
@@ -354,10 +454,8 @@ struct SbBufs {
 
 }  // namespace
 
-extern "C" int yx_scene_build_synthetic_gpu(const yx_scene_desc* d, yx_scene** out, void* stream) {
-    clear_error();
-    if (!d || !out) return fail(YX_ERR_INVALID_ARG, "null argument");
-    *out = nullptr;
+// Builds the scene of recipe d with edits into s (nodes, root, counts).
+static int gpu_build(const yx_scene_desc* d, const EditSet& edits, yx_scene* s, void* stream) {
     if (d->depth < 1 || d->depth > 12) return fail(YX_ERR_INVALID_ARG, "depth %d: 1..12 on the GPU", d->depth);
     if (d->num_spheres < 0 || d->num_spheres > (1 << 20))
         return fail(YX_ERR_INVALID_ARG, "num_spheres %d", d->num_spheres);
@@ -387,6 +485,19 @@ extern "C" int yx_scene_build_synthetic_gpu(const yx_scene_desc* d, yx_scene** o
 
     SbBufs B;
     int rc;
+    EditSphere* d_edits = nullptr;
+    uint32_t *d_cell_off = nullptr, *d_cell_ids = nullptr;
+    if (!edits.empty()) {
+        const size_t ne = edits.edits.size(), no = edits.cell_off.size(), ni = std::max<size_t>(1, edits.cell_ids.size());
+        if ((rc = B.alloc(&d_edits, ne)) != YX_OK || (rc = B.alloc(&d_cell_off, no)) != YX_OK ||
+            (rc = B.alloc(&d_cell_ids, ni)) != YX_OK)
+            return rc;
+        YX_HIP(hipMemcpyAsync(d_edits, edits.edits.data(), ne * sizeof(EditSphere), hipMemcpyHostToDevice, st));
+        YX_HIP(hipMemcpyAsync(d_cell_off, edits.cell_off.data(), no * 4, hipMemcpyHostToDevice, st));
+        if (!edits.cell_ids.empty())
+            YX_HIP(hipMemcpyAsync(d_cell_ids, edits.cell_ids.data(), edits.cell_ids.size() * 4,
+                                  hipMemcpyHostToDevice, st));
+    }
     SbSphere* d_sph = nullptr;
     int *d_perm = nullptr, *d_cnt = nullptr, *d_over = nullptr;
     SbInterval* d_iv = nullptr;
@@ -409,6 +520,10 @@ extern "C" int yx_scene_build_synthetic_gpu(const yx_scene_desc* d, yx_scene** o
     a.iv = d_iv;
     a.cnt = d_cnt;
     a.overflow = d_over;
+    a.edits = d_edits;
+    a.cell_off = d_cell_off;
+    a.cell_ids = d_cell_ids;
+    a.cells = edits.cells;
     hipLaunchKernelGGL(sb_columns_kernel, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, a);
     YX_HIP(hipGetLastError());
 
@@ -439,35 +554,29 @@ extern "C" int yx_scene_build_synthetic_gpu(const yx_scene_desc* d, yx_scene** o
     if (over) return fail(YX_ERR_RANGE, "a column holds more than %d inside intervals", kColMax);
     const uint64_t total = last_off + last_cnt;
 
-    yx_scene* s = new (std::nothrow) yx_scene();
-    if (!s) return fail(YX_ERR_OOM, "out of host memory");
-    auto give_up = [&](int code) {
-        delete s;
-        return code;
-    };
     s->num_leaves = total;
     if (total == 0) {
+        s->nodes.clear();
         s->root = YX_EMPTY_NODE;
         s->depth = 0;
-        *out = s;
         return YX_OK;
     }
-    if (total >= 0xFFFFFFF0ull) return give_up(fail(YX_ERR_RANGE, "%llu surface voxels", (unsigned long long)total));
+    if (total >= 0xFFFFFFF0ull) return fail(YX_ERR_RANGE, "%llu surface voxels", (unsigned long long)total);
     uint64_t *d_codes = nullptr, *d_sorted = nullptr;
-    if ((rc = B.alloc(&d_codes, total)) != YX_OK || (rc = B.alloc(&d_sorted, total)) != YX_OK) return give_up(rc);
+    if ((rc = B.alloc(&d_codes, total)) != YX_OK || (rc = B.alloc(&d_sorted, total)) != YX_OK) return rc;
     hipLaunchKernelGGL(sb_surface_kernel, dim3((unsigned)((ncol + 127) / 128)), dim3(128), 0, st, a, d_offs,
                        d_counts, d_codes);
-    if (hipGetLastError() != hipSuccess) return give_up(fail(YX_ERR_HIP, "surface kernel launch"));
+    if (hipGetLastError() != hipSuccess) return fail(YX_ERR_HIP, "surface kernel launch");
     {
         size_t bytes = 0;
         if (hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, d_codes, d_sorted, (int)total, 0, 3 * D, st) !=
             hipSuccess)
-            return give_up(fail(YX_ERR_HIP, "radix sort sizing"));
+            return fail(YX_ERR_HIP, "radix sort sizing");
         void* tmp = nullptr;
-        if ((rc = B.alloc((uint8_t**)&tmp, bytes)) != YX_OK) return give_up(rc);
+        if ((rc = B.alloc((uint8_t**)&tmp, bytes)) != YX_OK) return rc;
         if (hipcub::DeviceRadixSort::SortKeys(tmp, bytes, d_codes, d_sorted, (int)total, 0, 3 * D, st) !=
             hipSuccess)
-            return give_up(fail(YX_ERR_HIP, "radix sort"));
+            return fail(YX_ERR_HIP, "radix sort");
     }
 
     // 3. levels: lv[L] = distinct (lv[L + 1] >> 3), lv[D] = leaf codes
@@ -476,48 +585,72 @@ extern "C" int yx_scene_build_synthetic_gpu(const yx_scene_desc* d, yx_scene** o
     lv[(size_t)D] = d_sorted;
     cnt[(size_t)D] = total;
     uint64_t* d_nsel = nullptr;
-    if ((rc = B.alloc(&d_nsel, 1)) != YX_OK) return give_up(rc);
+    if ((rc = B.alloc(&d_nsel, 1)) != YX_OK) return rc;
     for (int L = D - 1; L >= 0; --L) {
-        if ((rc = B.alloc(&lv[(size_t)L], cnt[(size_t)L + 1])) != YX_OK) return give_up(rc);
+        if ((rc = B.alloc(&lv[(size_t)L], cnt[(size_t)L + 1])) != YX_OK) return rc;
         hipcub::TransformInputIterator<uint64_t, Shr3, const uint64_t*> in(lv[(size_t)L + 1], Shr3());
         size_t bytes = 0;
         if (hipcub::DeviceSelect::Unique(nullptr, bytes, in, lv[(size_t)L], d_nsel, (int)cnt[(size_t)L + 1], st) !=
             hipSuccess)
-            return give_up(fail(YX_ERR_HIP, "unique sizing"));
+            return fail(YX_ERR_HIP, "unique sizing");
         void* tmp = nullptr;
-        if ((rc = B.alloc((uint8_t**)&tmp, bytes)) != YX_OK) return give_up(rc);
+        if ((rc = B.alloc((uint8_t**)&tmp, bytes)) != YX_OK) return rc;
         if (hipcub::DeviceSelect::Unique(tmp, bytes, in, lv[(size_t)L], d_nsel, (int)cnt[(size_t)L + 1], st) !=
             hipSuccess)
-            return give_up(fail(YX_ERR_HIP, "unique"));
+            return fail(YX_ERR_HIP, "unique");
         uint64_t m = 0;
         if (hipMemcpyAsync(&m, d_nsel, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
-            return give_up(fail(YX_ERR_HIP, "level size copy"));
+            return fail(YX_ERR_HIP, "level size copy");
         cnt[(size_t)L] = m;
     }
     std::vector<uint64_t> base((size_t)D + 1, 0);
     for (int L = 0; L < D; ++L) base[(size_t)L + 1] = base[(size_t)L] + cnt[(size_t)L];
     const uint64_t total_nodes = base[(size_t)D];
-    if (total_nodes >= 0x7FFFFFFFull) return give_up(fail(YX_ERR_RANGE, "too many nodes"));
+    if (total_nodes >= 0x7FFFFFFFull) return fail(YX_ERR_RANGE, "too many nodes");
     uint32_t* d_nodes = nullptr;
-    if ((rc = B.alloc(&d_nodes, total_nodes * 10)) != YX_OK) return give_up(rc);
+    if ((rc = B.alloc(&d_nodes, total_nodes * 10)) != YX_OK) return rc;
     for (int L = 0; L < D; ++L) {
         const uint64_t m = cnt[(size_t)L];
         hipLaunchKernelGGL(sb_nodes_kernel, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st, lv[(size_t)L], m,
                            lv[(size_t)L + 1], cnt[(size_t)L + 1], base[(size_t)L], base[(size_t)L + 1],
-                           L == D - 1 ? 1 : 0, (uint64_t)d->seed, d_nodes);
-        if (hipGetLastError() != hipSuccess) return give_up(fail(YX_ERR_HIP, "node kernel launch"));
+                           L == D - 1 ? 1 : 0, (uint64_t)d->seed, a, d_nodes);
+        if (hipGetLastError() != hipSuccess) return fail(YX_ERR_HIP, "node kernel launch");
     }
     try {
+        // headroom for nodes that in-place edits append
+        s->nodes.reserve((size_t)(total_nodes + total_nodes / 16 + 1024) * 10);
         s->nodes.resize((size_t)total_nodes * 10);
     } catch (const std::bad_alloc&) {
-        return give_up(fail(YX_ERR_OOM, "out of host memory for %llu nodes", (unsigned long long)total_nodes));
+        return fail(YX_ERR_OOM, "out of host memory for %llu nodes", (unsigned long long)total_nodes);
     }
     if (hipMemcpyAsync(s->nodes.data(), d_nodes, total_nodes * 40, hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
-        return give_up(fail(YX_ERR_HIP, "node download"));
+        return fail(YX_ERR_HIP, "node download");
     s->root = 0;
     s->depth = D;
+    return YX_OK;
+}
+
+extern "C" int yx_scene_build_synthetic_gpu(const yx_scene_desc* d, yx_scene** out, void* stream) {
+    clear_error();
+    if (!d || !out) return fail(YX_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    yx_scene* s = new (std::nothrow) yx_scene();
+    if (!s) return fail(YX_ERR_OOM, "out of host memory");
+    const int rc = gpu_build(d, EditSet{}, s, stream);
+    if (rc != YX_OK) {
+        delete s;
+        return rc;
+    }
+    s->has_recipe = true;
+    s->recipe = *d;
     *out = s;
     return YX_OK;
 }
+
+namespace yx {
+int gpu_build_synthetic(const yx_scene_desc& d, const EditSet& edits, yx_scene& out, void* stream) {
+    return gpu_build(&d, edits, &out, stream);
+}
+}  // namespace yx

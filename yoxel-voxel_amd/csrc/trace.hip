@@ -151,6 +151,11 @@ struct yx_renderer {
     uint64_t last_bytes = 0;
     RelayoutWork* rl_work = nullptr;  // GPU re-layout workspace
     size_t recs_cap = 0;              // records d_recs / d_lod can hold
+    // staging of scattered dirty pages: page table + node bytes, pinned on
+    // the host and its device twin
+    uint8_t* stage_host = nullptr;
+    uint8_t* d_stage = nullptr;
+    size_t stage_cap = 0;
 };
 
 #define YX_HIP(call)                                                                        \
@@ -348,6 +353,8 @@ extern "C" void yx_renderer_destroy(yx_renderer* r) {
         if (z) (void)hipFree(z);
     if (r->d_albedo) (void)hipFree(r->d_albedo);
     if (r->d_vox) (void)hipFree(r->d_vox);
+    if (r->stage_host) (void)hipHostFree(r->stage_host);
+    if (r->d_stage) (void)hipFree(r->d_stage);
     relayout_work_free(r->rl_work);
     delete r;
 }
@@ -827,6 +834,63 @@ static void dirty_pages(const yx_renderer* r, const yx_scene* s, std::vector<std
     if (bytes) *bytes = b;
 }
 
+// One dirty page: `nodes` VoxNodes from staging node `src` to scene node `dst`.
+struct PageCopy {
+    uint32_t dst, src, nodes, pad;
+};
+
+// One workgroup per page, 8-byte words (node offsets are multiples of 40 B).
+__global__ __launch_bounds__(256) void page_scatter_kernel(const PageCopy* tab, const uint2* src, uint2* dst) {
+    const PageCopy c = tab[blockIdx.x];
+    const uint2* s = src + (size_t)c.src * 5;
+    uint2* d = dst + (size_t)c.dst * 5;
+    for (uint32_t w = threadIdx.x; w < c.nodes * 5u; w += 256) d[w] = s[w];
+}
+
+// Many scattered runs: gather the pages into pinned memory, send them with
+// one copy and scatter them on the GPU. Returns YX_OK once the copy is queued
+// (the caller synchronises before the staging is reused).
+static int upload_scattered(yx_renderer* r, const yx_scene* s, const std::vector<std::pair<size_t, size_t>>& runs,
+                            uint32_t pages, uint64_t bytes, hipStream_t st) {
+    const size_t tab_bytes = ((size_t)pages * sizeof(PageCopy) + 255) & ~(size_t)255;
+    const size_t need = tab_bytes + bytes;
+    if (need > r->stage_cap) {
+        const size_t cap = std::max(need, r->stage_cap + r->stage_cap / 2);
+        if (r->stage_host) YX_HIP(hipHostFree(r->stage_host));
+        if (r->d_stage) YX_HIP(hipFree(r->d_stage));
+        r->stage_host = nullptr;
+        r->d_stage = nullptr;
+        r->stage_cap = 0;
+        if (hipHostMalloc((void**)&r->stage_host, cap, hipHostMallocDefault) != hipSuccess) {
+            r->stage_host = nullptr;
+            return fail(YX_ERR_OOM, "hipHostMalloc(%zu) for page staging", cap);
+        }
+        if (hipMalloc((void**)&r->d_stage, cap) != hipSuccess) {
+            r->d_stage = nullptr;
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) for page staging", cap);
+        }
+        r->stage_cap = cap;
+    }
+    PageCopy* tab = reinterpret_cast<PageCopy*>(r->stage_host);
+    uint8_t* data = r->stage_host + tab_bytes;
+    uint32_t k = 0;
+    size_t at = 0;  // nodes staged so far
+    for (const auto& run : runs)
+        for (size_t n0 = run.first; n0 < run.second; n0 += kScenePageNodes) {
+            const size_t n1 = std::min(run.second, n0 + kScenePageNodes);
+            tab[k++] = PageCopy{(uint32_t)n0, (uint32_t)at, (uint32_t)(n1 - n0), 0u};
+            std::memcpy(data + at * 40, s->nodes.data() + n0 * 10, (n1 - n0) * 40);
+            at += n1 - n0;
+        }
+    if (k != pages || at * 40 != bytes) return fail(YX_ERR_HIP, "page staging: %u of %u pages", k, pages);
+    YX_HIP(hipMemcpyAsync(r->d_stage, r->stage_host, need, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(page_scatter_kernel, dim3(pages), dim3(256), 0, st,
+                       reinterpret_cast<const PageCopy*>(r->d_stage),
+                       reinterpret_cast<const uint2*>(r->d_stage + tab_bytes), reinterpret_cast<uint2*>(r->d_vox));
+    YX_HIP(hipGetLastError());
+    return YX_OK;
+}
+
 extern "C" int yx_renderer_count_changed_pages(const yx_renderer* r, const yx_scene* s, uint32_t* pages,
                                                uint64_t* bytes) {
     clear_error();
@@ -866,9 +930,13 @@ extern "C" int yx_renderer_update_scene(yx_renderer* r, const yx_scene* s, void*
     }
     // until the copy below completes the device copy is in no known state
     r->mirror_uid = 0;
-    for (const auto& run : runs)
-        YX_HIP(hipMemcpyAsync(r->d_vox + run.first * 10, s->nodes.data() + run.first * 10,
-                              (run.second - run.first) * 40u, hipMemcpyHostToDevice, st));
+    if (runs.size() >= 8 && bytes <= (64ull << 20)) {
+        if ((rc = upload_scattered(r, s, runs, pages, bytes, st)) != YX_OK) return rc;
+    } else {
+        for (const auto& run : runs)
+            YX_HIP(hipMemcpyAsync(r->d_vox + run.first * 10, s->nodes.data() + run.first * 10,
+                                  (run.second - run.first) * 40u, hipMemcpyHostToDevice, st));
+    }
     // the copies read the scene's host memory: complete them before returning
     YX_HIP(hipStreamSynchronize(st));
     r->vox_n = nn;

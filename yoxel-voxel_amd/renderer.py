@@ -93,7 +93,30 @@ class SVOData:
     def set_root(self, root):
         check(lib().yx_scene_set_root(self._h, int(root)))
 
-    # -- accessors ---------------------------------------------------------
+    GROW = _lib.YX_BUILD_GROW
+    CLEAR = _lib.YX_BUILD_CLEAR
+    IN_PLACE = _lib.YX_RANGE_IN_PLACE
+    REBUILD = _lib.YX_RANGE_REBUILD
+    REBUILD_GPU = _lib.YX_RANGE_REBUILD_GPU
+
+    def BuildRange(self, level, pos, mode, radius, color=0xFFC0B6C0, how=IN_PLACE, stream=None):
+        """DynamicSVO::BuildRange(level, pos, mode, SphereSource(radius, color))
+        (ore/src/main.cpp:121-127, demo/Demo.cpp:82-114) on a synthetic scene:
+        GROW unions, CLEAR subtracts a ball of `radius` voxels of `level` at
+        `pos` (voxels of `level`). `how`: IN_PLACE edits the node array keeping
+        untouched nodes' ids, REBUILD / REBUILD_GPU rebuild the canonical tree
+        on the host / the GPU."""
+        self.BuildRanges([(mode, level, pos, radius, color)], how, stream)
+
+    def BuildRanges(self, edits, how=IN_PLACE, stream=None):
+        """Several edits [(mode, level, pos, radius, color)] in one call, applied
+        in order (Demo::DoEdit makes 100 per frame, demo/Demo.cpp:82-114)."""
+        arr = (_lib.SphereEdit * max(1, len(edits)))()
+        for k, (mode, level, pos, radius, color) in enumerate(edits):
+            arr[k] = _lib.SphereEdit(int(mode), int(level), (ctypes.c_float * 3)(*pos), float(radius), int(color))
+        st = ctypes.c_void_p(_stream(stream) if how == _lib.YX_RANGE_REBUILD_GPU else 0)
+        check(lib().yx_scene_build_range(self._h, arr, len(edits), int(how), st))
+
     def info(self):
         i = _lib.SceneInfo()
         check(lib().yx_scene_get_info(self._h, ctypes.byref(i)))
