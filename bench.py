@@ -43,7 +43,9 @@ def parse():
     ap.add_argument("--no-pipeline", action="store_true",
                     help="skip the shading-pipeline measurement (LOD trace, SSNA blur, Phong shade)")
     ap.add_argument("--no-update", action="store_true", help="skip the scene-sync measurement")
-    ap.add_argument("--only", choices=["composite", "batch", "brick", "plain", "pipeline", "update", "pool"],
+    ap.add_argument("--no-voxtree", action="store_true", help="skip the brick-classification measurement")
+    ap.add_argument("--only", choices=["composite", "batch", "brick", "plain", "pipeline", "update", "pool",
+                                       "voxtree"],
                     default=None,
                     help="run one secondary workload alone and print its object (profiling)")
     ap.add_argument("--brick-frames", type=int, default=8)
@@ -81,6 +83,8 @@ def main():
             leg = batch_bench(yx, torch, args.batch_frames, args.batch_tune_cache)
         elif args.only == "pool":
             leg = pool_bench(yx, torch, dev)
+        elif args.only == "voxtree":
+            leg = voxtree_bench(yx, torch, not args.no_cpu_baseline)
         elif args.only == "update":
             leg = update_bench(yx, torch, yx.SVOData.build_synthetic_gpu(**wl.LARGE_1080P))
         elif args.only == "pipeline":
@@ -188,6 +192,7 @@ def main():
                 else pipeline_bench(yx, torch, scene, cams, not args.no_cpu_baseline))
     update = None if args.no_update or args.workload != "large_1080p" else update_bench(yx, torch, scene)
     pool = None if args.no_composite else pool_bench(yx, torch, dev)
+    voxtree = None if args.no_voxtree else voxtree_bench(yx, torch, not args.no_cpu_baseline)
 
     line = {
         "metric": BASELINE_METRIC,
@@ -221,6 +226,7 @@ def main():
         "pipeline": pipeline,
         "update": update,
         "brick_pool": pool,
+        "voxtree": voxtree,
     }
     print(json.dumps(line), flush=True)
 
@@ -551,6 +557,74 @@ def edit_bench(yx, torch, wall, set_scene_ms, reps=3):
             "cpu_rebuild_set_scene_ms": round(cpu_rebuild_ms + set_scene_ms, 1),
             "speedup_vs_cpu_rebuild_set_scene": round((cpu_rebuild_ms + set_scene_ms) / total, 1),
             "nodes": s.info()["num_nodes"]}
+
+
+def voxtree_bench(yx, torch, with_cpu=True, reps=20):
+    """Brick classification of the reference's 1024^3 bit volume
+    (reaction/voxtree/voxtree.py:80-95): CalcDensity, MarkBricks and PackBricks
+    each timed with HIP events over `reps` launches on the current stream, and
+    the whole classify call (with the column scan and the total read back) by
+    wall clock. Algorithmic bytes: density reads B^3/8 and writes V^3/2 bytes;
+    marking reads the V^3/2 density bytes once and writes 4 G^3 + 4 G^2; packing
+    reads 4 G^3 + 4 G^2 and writes 4 per mixed brick."""
+    cfg = yx.workloads.VOXTREE_1024
+    B = cfg["bit_size"]
+    V, G = B // 2, B // 8
+    vol = yx.workloads.voxtree_volume(B, cfg["seed"])
+    bits = yx.voxtree.pack_bits(vol)
+    del vol
+    stream = torch.cuda.current_stream()
+    r = yx.voxtree.classify(bits, B)
+    start = r["column_start"][:-1].contiguous()
+    total = r["total"]
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    d_out = torch.empty_like(r["density"])
+    m_out = (torch.empty_like(r["brick_state"]), torch.empty_like(r["column_count"]))
+    p_out = torch.empty(max(1, total), dtype=torch.int32, device=bits.device)
+    d_ms = timed(lambda: yx.voxtree.calc_density(bits, B, out=d_out))
+    m_ms = timed(lambda: yx.voxtree.mark_bricks(r["density"], V, out=m_out))
+    p_ms = timed(lambda: yx.voxtree.pack_bricks(r["brick_state"], start, V, total, out=p_out))
+    walls = []
+    for _ in range(5):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        yx.voxtree.classify(bits, B)
+        torch.cuda.synchronize()
+        walls.append((time.perf_counter() - t0) * 1e3)
+    d_bytes = B ** 3 // 8 + V ** 3 // 2
+    m_bytes = V ** 3 // 2 + 4 * G ** 3 + 4 * G * G
+    p_bytes = 4 * G ** 3 + 4 * G * G + 4 * total
+
+    def rf(nbytes, ms):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"ms": round(ms, 4), "bytes": nbytes, "achieved_gbs": round(gbs, 1),
+                "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+    out = {"workload": f"voxtree_1024: {B}^3 bit volume, {G}^3 bricks of 4^3 (5^3 with overlap)",
+           "mixed_bricks": total, "calc_density": rf(d_bytes, d_ms), "mark_bricks": rf(m_bytes, m_ms),
+           "pack_bricks": rf(p_bytes, p_ms), "classify_wall_ms": round(min(walls), 3),
+           "bricks_per_s": round(G ** 3 / (min(walls) * 1e-3), 1)}
+    if with_cpu:
+        orc = entry.load_oracle().VoxtreeOracle()
+        words = bits.cpu().numpy().view(np.uint32)
+        t0 = time.perf_counter()
+        exp = orc.classify(words, B)
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(G ** 3 / cpu_s, 1), "unit": "bricks/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole {B}^3 volume through the numpy restatement, {cpu_s:.1f} s"}
+        out["cpu_total_matches"] = exp["total"] == total
+    return out
 
 
 def pool_bench(yx, torch, dev, reps=5):

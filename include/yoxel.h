@@ -451,6 +451,42 @@ int yx_brick_pool_release(yx_brick_pool* p, const int32_t* bricks, uint32_t n, u
 int yx_brick_pool_update(yx_brick_pool* p, const int32_t* bricks, uint32_t n, const float* packed, void* stream);
 int yx_brick_pool_stats(const yx_brick_pool* p, uint32_t* used, uint32_t* capacity);
 
+/* ---- brick classification of a bit volume (reaction/voxtree) ---------------
+ * The voxtree pipeline (reaction/voxtree/voxtree.py:80-95): 2x2x2 density of
+ * a bit volume, uniform / mixed classification of 4^3 bricks with one voxel of
+ * overlap, and the packed list of mixed bricks. All pointers are device
+ * pointers; B = bit volume side (a power of two, 128..1024), V = B / 2,
+ * G = V / 4.
+ *   bits         uint32 [B/128][B (y)][B (x)][4]: bit b of word l of slice s
+ *                is voxel z = 128 s + 32 l + b (the voxelizer's layout)
+ *   density      uint32 [V (y)][V (x)][V/8]: 4-bit counts 0..8, word w of a
+ *                column holds z = 8w..8w+7 from the low nibble up
+ *   brick_state  uint32 [G (y)][G (x)][G (z)]: 0x40000000 empty, 0x80000000
+ *                solid (over the brick's 5^3 counts; counts past the volume
+ *                are 0), ORed with the number of mixed bricks below it in its
+ *                column
+ *   column_count uint32 [G (y)][G (x)]: mixed bricks per column
+ *   column_start uint32 exclusive prefix sum of column_count
+ *   packed       uint32 per mixed brick: x | y << 8 | z << 16, ordered by
+ *                column, then z
+ * bits, density, brick_state, column_count and column_start must be 16-byte
+ * aligned (YX_ERR_INVALID_ARG). */
+/* CalcDensity (reaction/voxtree/density.cu:32-60). */
+int yx_voxtree_calc_density(const uint32_t* bits, int32_t bit_size, uint32_t* density, void* stream);
+/* MarkBricks (reaction/voxtree/density.cu:131-176); vol_size = V. */
+int yx_voxtree_mark_bricks(const uint32_t* density, int32_t vol_size, uint32_t* brick_state,
+                           uint32_t* column_count, void* stream);
+/* PackBricks (reaction/voxtree/density.cu:210-221); writes below `capacity`. */
+int yx_voxtree_pack_bricks(const uint32_t* brick_state, const uint32_t* column_start, int32_t vol_size,
+                           uint32_t* packed, uint32_t capacity, void* stream);
+/* All three and the column scan that voxtree.py:88-89 does on the host
+ * (cumsum_ex), on the GPU. column_start holds G*G + 1 entries, the last the
+ * total; *num_packed (host) receives it after a synchronisation of `stream`.
+ * YX_ERR_RANGE, with nothing packed, when it exceeds `capacity`. */
+int yx_voxtree_classify(const uint32_t* bits, int32_t bit_size, uint32_t* density, uint32_t* brick_state,
+                        uint32_t* column_count, uint32_t* column_start, uint32_t* packed, uint32_t capacity,
+                        uint32_t* num_packed, void* stream);
+
 /* ---- brick n-tree volumes ---------------------------------------------------
  * The n-tree of ntree/ntree/nodes.h:13-100 (Grid nodes of 4^3 cells with the
  * ch2ofs cell order (z*4 + y)*4 + x, :99; Brick leaves, ntree/BrickManager.h

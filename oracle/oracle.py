@@ -562,3 +562,102 @@ class BrickPoolOracle:
             s = self.table[b]
             if s >= 0:
                 self.params[s * self.vox:(s + 1) * self.vox] = packed[i]
+
+
+# ---- brick classification of a bit volume (reaction/voxtree) ---------------
+
+VT_SOLID = 0x80000000
+VT_EMPTY = 0x40000000
+
+
+class VoxtreeOracle:
+    """numpy restatement of the voxtree pipeline (reaction/voxtree/voxtree.py:80-95):
+    CalcDensity (reaction/voxtree/density.cu:32-60), MarkBricks (:131-176),
+    the host cumsum_ex of the column counts (voxtree.py:12-18, 88-89) and
+    PackBricks (:210-221). It works on dense arrays: bits are unpacked to a
+    boolean volume, summed over 2x2x2 blocks, and bricks are classified from
+    min / max over their 5^3 windows. `window` and `nibble_order` exist so the
+    tests can plant mistakes."""
+
+    window = 5            # a brick's counts per axis: 4 + one of overlap
+    nibble_order = None   # None: nibble n of a density word is z = 8w + n
+
+    @staticmethod
+    def pack_bits(vol):
+        """bool [x, y, z] of side B -> uint32 words [B/128][B (y)][B (x)][4]."""
+        B = vol.shape[0]
+        v = np.ascontiguousarray(np.transpose(vol.reshape(B, B, B // 128, 4, 32), (2, 1, 0, 3, 4)))
+        return np.packbits(v, axis=-1, bitorder="little").view(np.uint32).reshape(-1)
+
+    @staticmethod
+    def unpack_bits(words, B):
+        w = np.ascontiguousarray(words, dtype=np.uint32).reshape(B // 128, B, B, 4)
+        bits = np.unpackbits(w.view(np.uint8).reshape(B // 128, B, B, 4, 4), axis=-1, bitorder="little")
+        bits = bits.reshape(B // 128, B, B, 4, 32)  # [s][y][x][layer][bit]
+        return np.transpose(bits, (2, 1, 0, 3, 4)).reshape(B, B, B).astype(bool)
+
+    def counts(self, words, B):
+        """2x2x2 block sums (CalcDensity) as uint8 [x, y, z] of side B / 2."""
+        v = self.unpack_bits(words, B).astype(np.uint8)
+        V = B // 2
+        return v.reshape(V, 2, V, 2, V, 2).sum(axis=(1, 3, 5), dtype=np.uint8)
+
+    def pack_density(self, c):
+        """uint8 counts [x, y, z] -> density words [V (y)][V (x)][V/8]."""
+        V = c.shape[0]
+        n = np.transpose(c, (1, 0, 2)).reshape(V, V, V // 8, 8).astype(np.uint32)
+        order = self.nibble_order if self.nibble_order is not None else range(8)
+        out = np.zeros((V, V, V // 8), np.uint32)
+        for k, src in enumerate(order):
+            out |= n[..., src] << np.uint32(4 * k)
+        return out.reshape(-1)
+
+    @staticmethod
+    def unpack_density(dens, V):
+        d = np.ascontiguousarray(dens, dtype=np.uint32).reshape(V, V, V // 8, 1)
+        nib = (d >> (np.arange(8, dtype=np.uint32) * 4)) & 0xF  # [y][x][w][n]
+        return np.transpose(nib.reshape(V, V, V).astype(np.uint8), (1, 0, 2))  # [x, y, z]
+
+    def density(self, words, B):
+        return self.pack_density(self.counts(words, B))
+
+    def mark(self, dens, V):
+        """(brick_state uint32 [G*G*G], column_count uint32 [G*G]) from density words."""
+        c = self.unpack_density(dens, V)
+        G = V // 4
+        k = self.window
+        pad = np.zeros((V + k, V + k, V + k), np.uint8)  # counts past the volume are 0
+        pad[:V, :V, :V] = c
+        win = np.lib.stride_tricks.sliding_window_view(pad, (k, k, k))[::4, ::4, ::4][:G, :G, :G]
+        lo = win.min(axis=(3, 4, 5))
+        hi = win.max(axis=(3, 4, 5))
+        st = np.where(lo == 8, VT_SOLID, np.where(hi == 0, VT_EMPTY, 0)).astype(np.uint32)  # [bx, by, bz]
+        st = np.transpose(st, (1, 0, 2)).reshape(G * G, G)  # [col = bx + by G][bz]
+        mixed = (st == 0).astype(np.uint32)
+        below = np.cumsum(mixed, axis=1, dtype=np.uint32) - mixed
+        return (st | below).reshape(-1), mixed.sum(axis=1, dtype=np.uint32)
+
+    @staticmethod
+    def column_start(col_count):
+        """Exclusive prefix sum and total (cumsum_ex, voxtree.py:12-18)."""
+        c = np.asarray(col_count, np.uint64)
+        s = np.cumsum(c) - c
+        return s.astype(np.uint32), int(c.sum())
+
+    @staticmethod
+    def pack(state, col_start, V):
+        G = V // 4
+        st = np.asarray(state, np.uint32).reshape(G * G, G)
+        col, z = np.nonzero((st & np.uint32(VT_SOLID | VT_EMPTY)) == 0)
+        idx = col_start[col] + st[col, z]
+        out = np.zeros(len(col), np.uint32)
+        out[idx] = (col % G).astype(np.uint32) | ((col // G).astype(np.uint32) << 8) | (z.astype(np.uint32) << 16)
+        return out
+
+    def classify(self, words, B):
+        V = B // 2
+        dens = self.density(words, B)
+        state, cnt = self.mark(dens, V)
+        start, total = self.column_start(cnt)
+        return dict(density=dens, brick_state=state, column_count=cnt, column_start=start, total=total,
+                    packed=self.pack(state, start, V))

@@ -10,7 +10,8 @@ from .batch import BatchRenderer
 from .ntree import BrickTree
 from . import workloads
 from . import composite
+from . import voxtree
 
-__all__ = ["SVOData", "SVORenderer", "BatchRenderer", "BrickTree", "YoxelError", "lib", "workloads", "composite", "LIB_PATH",
+__all__ = ["SVOData", "SVORenderer", "BatchRenderer", "BrickTree", "YoxelError", "lib", "workloads", "composite", "voxtree", "LIB_PATH",
            "YX_EMPTY_NODE", "YX_FULL_NODE", "YX_MISS", "YX_LIGHT_HEADLIGHT", "YX_LIGHT_SUN",
            "YX_LIGHT_SUN_SHADOW"]

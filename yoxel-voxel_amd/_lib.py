@@ -237,6 +237,11 @@ SIGNATURES = {
     "yx_brick_pool_release": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32), _P]),
     "yx_brick_pool_update": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, _P]),
     "yx_brick_pool_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+    "yx_voxtree_calc_density": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P]),
+    "yx_voxtree_mark_bricks": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P]),
+    "yx_voxtree_pack_bricks": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, ctypes.c_uint32, _P]),
+    "yx_voxtree_classify": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, ctypes.c_uint32,
+                                           ctypes.POINTER(ctypes.c_uint32), _P]),
     "yx_scene_build_range": (ctypes.c_int, [_P, ctypes.POINTER(SphereEdit), ctypes.c_uint32, ctypes.c_int32, _P]),
     "yx_scene_write_nodes": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
     "yx_scene_set_root": (ctypes.c_int, [_P, ctypes.c_uint32]),

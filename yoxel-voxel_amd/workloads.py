@@ -104,3 +104,24 @@ def brick_palette():
          | (a.astype(np.uint32) << 24))
     p[0] = 0
     return p.astype(np.uint32)
+
+
+# brick classification workload: the reference's 1024^3 bit volume (voxtree.py:75)
+VOXTREE_1024 = dict(bit_size=1024, seed=1024)
+
+
+def voxtree_volume(B, seed, dev="cuda", flip=2e-5):
+    """Synthetic bit volume (bool [x, y, z] of side B on `dev`): 16 spheres of
+    radius B/32..B/32+B/6 united with a wavy floor z < B (0.3 + 0.1 sin cos),
+    then `flip` of the voxels flipped (scattered voxels and holes)."""
+    import torch
+    g = torch.Generator(device=dev).manual_seed(seed)
+    c = torch.arange(B, device=dev, dtype=torch.float32) + 0.5
+    X, Y, Z = c[:, None, None], c[None, :, None], c[None, None, :]
+    vol = Z < B * (0.3 + 0.1 * torch.sin(X / B * 6.0) * torch.cos(Y / B * 5.0))
+    for _ in range(16):
+        p = torch.rand(3, generator=g, device=dev) * B
+        r = float(torch.rand(1, generator=g, device=dev)) * B / 6 + B / 32
+        vol |= (X - p[0]) ** 2 + (Y - p[1]) ** 2 + (Z - p[2]) ** 2 <= r * r
+    vol ^= torch.rand((B, B, B), generator=g, device=dev) < flip
+    return vol
