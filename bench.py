@@ -546,6 +546,15 @@ def edit_bench(yx, torch, wall, set_scene_ms, reps=3):
     gpu_pages = r.last_update()[0]
     cs = yx.SVOData.build_synthetic(**wl.LARGE_1080P)
     cpu_rebuild_ms = wall(lambda: cs.BuildRanges(b, how=yx.SVOData.REBUILD))
+    # an IsoSource of 128^3 values (scene_gen.py:111-130's bonsai is 256^3) grown at level 11
+    blob = wl.iso_blob((128, 128, 128), 7)
+    iso_place = wall(lambda: s.BuildRangeIso(11, (900, 900, 1100), 0, blob, 90, color=0xFF30A050))
+    iso_upd = wall(lambda: r.UpdateScene(s))
+    iso_pages = r.last_update()[0]
+    iso_gpu = wall(lambda: s.BuildRangeIso(11, (300, 1400, 1150), 0, blob, 90, color=0xFF30A050,
+                                           how=yx.SVOData.REBUILD_GPU))
+    iso_cpu = wall(lambda: cs.BuildRangeIso(11, (900, 900, 1100), 0, blob, 90, color=0xFF30A050,
+                                            how=yx.SVOData.REBUILD))
     k = int(np.argmin([a + u for a, u in zip(edit_ms, upd_ms)]))
     total = edit_ms[k] + upd_ms[k]
     return {"workload": "large_1080p, 100 sphere edits (radius 4, level 11) per batch",
@@ -556,6 +565,9 @@ def edit_bench(yx, torch, wall, set_scene_ms, reps=3):
             "cpu_rebuild_ms": round(cpu_rebuild_ms, 1),
             "cpu_rebuild_set_scene_ms": round(cpu_rebuild_ms + set_scene_ms, 1),
             "speedup_vs_cpu_rebuild_set_scene": round((cpu_rebuild_ms + set_scene_ms) / total, 1),
+            "iso_source_128": {"in_place_ms": round(iso_place, 1), "update_ms": round(iso_upd, 2),
+                               "update_pages": iso_pages, "gpu_rebuild_ms": round(iso_gpu, 1),
+                               "cpu_rebuild_ms": round(iso_cpu, 1)},
             "nodes": s.info()["num_nodes"]}
 
 

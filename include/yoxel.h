@@ -136,6 +136,30 @@ typedef struct yx_sphere_edit {
 } yx_sphere_edit;
 int yx_scene_build_range(yx_scene* scene, const yx_sphere_edit* edits, uint32_t count, int32_t how,
                          void* stream);
+/* BuildRange with a volume source: IsoSource (scene_gen.py:111-130, a uint8
+ * volume thresholded at an iso level, ore/src/main.cpp:53-63,71-75,115-119) or
+ * RawSource (scene_gen.py:74-82, RGBA8 voxels, ore/src/main.cpp:37-50,109-110). The
+ * source's voxel (a, b, c), a fastest in memory, covers voxel pos + (a, b, c)
+ * of `level` (2^(depth - level) finest voxels per axis); parts outside the
+ * domain are dropped. Solid voxels: IsoSource value >= iso (inside != 0) or
+ * value < iso (inside == 0), coloured `color`; RawSource alpha != 0, coloured
+ * by the voxel. GROW unions them, CLEAR subtracts them, in order with the
+ * sphere edits, by the same methods. The data are copied. */
+#define YX_SOURCE_ISO 1
+#define YX_SOURCE_RAW 2
+typedef struct yx_volume_source {
+    int32_t mode;     /* YX_BUILD_GROW or YX_BUILD_CLEAR                         */
+    int32_t level;    /* 1..depth                                                 */
+    int32_t pos[3];   /* where source voxel (0, 0, 0) lands, voxels of `level`    */
+    int32_t size[3];  /* source voxels per axis, each 1..4096                     */
+    int32_t kind;     /* YX_SOURCE_ISO: data = uint8 values; YX_SOURCE_RAW: data =
+                         RGBA8 (r low byte) per voxel                              */
+    int32_t iso;      /* IsoSource level (SetIsoLevel)                            */
+    int32_t inside;   /* IsoSource: 1 solid at or above iso, 0 below (SetInside)  */
+    uint32_t color;   /* IsoSource colour (SetColor)                              */
+    const void* data; /* host memory, size[0] * size[1] * size[2] voxels          */
+} yx_volume_source;
+int yx_scene_build_range_volume(yx_scene* scene, const yx_volume_source* source, int32_t how, void* stream);
 /* SVOData::GetRoot / operator[]: borrowed view of the node array (valid until
  * yx_scene_free). */
 int yx_scene_nodes(const yx_scene* scene, const uint32_t** nodes, uint32_t* num_nodes,

@@ -312,3 +312,83 @@ def edited_colors_np(xs, ys, zs, desc, edits):
         else:
             out[inb] = leaf_colors_np(morton3(xs[inb], ys[inb], zs[inb]), desc["seed"])
     return out
+
+
+def place_source(depth, level, pos, solid_zyx):
+    """Dense [x, y, z] mask of a volume source's solid voxels: source voxel
+    (a, b, c) covers voxels pos + (a, b, c) of `level`, 2^(depth - level)
+    finest voxels per axis; parts outside the domain are dropped."""
+    n = 1 << depth
+    s = 1 << (depth - level)
+    m = np.transpose(solid_zyx, (2, 1, 0))
+    m = m.repeat(s, 0).repeat(s, 1).repeat(s, 2)
+    out = np.zeros((n, n, n), bool)
+    lo = [p * s for p in pos]
+    src = [slice(max(0, -lo[a]), max(0, min(m.shape[a], n - lo[a]))) for a in range(3)]
+    dst = [slice(max(0, lo[a]), max(0, min(n, lo[a] + m.shape[a]))) for a in range(3)]
+    if all(d.stop > d.start for d in dst):
+        out[tuple(dst)] = m[tuple(src)]
+    return out
+
+
+def place_colors(depth, level, pos, col_zyx):
+    """Per-voxel colours of a volume source, placed as place_source does."""
+    n = 1 << depth
+    s = 1 << (depth - level)
+    m = np.transpose(col_zyx, (2, 1, 0))
+    m = m.repeat(s, 0).repeat(s, 1).repeat(s, 2)
+    out = np.zeros((n, n, n), np.uint32)
+    lo = [p * s for p in pos]
+    src = [slice(max(0, -lo[a]), max(0, min(m.shape[a], n - lo[a]))) for a in range(3)]
+    dst = [slice(max(0, lo[a]), max(0, min(n, lo[a] + m.shape[a]))) for a in range(3)]
+    if all(d.stop > d.start for d in dst):
+        out[tuple(dst)] = m[tuple(src)]
+    return out
+
+
+def source_mask(e):
+    """(solid [z][y][x], colours [z][y][x]) of a volume edit dict."""
+    if e["kind"] == "iso":
+        above = e["data"].astype(np.int64) >= e["iso"]
+        solid = above if e.get("inside", True) else ~above
+        return solid, np.full(e["data"].shape, e.get("color", 0xFFC0B6C0), np.uint32)
+    d = e["data"].astype(np.uint32)
+    return (d >> 24) != 0, d
+
+
+def mixed_inside_and_colors(desc, edits):
+    """Dense inside grid and leaf colours after mixed edits: spheres as
+    (mode, level, pos, radius, color) tuples, volume sources as dicts with kind
+    'iso' / 'raw', mode, level, pos, data [z][y][x] and (iso) iso, inside, color."""
+    depth = desc["depth"]
+    inside = synthetic_inside(**desc)
+    n = inside.shape[0]
+    c = np.arange(n, dtype=np.float64) + 0.5
+    X, Y, Z = np.meshgrid(c, c, c, indexing="ij")
+    last = np.full((n, n, n), -1, np.int64)    # last edit holding the voxel
+    colmap = np.zeros((n, n, n), np.uint32)
+    grow = np.zeros((n, n, n), bool)
+    for e in edits:
+        if isinstance(e, dict):
+            solid, cols = source_mask(e)
+            m = place_source(depth, e["level"], e["pos"], solid)
+            col = place_colors(depth, e["level"], e["pos"], cols)
+            mode = e["mode"]
+        else:
+            mode, level, pos, radius, color = e
+            (_, (cx, cy, cz), r, _), = edit_balls(depth, [e])
+            m = (X - cx) ** 2 + (Y - cy) ** 2 + (Z - cz) ** 2 <= r * r
+            col = np.full((n, n, n), color, np.uint32)
+        inside = inside | m if mode == 0 else inside & ~m
+        colmap[m] = col[m]
+        grow[m] = mode == 0
+        last[m] = 1
+    return inside, colmap, grow, last >= 0
+
+
+def mixed_leaf_colors(desc, edits, xs, ys, zs):
+    inside, colmap, grow, touched = mixed_inside_and_colors(desc, edits)
+    out = leaf_colors_np(morton3(xs, ys, zs), desc["seed"])
+    sel = touched[xs, ys, zs] & grow[xs, ys, zs]
+    out[sel] = colmap[xs[sel], ys[sel], zs[sel]]
+    return out

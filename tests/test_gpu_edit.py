@@ -99,3 +99,33 @@ def test_edit_versions_update_and_render(yx, cuda, orc):
                        light_mode=2, light=(0.4, 0.3, 0.85))
         assert np.array_equal(g["hit_id"].view(np.uint32), o["hit_id"])
         assert np.array_equal(g["rgba"].view(np.uint32), o["rgba"])
+
+
+def test_gpu_rebuild_with_volume_sources(yx, cuda):
+    """IsoSource / RawSource edits: the GPU rebuild equals the host rebuild byte
+    for byte, at depth 6 (the mixed sequence) and depth 9 (a 48^3 source at a
+    coarser level, scaled x2), and the in-place tree holds the same voxels."""
+    from test_scene_edit import MIXED, apply_mixed, blob, raw_block
+    c = yx.SVOData.build_synthetic(**DESC)
+    g = yx.SVOData.build_synthetic_gpu(**DESC)
+    for k, e in enumerate(MIXED):
+        print("mixed edit", k, flush=True)
+        apply_mixed(c, [e], yx.SVOData.REBUILD)
+        apply_mixed(g, [e], yx.SVOData.REBUILD_GPU)
+        assert c.info() == g.info()
+        assert np.array_equal(c.nodes(), g.nodes())
+    desc = dict(depth=9, num_spheres=40, radius_min=6.0, radius_max=30.0, heightfield=True, seed=17)
+    c = yx.SVOData.build_synthetic(**desc)
+    g = yx.SVOData.build_synthetic_gpu(**desc)
+    p = yx.SVOData.build_synthetic(**desc)
+    vol = [dict(kind="iso", mode=0, level=8, pos=(40, 60, 70), data=blob((48, 48, 48), 9), iso=70,
+                color=0xFF4080FF),
+           dict(kind="raw", mode=0, level=9, pos=(300, 310, 200), data=raw_block((40, 30, 20), 5)),
+           dict(kind="iso", mode=1, level=9, pos=(100, 120, 100), data=blob((64, 64, 64), 11), iso=50)]
+    for e in vol:
+        apply_mixed(c, [e], yx.SVOData.REBUILD)
+        apply_mixed(g, [e], yx.SVOData.REBUILD_GPU)
+        apply_mixed(p, [e], yx.SVOData.IN_PLACE)
+        assert np.array_equal(c.nodes(), g.nodes())
+    assert leaf_colors(p) == leaf_colors(c)
+    check_pool(p.nodes(), p.GetRoot())

@@ -257,3 +257,93 @@ def test_build_range_errors(yx):
     t = yx.SVOData.from_nodes(before, 0)
     with pytest.raises(yx.YoxelError):
         t.BuildRange(6, (1.0, 1.0, 1.0), 0, 1.0)
+
+
+def blob(shape, seed, scale=255):
+    """Smooth uint8 values [z][y][x] (a sum of Gaussians)."""
+    rng = np.random.default_rng(seed)
+    z, y, x = np.meshgrid(*[np.arange(k, dtype=np.float64) for k in shape], indexing="ij")
+    v = np.zeros(shape)
+    for _ in range(4):
+        c = rng.uniform(0, 1, 3) * np.array(shape)
+        r = rng.uniform(1.5, max(shape) / 2.5)
+        v += np.exp(-((z - c[0]) ** 2 + (y - c[1]) ** 2 + (x - c[2]) ** 2) / (2 * r * r))
+    return np.clip(v / v.max() * scale, 0, 255).astype(np.uint8)
+
+
+def raw_block(shape, seed):
+    rng = np.random.default_rng(seed)
+    b = blob(shape, seed + 50)
+    col = rng.integers(0, 2 ** 24, size=shape, dtype=np.uint64).astype(np.uint32)
+    return np.where(b > 90, col | np.uint32(0xFF000000), col & np.uint32(0x00FFFFFF)).astype(np.uint32)
+
+
+MIXED = [
+    dict(kind="iso", mode=0, level=6, pos=(30, 20, 36), data=blob((10, 14, 12), 1), iso=100, color=0xFF2020E0),
+    (0, 6, (20.0, 30.5, 40.25), 7.5, 0xFF0000FF),
+    dict(kind="raw", mode=0, level=5, pos=(-2, 25, 12), data=raw_block((6, 8, 7), 2)),       # scaled x2, crosses x = 0
+    dict(kind="iso", mode=1, level=6, pos=(10, 10, 4), data=blob((12, 12, 12), 3), iso=60),  # carve into the ground
+    dict(kind="iso", mode=0, level=6, pos=(55, 55, 50), data=blob((12, 12, 12), 4), iso=128, inside=False,
+         color=0xFF00FF00),                                                                  # below iso, past the corner
+    (1, 6, (36.0, 27.0, 40.0), 4.0, 0),
+]
+
+
+def apply_mixed(s, edits, how):
+    for e in edits:
+        if isinstance(e, dict):
+            if e["kind"] == "iso":
+                s.BuildRangeIso(e["level"], e["pos"], e["mode"], e["data"], e["iso"], e.get("inside", True),
+                                e.get("color", 0xFFC0B6C0), how=how)
+            else:
+                s.BuildRangeRaw(e["level"], e["pos"], e["mode"], e["data"], how=how)
+        else:
+            mode, level, pos, radius, color = e
+            s.BuildRange(level, pos, mode, radius, color, how=how)
+
+
+def check_mixed(yx, s, desc, edits):
+    nodes, root = s.nodes(), s.GetRoot()
+    got = H.walk_leaves(nodes, root)
+    inside, _, _, _ = H.mixed_inside_and_colors(desc, edits)
+    xs, ys, zs = np.nonzero(H.surface(inside))
+    depth = desc["depth"]
+    assert set(got) == {(depth, int(x), int(y), int(z)) for x, y, z in zip(xs, ys, zs)}
+    cols = H.mixed_leaf_colors(desc, edits, xs, ys, zs)
+    assert [got[(depth, int(x), int(y), int(z))][0] for x, y, z in zip(xs, ys, zs)] == cols.tolist()
+    check_pool(nodes, root)
+
+
+@pytest.mark.parametrize("how", ["IN_PLACE", "REBUILD"])
+def test_volume_sources_match_dense_definition(yx, how):
+    """BuildRange with IsoSource / RawSource (scene_gen.py:74-82,111-130): every
+    prefix of a mixed sequence equals the dense definition: the source's solid
+    voxels placed at pos of its level (scaled to the finest level, clipped to
+    the domain) united or removed in order with the sphere edits, the last
+    source holding a voxel giving its colour."""
+    how = getattr(yx.SVOData, how)
+    s = yx.SVOData.build_synthetic(**DESC)
+    for k in range(len(MIXED)):
+        apply_mixed(s, MIXED[k:k + 1], how)
+        check_mixed(yx, s, DESC, MIXED[:k + 1])
+    r = yx.SVOData.build_synthetic(**DESC)
+    apply_mixed(r, MIXED, yx.SVOData.REBUILD)
+    assert leaf_colors(s) == leaf_colors(r)
+
+
+def test_volume_source_errors(yx):
+    s = yx.SVOData.build_synthetic(**DESC)
+    before = s.nodes()
+    v = blob((4, 4, 4), 0)
+    with pytest.raises(yx.YoxelError):
+        s.BuildRangeIso(7, (0, 0, 0), 0, v, 10)       # deeper than the tree
+    with pytest.raises(yx.YoxelError):
+        s.BuildRangeIso(6, (0, 0, 0), 3, v, 10)       # unknown mode
+    with pytest.raises(yx.YoxelError):
+        s.BuildRangeIso(6, (0, 0, 0), 0, np.zeros((0, 4, 4), np.uint8), 10)
+    with pytest.raises(yx.YoxelError):
+        s.BuildRangeIso(6, (1 << 25, 0, 0), 0, v, 10)
+    assert np.array_equal(s.nodes(), before)
+    # a source entirely outside the domain changes nothing
+    s.BuildRangeIso(6, (-10, 0, 0), 0, v, 0)
+    assert np.array_equal(s.nodes(), before)

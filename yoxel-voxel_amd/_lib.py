@@ -106,6 +106,25 @@ YX_RANGE_REBUILD = 1
 YX_RANGE_REBUILD_GPU = 2
 
 
+YX_SOURCE_ISO = 1
+YX_SOURCE_RAW = 2
+
+
+class VolumeSource(ctypes.Structure):
+    """yx_volume_source (BuildRange with an IsoSource / RawSource)."""
+    _fields_ = [
+        ("mode", ctypes.c_int32),
+        ("level", ctypes.c_int32),
+        ("pos", ctypes.c_int32 * 3),
+        ("size", ctypes.c_int32 * 3),
+        ("kind", ctypes.c_int32),
+        ("iso", ctypes.c_int32),
+        ("inside", ctypes.c_int32),
+        ("color", ctypes.c_uint32),
+        ("data", ctypes.c_void_p),
+    ]
+
+
 class SphereEdit(ctypes.Structure):
     """yx_sphere_edit (BuildRange with a SphereSource)."""
     _fields_ = [
@@ -242,6 +261,7 @@ SIGNATURES = {
     "yx_voxtree_pack_bricks": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, ctypes.c_uint32, _P]),
     "yx_voxtree_classify": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, ctypes.c_uint32,
                                            ctypes.POINTER(ctypes.c_uint32), _P]),
+    "yx_scene_build_range_volume": (ctypes.c_int, [_P, ctypes.POINTER(VolumeSource), ctypes.c_int32, _P]),
     "yx_scene_build_range": (ctypes.c_int, [_P, ctypes.POINTER(SphereEdit), ctypes.c_uint32, ctypes.c_int32, _P]),
     "yx_scene_write_nodes": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),
     "yx_scene_set_root": (ctypes.c_int, [_P, ctypes.c_uint32]),

@@ -8,6 +8,49 @@
 
 #include "yoxel.h"
 
+namespace yx {
+
+// An edit in voxel units of the finest level (BuildRange with a source): GROW
+// unions the source's solid voxels, CLEAR subtracts them.
+// kind 0, SphereSource: the ball of voxel centres within r of (cx, cy, cz);
+//   `color` paints grown voxels.
+// kind 1, a volume source (IsoSource / RawSource): source voxel (a, b, c)
+//   covers finest voxels vx + a s .. vx + (a + 1) s - 1 (and so on per axis);
+//   it is solid where EditSet::vsolid[off + a + sx (b + sy c)] is set, and
+//   grown voxels take EditSet::vcol at the same index.
+struct EditSphere {
+    int mode;
+    double cx, cy, cz, r;
+    uint32_t color;
+    int xlo, xhi, ylo, yhi;  // columns the edit can touch
+    int zlo, zhi;            // and its z range
+    int kind;
+    int vx, vy, vz, s;
+    int sx, sy, sz;
+    uint64_t off;
+};
+// Edits in order and, per cell of 64 x 64 columns of the (x, y) plane, the ids
+// of the edits whose column range meets the cell, ascending (CSR form). A
+// column or voxel only ever needs its cell's edits.
+struct EditSet {
+    static constexpr int kCellShift = 6;
+    std::vector<EditSphere> edits;
+    std::vector<uint32_t> vcol;   // volume sources' voxel colours
+    std::vector<uint8_t> vsolid;  // and solid flags
+    int cells = 0;                   // cells per axis
+    std::vector<uint32_t> cell_off;  // cells * cells + 1
+    std::vector<uint32_t> cell_ids;
+    void index(int n);
+    bool empty() const { return edits.empty(); }
+    const uint32_t* begin(int i, int j) const {
+        return cell_ids.data() + cell_off[(size_t)(j >> kCellShift) * cells + (size_t)(i >> kCellShift)];
+    }
+    const uint32_t* end(int i, int j) const {
+        return cell_ids.data() + cell_off[(size_t)(j >> kCellShift) * cells + (size_t)(i >> kCellShift) + 1];
+    }
+};
+}  // namespace yx
+
 // Host copy of an octree in the reference's VoxNode layout
 // (reaction/report/main.tex:36-56): 10 u32 per node, canonical ids are indices.
 // Edits (yx_scene_write_nodes / yx_scene_set_root) bump `version` and stamp
@@ -27,11 +70,11 @@ struct yx_scene {
     uint64_t version = 1;                  // 1 at creation, +1 per edit
     std::vector<uint64_t> page_version;    // per page; pages past the end: 1
     uint64_t page_ver(size_t p) const { return p < page_version.size() ? page_version[p] : 1u; }
-    // synthetic scenes: the recipe and the sphere edits applied so far
-    // (rebuilt by yx_scene_build_range); cleared by node writes
+    // synthetic scenes: the recipe and the edits applied so far (sphere and
+    // volume sources, yx_scene_build_range*); cleared by node writes
     bool has_recipe = false;
     yx_scene_desc recipe{};
-    std::vector<yx_sphere_edit> edits;
+    yx::EditSet edits;
     std::vector<uint32_t> free_nodes;      // blanked node ids, a min-heap (in-place edits)
     std::vector<int32_t> hf_top;           // heightfield top per column, filled on demand
 };
@@ -42,36 +85,12 @@ namespace yx {
 // per-ray traversal stack.
 constexpr int kMaxDepth = 16;
 
-// A sphere edit in voxel units of the finest level (BuildRange with a
-// SphereSource): GROW unions the ball of voxel centres within r of the centre,
-// CLEAR subtracts it; `color` paints grown voxels.
-struct EditSphere {
-    int mode;
-    double cx, cy, cz, r;
-    uint32_t color;
-    int xlo, xhi, ylo, yhi;  // columns the edit can touch
-};
-// Edits in order and, per cell of 64 x 64 columns of the (x, y) plane, the ids
-// of the edits whose column range meets the cell, ascending (CSR form). A
-// column or voxel only ever needs its cell's edits.
-struct EditSet {
-    static constexpr int kCellShift = 6;
-    std::vector<EditSphere> edits;
-    int cells = 0;                   // cells per axis
-    std::vector<uint32_t> cell_off;  // cells * cells + 1
-    std::vector<uint32_t> cell_ids;
-    void index(int n);
-    bool empty() const { return edits.empty(); }
-    const uint32_t* begin(int i, int j) const {
-        return cell_ids.data() + cell_off[(size_t)(j >> kCellShift) * cells + (size_t)(i >> kCellShift)];
-    }
-    const uint32_t* end(int i, int j) const {
-        return cell_ids.data() + cell_off[(size_t)(j >> kCellShift) * cells + (size_t)(i >> kCellShift) + 1];
-    }
-};
-// Converts edits to the finest level of a depth-D grid and indexes them;
-// YX_ERR_INVALID_ARG for a bad mode, level or radius.
-int edit_spheres(const yx_sphere_edit* e, size_t n, int depth, EditSet& out);
+// Appends sphere edits, converted to the finest level of a depth-D grid, to
+// `out` (not re-indexed); YX_ERR_INVALID_ARG for a bad mode, level or radius,
+// with `out` unchanged.
+int add_sphere_edits(const yx_sphere_edit* e, size_t n, int depth, EditSet& out);
+// Appends a volume source (IsoSource / RawSource) the same way.
+int add_volume_edit(const yx_volume_source& v, int depth, EditSet& out);
 // The synthetic scene of a recipe with edits, built on the GPU (svo_build.hip).
 int gpu_build_synthetic(const yx_scene_desc& d, const EditSet& edits, yx_scene& out, void* stream);
 

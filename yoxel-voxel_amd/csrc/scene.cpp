@@ -175,14 +175,32 @@ static inline uint32_t compact3(uint64_t v) {
     return (uint32_t)v;
 }
 
-static inline bool in_ball(const EditSphere& e, uint32_t i, uint32_t j, uint32_t k) {
-    double dx = ((double)i + 0.5) - e.cx;
-    double dy = ((double)j + 0.5) - e.cy;
-    double dz = ((double)k + 0.5) - e.cz;
-    return dx * dx + dy * dy + dz * dz <= e.r * e.r;
+// Source voxel index of finest voxel (i, j, k) in a volume edit, or -1.
+static inline int64_t vol_index(const EditSphere& e, int i, int j, int k) {
+    if (i < e.vx || j < e.vy || k < e.vz) return -1;
+    const int a = (i - e.vx) / e.s, b = (j - e.vy) / e.s, c = (k - e.vz) / e.s;
+    if (a >= e.sx || b >= e.sy || c >= e.sz) return -1;
+    return (int64_t)e.off + a + (int64_t)e.sx * (b + (int64_t)e.sy * c);
 }
 
-// Leaf colour: the last edit whose ball holds the voxel decides (a GROW's
+// Whether the edit's source holds finest voxel (i, j, k); for a volume
+// source, its colour in *col.
+static inline bool in_edit(const EditSet& set, const EditSphere& e, uint32_t i, uint32_t j, uint32_t k,
+                           uint32_t* col) {
+    if (e.kind == 0) {
+        double dx = ((double)i + 0.5) - e.cx;
+        double dy = ((double)j + 0.5) - e.cy;
+        double dz = ((double)k + 0.5) - e.cz;
+        *col = e.color;
+        return dx * dx + dy * dy + dz * dz <= e.r * e.r;
+    }
+    const int64_t v = vol_index(e, (int)i, (int)j, (int)k);
+    if (v < 0 || !set.vsolid[(size_t)v]) return false;
+    *col = set.vcol[(size_t)v];
+    return true;
+}
+
+// Leaf colour: the last edit whose source holds the voxel decides (a GROW's
 // colour, or after a CLEAR the recipe's); without one, the recipe's hash.
 // Edits with ids >= limit are not applied yet.
 static inline uint32_t edited_color(uint64_t code, uint64_t seed, const EditSet* edits, size_t limit) {
@@ -193,7 +211,8 @@ static inline uint32_t edited_color(uint64_t code, uint64_t seed, const EditSet*
             if (*p >= limit) continue;
             const EditSphere& e = edits->edits[*p];
             if ((int)i < e.xlo || (int)i > e.xhi || (int)j < e.ylo || (int)j > e.yhi) continue;
-            if (in_ball(e, i, j, k)) return e.mode == YX_BUILD_GROW ? e.color : leaf_color(code, seed);
+            uint32_t col = 0;
+            if (in_edit(*edits, e, i, j, k, &col)) return e.mode == YX_BUILD_GROW ? col : leaf_color(code, seed);
         }
     }
     return leaf_color(code, seed);
@@ -456,9 +475,38 @@ void EditSet::index(int n) {
     }
 }
 
-int edit_spheres(const yx_sphere_edit* e, size_t n, int depth, EditSet& set) {
-    std::vector<EditSphere>& out = set.edits;
-    out.clear();
+// Applies edit x to the inside intervals of column (i, j): GROW unions the
+// source's solid runs, CLEAR subtracts them.
+static void apply_edit_column(const EditSet& set, const EditSphere& x, int i, int j, int n,
+                              std::vector<Interval>& colv) {
+    if (i < x.xlo || i > x.xhi || j < x.ylo || j > x.yhi) return;
+    auto apply = [&](Interval iv) {
+        if (iv.lo > iv.hi) return;
+        if (x.mode == YX_BUILD_GROW)
+            interval_union(colv, iv);
+        else
+            interval_subtract(colv, iv);
+    };
+    if (x.kind == 0) {
+        apply(sphere_column(Sphere{x.cx, x.cy, x.cz, x.r}, i, j, n));
+        return;
+    }
+    const int64_t base = vol_index(x, i, j, x.vz);
+    if (base < 0) return;
+    const int64_t stride = (int64_t)x.sx * x.sy;
+    int c = 0;
+    while (c < x.sz) {
+        while (c < x.sz && !set.vsolid[(size_t)(base + stride * c)]) ++c;
+        if (c == x.sz) break;
+        const int c0 = c;
+        while (c < x.sz && set.vsolid[(size_t)(base + stride * c)]) ++c;
+        const int64_t lo = (int64_t)x.vz + (int64_t)c0 * x.s, hi = (int64_t)x.vz + (int64_t)c * x.s - 1;
+        apply(Interval{(int32_t)std::max<int64_t>(0, lo), (int32_t)std::min<int64_t>(n - 1, hi)});
+    }
+}
+
+int add_sphere_edits(const yx_sphere_edit* e, size_t n, int depth, EditSet& set) {
+    std::vector<EditSphere> out;
     const int grid = 1 << depth;
     for (size_t q = 0; q < n; ++q) {
         const yx_sphere_edit& x = e[q];
@@ -486,9 +534,80 @@ int edit_spheres(const yx_sphere_edit* e, size_t n, int depth, EditSet& set) {
         s.xhi = std::min(grid - 1, (int)std::min((double)grid, std::ceil(s.cx + s.r + 1.0)));
         s.ylo = std::max(0, (int)std::max(-1.0, std::floor(s.cy - s.r - 1.0)));
         s.yhi = std::min(grid - 1, (int)std::min((double)grid, std::ceil(s.cy + s.r + 1.0)));
+        s.zlo = std::max(0, (int)std::max(-1.0, std::floor(s.cz - s.r - 1.0)));
+        s.zhi = std::min(grid - 1, (int)std::min((double)grid, std::ceil(s.cz + s.r + 1.0)));
+        s.kind = 0;
+        s.vx = s.vy = s.vz = 0;
+        s.s = 1;
+        s.sx = s.sy = s.sz = 0;
+        s.off = 0;
         out.push_back(s);
     }
-    set.index(grid);
+    set.edits.insert(set.edits.end(), out.begin(), out.end());
+    return YX_OK;
+}
+
+int add_volume_edit(const yx_volume_source& v, int depth, EditSet& set) {
+    const int grid = 1 << depth;
+    if (v.mode != YX_BUILD_GROW && v.mode != YX_BUILD_CLEAR) return fail(YX_ERR_INVALID_ARG, "unknown mode %d", v.mode);
+    if (v.level < 1 || v.level > depth) return fail(YX_ERR_INVALID_ARG, "level %d outside 1..%d", v.level, depth);
+    if (v.kind != YX_SOURCE_ISO && v.kind != YX_SOURCE_RAW)
+        return fail(YX_ERR_INVALID_ARG, "unknown source kind %d", v.kind);
+    for (int a = 0; a < 3; ++a) {
+        if (v.size[a] < 1 || v.size[a] > 4096) return fail(YX_ERR_INVALID_ARG, "source size %d: 1..4096", v.size[a]);
+        if (v.pos[a] < -(1 << 24) || v.pos[a] > (1 << 24))
+            return fail(YX_ERR_INVALID_ARG, "source position beyond 2^24 voxels");
+    }
+    if (!v.data) return fail(YX_ERR_INVALID_ARG, "null source data");
+    const int64_t sc = (int64_t)1 << (depth - v.level);
+    const size_t nv = (size_t)v.size[0] * (size_t)v.size[1] * (size_t)v.size[2];
+    EditSphere e{};
+    e.mode = v.mode;
+    e.kind = 1;
+    e.color = v.color;
+    e.s = (int)sc;
+    e.sx = v.size[0];
+    e.sy = v.size[1];
+    e.sz = v.size[2];
+    const int64_t lo[3] = {v.pos[0] * sc, v.pos[1] * sc, v.pos[2] * sc};
+    const int64_t hi[3] = {lo[0] + v.size[0] * sc - 1, lo[1] + v.size[1] * sc - 1, lo[2] + v.size[2] * sc - 1};
+    if (std::max({hi[0], hi[1], hi[2]}) >= ((int64_t)1 << 30) || std::min({lo[0], lo[1], lo[2]}) <= -((int64_t)1 << 30))
+        return fail(YX_ERR_INVALID_ARG, "source extent beyond 2^30 voxels");
+    e.vx = (int)lo[0];
+    e.vy = (int)lo[1];
+    e.vz = (int)lo[2];
+    auto clampc = [&](int64_t x) { return (int)std::min<int64_t>(grid, std::max<int64_t>(-1, x)); };
+    e.xlo = std::max(0, clampc(lo[0]));
+    e.xhi = std::min(grid - 1, clampc(hi[0]));
+    e.ylo = std::max(0, clampc(lo[1]));
+    e.yhi = std::min(grid - 1, clampc(hi[1]));
+    e.zlo = std::max(0, clampc(lo[2]));
+    e.zhi = std::min(grid - 1, clampc(hi[2]));
+    e.off = set.vcol.size();
+    const size_t old = set.vcol.size();
+    try {
+        set.vcol.resize(old + nv);
+        set.vsolid.resize(old + nv);
+    } catch (const std::bad_alloc&) {
+        set.vcol.resize(old);
+        set.vsolid.resize(old);
+        return fail(YX_ERR_OOM, "out of host memory for a %zu-voxel source", nv);
+    }
+    if (v.kind == YX_SOURCE_ISO) {
+        const uint8_t* d = static_cast<const uint8_t*>(v.data);
+        for (size_t q = 0; q < nv; ++q) {
+            const bool above = (int)d[q] >= v.iso;
+            set.vsolid[old + q] = (uint8_t)(v.inside ? above : !above);
+            set.vcol[old + q] = v.color;
+        }
+    } else {
+        const uint32_t* d = static_cast<const uint32_t*>(v.data);
+        for (size_t q = 0; q < nv; ++q) {
+            set.vsolid[old + q] = (uint8_t)((d[q] >> 24) != 0);
+            set.vcol[old + q] = d[q];
+        }
+    }
+    set.edits.push_back(e);
     return YX_OK;
 }
 
@@ -574,16 +693,8 @@ static int build_synthetic(const yx_scene_desc& d, yx_scene& out, const EditSet*
             // sphere edits in order: union (GROW) or difference (CLEAR)
             const int i = (int)(col % (uint32_t)n), j = j0 + (int)(col / (uint32_t)n);
             if (edits && !edits->empty())
-            for (const uint32_t* e = edits->begin(i, j); e != edits->end(i, j); ++e) {
-                const EditSphere& x = edits->edits[*e];
-                if (i < x.xlo || i > x.xhi || j < x.ylo || j > x.yhi) continue;
-                const Interval iv = sphere_column(Sphere{x.cx, x.cy, x.cz, x.r}, i, j, n);
-                if (iv.lo > iv.hi) continue;
-                if (x.mode == YX_BUILD_GROW)
-                    interval_union(colv, iv);
-                else
-                    interval_subtract(colv, iv);
-            }
+                for (const uint32_t* e = edits->begin(i, j); e != edits->end(i, j); ++e)
+                    apply_edit_column(*edits, edits->edits[*e], i, j, n, colv);
             ivs.insert(ivs.end(), colv.begin(), colv.end());
         }
         off[(size_t)(j1 - j0) * n] = (uint32_t)ivs.size();
@@ -759,16 +870,12 @@ static int edit_in_place(yx_scene& s, const EditSet& set, size_t first) {
     if (d.heightfield && s.hf_top.size() != (size_t)n * n) s.hf_top.assign((size_t)n * n, kTopUnknown);
     for (size_t q = first; q < edits.size(); ++q) {
         const EditSphere& e = edits[q];
-        // voxels the edit can change: the ball's voxel box (with a voxel of
-        // slack), then one more for the surface test of its neighbours
-        int lo[3], hi[3];
-        const double c[3] = {e.cx, e.cy, e.cz};
+        // voxels the edit can change: its box within the domain (a sphere's
+        // with a voxel of slack), then one more for the surface test of its
+        // neighbours
+        int lo[3] = {e.xlo, e.ylo, e.zlo}, hi[3] = {e.xhi, e.yhi, e.zhi};
         bool empty = false;
         for (int a = 0; a < 3; ++a) {
-            lo[a] = (int)std::max(-1.0, std::floor(c[a] - e.r - 0.5) - 1.0);
-            hi[a] = (int)std::min((double)n, std::ceil(c[a] + e.r - 0.5) + 1.0);
-            lo[a] = std::max(0, lo[a]);
-            hi[a] = std::min(n - 1, hi[a]);
             if (lo[a] > hi[a]) empty = true;
             lo[a] = std::max(0, lo[a] - 1);
             hi[a] = std::min(n - 1, hi[a] + 1);
@@ -808,16 +915,8 @@ static int edit_in_place(yx_scene& s, const EditSet& set, size_t first) {
                     else
                         colv.push_back(iv);
                 }
-                for (const uint32_t* ek = set.begin(i, j); ek != set.end(i, j) && *ek <= q; ++ek) {
-                    const EditSphere& x = edits[*ek];
-                    if (i < x.xlo || i > x.xhi || j < x.ylo || j > x.yhi) continue;
-                    const Interval iv = sphere_column(Sphere{x.cx, x.cy, x.cz, x.r}, i, j, n);
-                    if (iv.lo > iv.hi) continue;
-                    if (x.mode == YX_BUILD_GROW)
-                        interval_union(colv, iv);
-                    else
-                        interval_subtract(colv, iv);
-                }
+                for (const uint32_t* ek = set.begin(i, j); ek != set.end(i, j) && *ek <= q; ++ek)
+                    apply_edit_column(set, edits[*ek], i, j, n, colv);
             }
         };
         // threads only pay off past a few thousand columns
@@ -1073,34 +1172,31 @@ extern "C" int yx_scene_build_synthetic(const yx_scene_desc* desc, yx_scene** ou
     return YX_OK;
 }
 
-extern "C" int yx_scene_build_range(yx_scene* s, const yx_sphere_edit* edits, uint32_t count, int32_t how,
-                                    void* stream) {
-    clear_error();
-    if (!s || (!edits && count)) return fail(YX_ERR_INVALID_ARG, "null argument");
-    if (how != YX_RANGE_IN_PLACE && how != YX_RANGE_REBUILD && how != YX_RANGE_REBUILD_GPU)
-        return fail(YX_ERR_INVALID_ARG, "unknown build-range method %d", how);
-    if (!s->has_recipe)
-        return fail(YX_ERR_INVALID_ARG, "the scene has no synthetic recipe (loaded, or edited node by node)");
-    if (count == 0) return YX_OK;
-    std::vector<yx_sphere_edit> all;
-    EditSet es;
+// Applies the edits appended to s->edits from index `first` (volume data from
+// `vfirst`) by method `how`; on failure they are taken off again.
+static int apply_new_edits(yx_scene* s, size_t first, size_t vfirst, int32_t how, void* stream) {
+    EditSet& es = s->edits;
+    auto rollback = [&](int code) {
+        es.edits.resize(first);
+        es.vcol.resize(vfirst);
+        es.vsolid.resize(vfirst);
+        es.index(1 << s->recipe.depth);
+        return code;
+    };
     try {
-        all = s->edits;
-        all.insert(all.end(), edits, edits + count);
+        es.index(1 << s->recipe.depth);
     } catch (const std::bad_alloc&) {
-        return fail(YX_ERR_OOM, "out of host memory");
+        return rollback(fail(YX_ERR_OOM, "out of host memory"));
     }
-    int rc = edit_spheres(all.data(), all.size(), s->recipe.depth, es);
-    if (rc != YX_OK) return rc;
+    int rc;
     if (how == YX_RANGE_IN_PLACE) {
         try {
-            rc = edit_in_place(*s, es, s->edits.size());
+            rc = edit_in_place(*s, es, first);
         } catch (const std::bad_alloc&) {
             return fail(YX_ERR_OOM, "out of host memory editing the scene (the scene is partly edited)");
         } catch (const std::exception& x) {
             return fail(YX_ERR_RANGE, "%s (the scene is partly edited)", x.what());
         }
-        if (rc == YX_OK) s->edits.swap(all);
         return rc;
     }
     yx_scene t;
@@ -1108,9 +1204,9 @@ extern "C" int yx_scene_build_range(yx_scene* s, const yx_sphere_edit* edits, ui
         rc = how == YX_RANGE_REBUILD_GPU ? gpu_build_synthetic(s->recipe, es, t, stream)
                                          : build_synthetic(s->recipe, t, &es);
     } catch (const std::bad_alloc&) {
-        return fail(YX_ERR_OOM, "out of host memory rebuilding the scene");
+        return rollback(fail(YX_ERR_OOM, "out of host memory rebuilding the scene"));
     }
-    if (rc != YX_OK) return rc;
+    if (rc != YX_OK) return rollback(rc);
     // new version on the pages whose bytes changed (and on new pages)
     const size_t old_n = s->nodes.size() / 10, new_n = t.nodes.size() / 10;
     const size_t pages = (new_n + kScenePageNodes - 1) / kScenePageNodes;
@@ -1118,7 +1214,7 @@ extern "C" int yx_scene_build_range(yx_scene* s, const yx_sphere_edit* edits, ui
     try {
         s->page_version.resize(std::max(pages, s->page_version.size()), 1u);
     } catch (const std::bad_alloc&) {
-        return fail(YX_ERR_OOM, "out of host memory");
+        return rollback(fail(YX_ERR_OOM, "out of host memory"));
     }
     for (size_t p = 0; p < pages; ++p) {
         const size_t n0 = p * kScenePageNodes, n1 = std::min(new_n, n0 + kScenePageNodes);
@@ -1133,8 +1229,50 @@ extern "C" int yx_scene_build_range(yx_scene* s, const yx_sphere_edit* edits, ui
     s->depth_stale = false;
     s->free_nodes.clear();
     if (!t.hf_top.empty()) s->hf_top.swap(t.hf_top);
-    s->edits.swap(all);
     return YX_OK;
+}
+
+static int check_range_call(yx_scene* s, int32_t how) {
+    if (how != YX_RANGE_IN_PLACE && how != YX_RANGE_REBUILD && how != YX_RANGE_REBUILD_GPU)
+        return fail(YX_ERR_INVALID_ARG, "unknown build-range method %d", how);
+    if (!s->has_recipe)
+        return fail(YX_ERR_INVALID_ARG, "the scene has no synthetic recipe (loaded, or edited node by node)");
+    return YX_OK;
+}
+
+extern "C" int yx_scene_build_range(yx_scene* s, const yx_sphere_edit* edits, uint32_t count, int32_t how,
+                                    void* stream) {
+    clear_error();
+    if (!s || (!edits && count)) return fail(YX_ERR_INVALID_ARG, "null argument");
+    int rc = check_range_call(s, how);
+    if (rc != YX_OK || count == 0) return rc;
+    const size_t first = s->edits.edits.size(), vfirst = s->edits.vcol.size();
+    try {
+        rc = add_sphere_edits(edits, count, s->recipe.depth, s->edits);
+    } catch (const std::bad_alloc&) {
+        s->edits.edits.resize(first);
+        return fail(YX_ERR_OOM, "out of host memory");
+    }
+    if (rc != YX_OK) return rc;
+    return apply_new_edits(s, first, vfirst, how, stream);
+}
+
+extern "C" int yx_scene_build_range_volume(yx_scene* s, const yx_volume_source* src, int32_t how, void* stream) {
+    clear_error();
+    if (!s || !src) return fail(YX_ERR_INVALID_ARG, "null argument");
+    int rc = check_range_call(s, how);
+    if (rc != YX_OK) return rc;
+    const size_t first = s->edits.edits.size(), vfirst = s->edits.vcol.size();
+    try {
+        rc = add_volume_edit(*src, s->recipe.depth, s->edits);
+    } catch (const std::bad_alloc&) {
+        s->edits.edits.resize(first);
+        s->edits.vcol.resize(vfirst);
+        s->edits.vsolid.resize(vfirst);
+        return fail(YX_ERR_OOM, "out of host memory");
+    }
+    if (rc != YX_OK) return rc;
+    return apply_new_edits(s, first, vfirst, how, stream);
 }
 
 extern "C" int yx_scene_from_nodes(const uint32_t* nodes, uint32_t num_nodes, uint32_t root,

@@ -55,7 +55,17 @@ struct SbArgs {
     const uint32_t* cell_off;   // edits per 64 x 64-column cell (EditSet's CSR)
     const uint32_t* cell_ids;
     int cells;
+    const uint32_t* vcol;    // volume sources' voxel colours
+    const uint8_t* vsolid;   // and solid flags
 };
+
+// Source voxel index of finest voxel (i, j, k) in a volume edit, or -1.
+__device__ __forceinline__ int64_t sb_vol_index(const EditSphere& e, int i, int j, int k) {
+    if (i < e.vx || j < e.vy || k < e.vz) return -1;
+    const int a = (i - e.vx) / e.s, b = (j - e.vy) / e.s, c = (k - e.vz) / e.s;
+    if (a >= e.sx || b >= e.sy || c >= e.sz) return -1;
+    return (int64_t)e.off + a + (int64_t)e.sx * (b + (int64_t)e.sy * c);
+}
 
 __device__ __forceinline__ size_t sb_cell(int cells, int i, int j) {
     return (size_t)(j >> EditSet::kCellShift) * cells + (size_t)(i >> EditSet::kCellShift);
@@ -187,28 +197,14 @@ __global__ __launch_bounds__(256) void sb_columns_kernel(SbArgs a) {
         else
             out[m++] = v;
     }
-    // sphere edits in order: union (GROW) or difference (CLEAR); the list
-    // stays sorted, disjoint and non-touching
-    const size_t cell = a.edits ? sb_cell(a.cells, i, j) : 0;
-    const uint32_t e0 = a.edits ? a.cell_off[cell] : 0, e1 = a.edits ? a.cell_off[cell + 1] : 0;
-    for (uint32_t e = e0; e < e1 && !over; ++e) {
-        const EditSphere& x = a.edits[a.cell_ids[e]];
-        if (i < x.xlo || i > x.xhi || j < x.ylo || j > x.yhi) continue;
-        SbSphere sp;
-        sp.cx = x.cx;
-        sp.cy = x.cy;
-        sp.cz = x.cz;
-        sp.r = x.r;
-        sp.xlo = x.xlo;
-        sp.xhi = x.xhi;
-        sp.ylo = x.ylo;
-        sp.yhi = x.yhi;
-        const SbInterval v = sb_sphere_column(sp, i, j, n);
-        if (v.lo > v.hi) continue;
-        if (x.mode == YX_BUILD_GROW) {
+    // edits in order: union (GROW) or difference (CLEAR) of each source's
+    // solid runs; the list stays sorted, disjoint and non-touching
+    auto apply = [&](SbInterval v, bool grow) {
+        if (v.lo > v.hi || over) return;
+        if (grow) {
             if (m == kColMax) {
                 over = true;
-                break;
+                return;
             }
             int p = m;
             while (p > 0 && out[p - 1].lo > v.lo) {
@@ -232,7 +228,7 @@ __global__ __launch_bounds__(256) void sb_columns_kernel(SbArgs a) {
             while (p < m && out[p].hi < v.lo) ++p;
             int q = p;
             while (q < m && out[q].lo <= v.hi) ++q;
-            if (q == p) continue;  // nothing inside the ball
+            if (q == p) return;  // nothing inside
             const SbInterval first = out[p], last = out[q - 1];
             SbInterval keep[2];
             int nk = 0;
@@ -241,7 +237,7 @@ __global__ __launch_bounds__(256) void sb_columns_kernel(SbArgs a) {
             const int removed = q - p, delta = nk - removed;
             if (m + delta > kColMax) {
                 over = true;
-                break;
+                return;
             }
             if (delta > 0) {
                 for (int r = m - 1; r >= q; --r) out[r + delta] = out[r];
@@ -250,6 +246,39 @@ __global__ __launch_bounds__(256) void sb_columns_kernel(SbArgs a) {
             }
             for (int r = 0; r < nk; ++r) out[p + r] = keep[r];
             m += delta;
+        }
+    };
+    const size_t cell = a.edits ? sb_cell(a.cells, i, j) : 0;
+    const uint32_t e0 = a.edits ? a.cell_off[cell] : 0, e1 = a.edits ? a.cell_off[cell + 1] : 0;
+    for (uint32_t e = e0; e < e1 && !over; ++e) {
+        const EditSphere& x = a.edits[a.cell_ids[e]];
+        if (i < x.xlo || i > x.xhi || j < x.ylo || j > x.yhi) continue;
+        const bool grow = x.mode == YX_BUILD_GROW;
+        if (x.kind == 0) {
+            SbSphere sp;
+            sp.cx = x.cx;
+            sp.cy = x.cy;
+            sp.cz = x.cz;
+            sp.r = x.r;
+            sp.xlo = x.xlo;
+            sp.xhi = x.xhi;
+            sp.ylo = x.ylo;
+            sp.yhi = x.yhi;
+            apply(sb_sphere_column(sp, i, j, n), grow);
+            continue;
+        }
+        // a volume source: its solid runs along the column
+        const int64_t base = sb_vol_index(x, i, j, x.vz);
+        if (base < 0) continue;
+        const int64_t stride = (int64_t)x.sx * x.sy;
+        int c = 0;
+        while (c < x.sz && !over) {
+            while (c < x.sz && !a.vsolid[base + stride * c]) ++c;
+            if (c == x.sz) break;
+            const int c0 = c;
+            while (c < x.sz && a.vsolid[base + stride * c]) ++c;
+            const int64_t lo = (int64_t)x.vz + (int64_t)c0 * x.s, hi = (int64_t)x.vz + (int64_t)c * x.s - 1;
+            apply(SbInterval{(int)max((int64_t)0, lo), (int)min((int64_t)(n - 1), hi)}, grow);
         }
     }
     a.cnt[col] = m;
@@ -365,10 +394,16 @@ __device__ __forceinline__ uint32_t sb_edited_color(uint64_t code, uint64_t seed
     for (uint32_t q = a.cell_off[cell + 1]; q-- > a.cell_off[cell];) {
         const EditSphere& e = a.edits[a.cell_ids[q]];
         if ((int)i < e.xlo || (int)i > e.xhi || (int)j < e.ylo || (int)j > e.yhi) continue;
-        const double dx = ((double)i + 0.5) - e.cx;
-        const double dy = ((double)j + 0.5) - e.cy;
-        const double dz = ((double)k + 0.5) - e.cz;
-        if (dx * dx + dy * dy + dz * dz <= e.r * e.r) return e.mode == YX_BUILD_GROW ? e.color : sb_leaf_color(code, seed);
+        if (e.kind == 0) {
+            const double dx = ((double)i + 0.5) - e.cx;
+            const double dy = ((double)j + 0.5) - e.cy;
+            const double dz = ((double)k + 0.5) - e.cz;
+            if (dx * dx + dy * dy + dz * dz <= e.r * e.r)
+                return e.mode == YX_BUILD_GROW ? e.color : sb_leaf_color(code, seed);
+        } else {
+            const int64_t v = sb_vol_index(e, (int)i, (int)j, (int)k);
+            if (v >= 0 && a.vsolid[v]) return e.mode == YX_BUILD_GROW ? a.vcol[v] : sb_leaf_color(code, seed);
+        }
     }
     return sb_leaf_color(code, seed);
 }
@@ -486,7 +521,14 @@ static int gpu_build(const yx_scene_desc* d, const EditSet& edits, yx_scene* s, 
     SbBufs B;
     int rc;
     EditSphere* d_edits = nullptr;
-    uint32_t *d_cell_off = nullptr, *d_cell_ids = nullptr;
+    uint32_t *d_cell_off = nullptr, *d_cell_ids = nullptr, *d_vcol = nullptr;
+    uint8_t* d_vsolid = nullptr;
+    if (!edits.vcol.empty()) {
+        const size_t nv = edits.vcol.size();
+        if ((rc = B.alloc(&d_vcol, nv)) != YX_OK || (rc = B.alloc(&d_vsolid, nv)) != YX_OK) return rc;
+        YX_HIP(hipMemcpyAsync(d_vcol, edits.vcol.data(), nv * 4, hipMemcpyHostToDevice, st));
+        YX_HIP(hipMemcpyAsync(d_vsolid, edits.vsolid.data(), nv, hipMemcpyHostToDevice, st));
+    }
     if (!edits.empty()) {
         const size_t ne = edits.edits.size(), no = edits.cell_off.size(), ni = std::max<size_t>(1, edits.cell_ids.size());
         if ((rc = B.alloc(&d_edits, ne)) != YX_OK || (rc = B.alloc(&d_cell_off, no)) != YX_OK ||
@@ -524,6 +566,8 @@ static int gpu_build(const yx_scene_desc* d, const EditSet& edits, yx_scene* s, 
     a.cell_off = d_cell_off;
     a.cell_ids = d_cell_ids;
     a.cells = edits.cells;
+    a.vcol = d_vcol;
+    a.vsolid = d_vsolid;
     hipLaunchKernelGGL(sb_columns_kernel, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, st, a);
     YX_HIP(hipGetLastError());
 

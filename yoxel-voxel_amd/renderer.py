@@ -117,6 +117,32 @@ class SVOData:
         st = ctypes.c_void_p(_stream(stream) if how == _lib.YX_RANGE_REBUILD_GPU else 0)
         check(lib().yx_scene_build_range(self._h, arr, len(edits), int(how), st))
 
+    def BuildRangeIso(self, level, pos, mode, values, iso, inside=True, color=0xFFC0B6C0, how=IN_PLACE,
+                      stream=None):
+        """BuildRange with an IsoSource (scene_gen.py:111-130): `values` uint8
+        [z][y][x]; solid where value >= iso (inside) or < iso; voxel (0, 0, 0)
+        lands at integer `pos` of `level`."""
+        v = np.ascontiguousarray(values, dtype=np.uint8)
+        self._volume(level, pos, mode, v, _lib.YX_SOURCE_ISO, int(iso), 1 if inside else 0, color, how, stream)
+
+    def BuildRangeRaw(self, level, pos, mode, rgba, how=IN_PLACE, stream=None):
+        """BuildRange with a RawSource (scene_gen.py:74-82): `rgba` uint32 [z][y][x]
+        (or uint8 [z][y][x][4]) RGBA8; solid where alpha != 0."""
+        v = np.ascontiguousarray(rgba)
+        if v.dtype == np.uint8 and v.ndim == 4 and v.shape[3] == 4:
+            v = np.ascontiguousarray(v).view(np.uint32)[..., 0]
+        v = np.ascontiguousarray(v, dtype=np.uint32)
+        self._volume(level, pos, mode, v, _lib.YX_SOURCE_RAW, 0, 1, 0, how, stream)
+
+    def _volume(self, level, pos, mode, v, kind, iso, inside, color, how, stream):
+        if v.ndim != 3:
+            raise ValueError("a source volume is [z][y][x]")
+        src = _lib.VolumeSource(int(mode), int(level), (ctypes.c_int32 * 3)(*[int(p) for p in pos]),
+                                (ctypes.c_int32 * 3)(v.shape[2], v.shape[1], v.shape[0]), int(kind), int(iso),
+                                int(inside), int(color), ctypes.c_void_p(v.ctypes.data))
+        st = ctypes.c_void_p(_stream(stream) if how == _lib.YX_RANGE_REBUILD_GPU else 0)
+        check(lib().yx_scene_build_range_volume(self._h, ctypes.byref(src), int(how), st))
+
     def info(self):
         i = _lib.SceneInfo()
         check(lib().yx_scene_get_info(self._h, ctypes.byref(i)))

@@ -125,3 +125,18 @@ def voxtree_volume(B, seed, dev="cuda", flip=2e-5):
         vol |= (X - p[0]) ** 2 + (Y - p[1]) ** 2 + (Z - p[2]) ** 2 <= r * r
     vol ^= torch.rand((B, B, B), generator=g, device=dev) < flip
     return vol
+
+
+def iso_blob(shape, seed):
+    """uint8 values [z][y][x] for an IsoSource: a sum of 6 Gaussians of radius
+    shape/16..shape/4 at uniform centres, scaled to 0..255."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    axes = [np.arange(k, dtype=np.float32) for k in shape]
+    v = np.zeros(shape, np.float32)
+    for _ in range(6):
+        c = rng.uniform(0, 1, 3) * np.array(shape)
+        r = rng.uniform(max(shape) / 16, max(shape) / 4)
+        g = [np.exp(-((axes[a] - c[a]) ** 2) / (2 * r * r)).astype(np.float32) for a in range(3)]
+        v += g[0][:, None, None] * g[1][None, :, None] * g[2][None, None, :]
+    return np.clip(v / v.max() * 255.0, 0, 255).astype(np.uint8)
