@@ -459,11 +459,12 @@ def pipeline_bench(yx, torch, scene, cams, with_cpu, reps=2, cpu_frames=4):
 
 def update_bench(yx, torch, scene, reps=3, with_edit=True):
     """Scene sync on large_1080p (14.2 M VoxNodes, 46.4 M records):
-    SetScene (host breadth-first re-layout + LOD colours + upload of the
-    records) against UpdateScene (CudaSVO::Update, demo/SVORenderer.cpp:39-53:
-    upload of the changed 256-node pages to the device node copy, re-layout
-    and LOD colours on the GPU). Wall time per call, synchronised; the edit
-    recolours the leaves of 64 nodes spread over 4 pages."""
+    SetSceneHost (host breadth-first re-layout + LOD colours + upload of the
+    records) and SetScene (upload of the VoxNodes, re-layout on the GPU)
+    against UpdateScene (CudaSVO::Update, demo/SVORenderer.cpp:39-53: upload
+    of the changed 256-node pages to the device node copy, re-layout and LOD
+    colours on the GPU). Wall time per call, synchronised; the edit recolours
+    the leaves of 64 nodes spread over 4 pages."""
     nodes = scene.nodes()
     cand = np.nonzero(nodes[:, 0] & 0xFF)[0]
     pages = sorted({int(cand[int(k * len(cand) / 4)]) // 256 for k in range(4)})
@@ -484,7 +485,8 @@ def update_bench(yx, torch, scene, reps=3, with_edit=True):
 
     r_host = yx.SVORenderer()
     r_gpu = yx.SVORenderer()
-    host_ms = min(wall(lambda: r_host.SetScene(scene)) for _ in range(reps))
+    host_ms = min(wall(lambda: r_host.SetSceneHost(scene)) for _ in range(reps))
+    set_ms = min(wall(lambda: r_host.SetScene(scene)) for _ in range(reps))
     full_ms = wall(lambda: r_gpu.UpdateScene(scene))
     full_pages, full_bytes = r_gpu.last_update()
     nop_ms = min(wall(lambda: r_gpu.UpdateScene(scene)) for _ in range(reps))
@@ -499,13 +501,13 @@ def update_bench(yx, torch, scene, reps=3, with_edit=True):
     r_gpu.UpdateScene(scene)
     edit = edit_bench(yx, torch, wall, host_ms) if with_edit else None
     return {"workload": "large_1080p scene sync", "nodes": int(nodes.shape[0]),
-            "set_scene_host_ms": round(host_ms, 1),
+            "set_scene_host_ms": round(host_ms, 1), "set_scene_ms": round(set_ms, 1),
             "update_full_ms": round(full_ms, 1), "update_full_pages": full_pages,
             "update_full_mb": round(full_bytes / 1e6, 1),
             "update_unchanged_ms": round(nop_ms, 2),
             "update_edit_ms": round(min(edit_ms), 2), "update_edit_pages": edit_pages,
             "update_edit_kb": round(edit_bytes / 1e3, 1),
-            "speedup_edit_vs_set_scene": round(host_ms / min(edit_ms), 1), "sphere_edits": edit}
+            "speedup_edit_vs_set_scene_host": round(host_ms / min(edit_ms), 1), "sphere_edits": edit}
 
 
 def edit_bench(yx, torch, wall, set_scene_ms, reps=3):
@@ -514,7 +516,7 @@ def edit_bench(yx, torch, wall, set_scene_ms, reps=3):
     hit, applied with BuildRange in place on the host, then UpdateScene
     (CudaSVO::Update). Alternate batches GROW and CLEAR. Against: the same
     batch applied by rebuilding the scene (on the host, or on the GPU) and
-    SetScene's full host re-layout. Wall time, synchronised."""
+    SetSceneHost's full host re-layout. Wall time, synchronised."""
     wl = yx.workloads
     s = yx.SVOData.build_synthetic_gpu(**wl.LARGE_1080P)
     r = yx.SVORenderer()

@@ -190,9 +190,15 @@ typedef struct yx_outputs {
 
 int yx_renderer_create(yx_renderer** out);
 void yx_renderer_destroy(yx_renderer* r);
-/* SetScene: uploads the scene to HBM (re-laid out). The scene may be freed
- * afterwards; the renderer keeps its own device copy. */
+/* SetScene (CudaSVO::SetSVO, demo/SVORenderer.cpp:39-53): uploads the
+ * scene's VoxNode array to HBM and lays it out for traversal on the GPU (the
+ * first-sync path of yx_renderer_update_scene), then waits for it. The scene
+ * may be freed afterwards; the renderer keeps its own device copies. */
 int yx_renderer_set_scene(yx_renderer* r, const yx_scene* scene);
+/* The same layout computed on the host and uploaded (no device VoxNode
+ * copy). Bit-identical records and LOD colours; slower for large scenes. No
+ * counterpart in the reference: kept as a second, independent layout path. */
+int yx_renderer_set_scene_host(yx_renderer* r, const yx_scene* scene);
 int yx_renderer_set_view_pos(yx_renderer* r, const float pos[3]);
 int yx_renderer_set_view_dir(yx_renderer* r, const float dir[3]);
 int yx_renderer_set_view_up(yx_renderer* r, const float up[3]);
@@ -219,7 +225,7 @@ int yx_render_frame(yx_renderer* r, const uint32_t** colors);
  * qtview.py:69-73): the renderer keeps a device copy of the scene's VoxNode
  * array and uploads only the 256-node pages edited since its last sync (all of
  * them for another scene object), then rebuilds its traversal records and LOD
- * colours on the GPU. Results are identical to yx_renderer_set_scene. On a
+ * colours on the GPU. Results are identical to yx_renderer_set_scene_host. On a
  * format error the previous records stay in place. */
 int yx_renderer_update_scene(yx_renderer* r, const yx_scene* scene, void* stream);
 /* DynamicSVO::CountChangedPages / CountTransfrerSize (ore/src/main.cpp:127-128):

@@ -1,7 +1,7 @@
 """Incremental scene sync on the GPU (yx_renderer_update_scene; CudaSVO::Update,
 demo/SVORenderer.cpp:39-53): the traversal records and LOD colours rebuilt on
 the GPU from the page-synced device copy equal the host re-layout of
-yx_renderer_set_scene bit for bit, only edited pages are uploaded, and a
+yx_renderer_set_scene_host bit for bit, only edited pages are uploaded, and a
 malformed edit is rejected without disturbing the records in place."""
 import numpy as np
 import pytest
@@ -13,7 +13,7 @@ pytestmark = pytest.mark.gpu
 
 def same_layout(yx, scene, r_upd=None, what=""):
     r_ref = yx.SVORenderer()
-    r_ref.SetScene(scene)
+    r_ref.SetSceneHost(scene)
     r = r_upd or yx.SVORenderer()
     r.UpdateScene(scene, stream=0)
     a, la = r_ref.layout()

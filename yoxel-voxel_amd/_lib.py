@@ -234,6 +234,7 @@ SIGNATURES = {
     "yx_renderer_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
     "yx_renderer_destroy": (None, [_P]),
     "yx_renderer_set_scene": (ctypes.c_int, [_P, _P]),
+    "yx_renderer_set_scene_host": (ctypes.c_int, [_P, _P]),
     "yx_renderer_set_view_pos": (ctypes.c_int, [_P, _F3]),
     "yx_renderer_set_view_dir": (ctypes.c_int, [_P, _F3]),
     "yx_renderer_set_view_up": (ctypes.c_int, [_P, _F3]),

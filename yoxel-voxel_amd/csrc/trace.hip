@@ -386,7 +386,7 @@ extern "C" int yx_renderer_get_tuning(const yx_renderer* r, yx_tuning* t) {
     return YX_OK;
 }
 
-extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
+extern "C" int yx_renderer_set_scene_host(yx_renderer* r, const yx_scene* s) {
     clear_error();
     if (!r || !s) return fail(YX_ERR_INVALID_ARG, "null argument");
     GpuTree t;
@@ -434,6 +434,21 @@ extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
         r->level_records.clear();
     }
     r->has_scene = true;
+    return YX_OK;
+}
+
+// SetScene: the whole VoxNode array goes up once and is laid out on the GPU
+// (the path of yx_renderer_update_scene for a scene object the renderer has
+// not seen), then the call waits for the records.
+extern "C" int yx_renderer_set_scene(yx_renderer* r, const yx_scene* s) {
+    clear_error();
+    if (!r || !s) return fail(YX_ERR_INVALID_ARG, "null argument");
+    // nothing to lay out: the host path needs no device
+    if (s->root == YX_EMPTY_NODE || s->nodes.empty()) return yx_renderer_set_scene_host(r, s);
+    r->mirror_uid = 0;  // upload every page
+    const int rc = yx_renderer_update_scene(r, s, nullptr);
+    if (rc != YX_OK) return rc;
+    YX_HIP(hipStreamSynchronize(nullptr));
     return YX_OK;
 }
 

@@ -185,7 +185,12 @@ class SVORenderer:
             self._h = ctypes.c_void_p()
 
     def SetScene(self, svo):
+        """Upload `svo` and lay it out on the GPU (waits for it)."""
         check(lib().yx_renderer_set_scene(self._h, svo.handle))
+
+    def SetSceneHost(self, svo):
+        """The same layout computed on the host (bit-identical, slower)."""
+        check(lib().yx_renderer_set_scene_host(self._h, svo.handle))
 
     def SetViewPos(self, pos):
         check(lib().yx_renderer_set_view_pos(self._h, f3(pos)))
