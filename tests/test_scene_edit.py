@@ -266,7 +266,7 @@ def blob(shape, seed, scale=255):
     v = np.zeros(shape)
     for _ in range(4):
         c = rng.uniform(0, 1, 3) * np.array(shape)
-        r = rng.uniform(1.5, max(shape) / 2.5)
+        r = rng.uniform(1.5, max(1.6, max(shape) / 2.5))
         v += np.exp(-((z - c[0]) ** 2 + (y - c[1]) ** 2 + (x - c[2]) ** 2) / (2 * r * r))
     return np.clip(v / v.max() * scale, 0, 255).astype(np.uint8)
 
@@ -347,3 +347,39 @@ def test_volume_source_errors(yx):
     # a source entirely outside the domain changes nothing
     s.BuildRangeIso(6, (-10, 0, 0), 0, v, 0)
     assert np.array_equal(s.nodes(), before)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_random_edit_sequences(yx, seed):
+    """40 random GROW / CLEAR edits (spheres at levels 3-5, iso and raw volumes,
+    some past the domain's faces) on a depth-5 scene: after every edit the
+    in-place tree equals the dense definition, its blank rows are free, and a
+    host rebuild of the same sequence holds the same voxels and colours."""
+    rng = np.random.default_rng(100 + seed)
+    desc = dict(depth=5, num_spheres=4, radius_min=2.0, radius_max=6.0, heightfield=True, hf_octaves=3,
+                hf_amplitude=0.2, hf_base=0.3, seed=seed)
+    s = yx.SVOData.build_synthetic(**desc)
+    edits = []
+    for k in range(40):
+        mode = int(rng.integers(2))
+        kind = rng.choice(["sphere", "sphere", "iso", "raw"])
+        level = int(rng.integers(3, 6))
+        n = 1 << level
+        if kind == "sphere":
+            pos = tuple(float(v) for v in rng.uniform(-0.2 * n, 1.2 * n, 3))
+            e = (mode, level, pos, float(rng.uniform(0.3, 0.4 * n)), int(rng.integers(0, 2 ** 32)))
+        else:
+            shape = tuple(int(v) for v in rng.integers(1, 7, 3))
+            pos = tuple(int(v) for v in rng.integers(-3, n, 3))
+            if kind == "iso":
+                e = dict(kind="iso", mode=mode, level=level, pos=pos, data=blob(shape, int(rng.integers(1000))),
+                         iso=int(rng.integers(20, 200)), inside=bool(rng.integers(2)),
+                         color=int(rng.integers(0, 2 ** 32)))
+            else:
+                e = dict(kind="raw", mode=mode, level=level, pos=pos, data=raw_block(shape, int(rng.integers(1000))))
+        edits.append(e)
+        apply_mixed(s, [e], yx.SVOData.IN_PLACE)
+        check_mixed(yx, s, desc, edits)
+    r = yx.SVOData.build_synthetic(**desc)
+    apply_mixed(r, edits, yx.SVOData.REBUILD)
+    assert leaf_colors(s) == leaf_colors(r)
