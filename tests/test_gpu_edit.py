@@ -129,3 +129,37 @@ def test_gpu_rebuild_with_volume_sources(yx, cuda):
         assert np.array_equal(c.nodes(), g.nodes())
     assert leaf_colors(p) == leaf_colors(c)
     check_pool(p.nodes(), p.GetRoot())
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_gpu_rebuild_random_sequences(yx, cuda, seed):
+    """20 random sphere and volume edits: GPU and host rebuilds identical byte for
+    byte after every fifth edit."""
+    from test_scene_edit import apply_mixed, blob, raw_block
+    rng = np.random.default_rng(200 + seed)
+    desc = dict(depth=6, num_spheres=6, radius_min=2.0, radius_max=9.0, heightfield=True, seed=seed)
+    c = yx.SVOData.build_synthetic(**desc)
+    g = yx.SVOData.build_synthetic_gpu(**desc)
+    batch = []
+    for k in range(20):
+        mode = int(rng.integers(2))
+        level = int(rng.integers(4, 7))
+        n = 1 << level
+        kind = k % 3
+        if kind == 0:
+            e = (mode, level, tuple(float(v) for v in rng.uniform(-0.2 * n, 1.2 * n, 3)),
+                 float(rng.uniform(0.3, 0.4 * n)), int(rng.integers(0, 2 ** 32)))
+        elif kind == 1:
+            e = dict(kind="iso", mode=mode, level=level, pos=tuple(int(v) for v in rng.integers(-3, n, 3)),
+                     data=blob(tuple(int(v) for v in rng.integers(1, 9, 3)), k), iso=int(rng.integers(20, 200)),
+                     inside=bool(rng.integers(2)), color=int(rng.integers(0, 2 ** 32)))
+        else:
+            e = dict(kind="raw", mode=mode, level=level, pos=tuple(int(v) for v in rng.integers(-3, n, 3)),
+                     data=raw_block(tuple(int(v) for v in rng.integers(1, 9, 3)), k))
+        batch.append(e)
+        if len(batch) == 5:
+            apply_mixed(c, batch, yx.SVOData.REBUILD)
+            apply_mixed(g, batch, yx.SVOData.REBUILD_GPU)
+            batch = []
+            assert c.info() == g.info()
+            assert np.array_equal(c.nodes(), g.nodes()), f"seed {seed} after edit {k}"
