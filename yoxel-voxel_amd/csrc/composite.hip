@@ -20,7 +20,8 @@
 // The backward recomputes the exact discrete sample sequence instead of
 // storing per-ray lists, writes one (voxel key, float4 gradient) record per
 // sample at the ray's exclusive-scan offset, radix-sorts the records by key
-// and reduces each key's run in sorted order: no atomics, deterministic.
+// (with their ids, 0..n-1 from a buffer written once per size) and reduces
+// each key's run in sorted order: no atomics, deterministic.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -52,7 +53,7 @@ struct CompArgs {
     uint32_t total_records;
     uint32_t n_waves;
     uint32_t* keys;
-    uint32_t* ids;  // record index (the sort's payload)
+
     float4* vals;
 };
 
@@ -304,7 +305,6 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
                 const uint32_t pos = cursor + lanes_rank(mask);
                 if (pos < a.total_records) {
                     a.keys[pos] = ix[j];
-                    a.ids[pos] = pos;
                     a.vals[pos] = make_float4(dsig * sgm, wgt * gr, wgt * gg, wgt * gb);
                 }
                 T = tn;
@@ -325,6 +325,13 @@ __global__ __launch_bounds__(256) void comp_count_runs_kernel(const uint32_t* ke
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(count, part[0] + part[1] + part[2] + part[3]);
+}
+
+// 0, 1, 2, ...: the record ids the sort carries along with the keys (a
+// record's id is its position in emission order)
+__global__ void comp_iota_kernel(uint32_t* ids, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) ids[i] = (uint32_t)i;
 }
 
 __global__ void comp_zero_kernel(float4* g, size_t n) {
@@ -395,6 +402,8 @@ struct yx_compositor {
     uint64_t last_records = 0;
     const uint32_t* sorted_keys = nullptr;  // the last backward's sorted keys (workspace)
     uint32_t* counter = nullptr;
+    uint32_t* d_iota = nullptr;  // record ids 0..iota_n-1, written once per size
+    size_t iota_n = 0;
 };
 
 static int check_vol(const yx_volume_dev* v, int32_t n_views, int32_t w, int32_t h) {
@@ -448,6 +457,7 @@ extern "C" int yx_compositor_create(yx_compositor** out) {
 extern "C" void yx_compositor_destroy(yx_compositor* c) {
     if (!c) return;
     if (c->ws) (void)hipFree(c->ws);
+    if (c->d_iota) (void)hipFree(c->d_iota);
     delete c;
 }
 
@@ -559,7 +569,21 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     YX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
                                               (uint32_t*)nullptr, (uint32_t*)nullptr, (int)total, 0, key_bits, st));
     const size_t kb = align256(total * 4), vb = align256(total * 16);
-    const size_t need = off_bytes + 4 * kb + vb + 256 + align256(std::max(sort_tmp, scan_tmp));
+    const size_t need = off_bytes + 3 * kb + vb + 256 + align256(std::max(sort_tmp, scan_tmp));
+    if (c->iota_n < total) {
+        if (c->d_iota) YX_HIP(hipFree(c->d_iota));
+        c->d_iota = nullptr;
+        c->iota_n = 0;
+        const size_t cap = total + total / 4;
+        hipError_t e = hipMalloc((void**)&c->d_iota, cap * 4);
+        if (e != hipSuccess) {
+            c->d_iota = nullptr;
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) record ids: %s", cap * 4, hipGetErrorString(e));
+        }
+        hipLaunchKernelGGL(comp_iota_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, c->d_iota, cap);
+        YX_HIP(hipGetLastError());
+        c->iota_n = cap;
+    }
     if (c->ws_bytes < need) {
         // grow, then recompute the offsets into the new block
         if (c->ws) YX_HIP(hipFree(c->ws));
@@ -578,8 +602,6 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     p += kb;
     uint32_t* keys_sorted = (uint32_t*)p;
     p += kb;
-    uint32_t* ids = (uint32_t*)p;
-    p += kb;
     uint32_t* ids_sorted = (uint32_t*)p;
     p += kb;
     float4* vals = (float4*)p;
@@ -594,15 +616,14 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     a.offsets = offsets;
     a.total_records = (uint32_t)total;
     a.keys = keys;
-    a.ids = ids;
     a.vals = vals;
     hipLaunchKernelGGL(comp_backward_kernel, dim3((unsigned)((n_waves * 64 + 255) / 256)), dim3(256), 0, st, a);
     YX_HIP(hipGetLastError());
 
     // 4. sort (key, id) by voxel key (LSD radix, stable: a key's records stay
     //    in emission order), 5. sum each key's run in that order
-    YX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys_sorted, ids, ids_sorted, (int)total, 0,
-                                              key_bits, st));
+    YX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys_sorted, (const uint32_t*)c->d_iota, ids_sorted,
+                                              (int)total, 0, key_bits, st));
     hipLaunchKernelGGL(comp_reduce_runs_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, keys_sorted,
                        ids_sorted, vals, (uint32_t)total, (float4*)grad_params);
     YX_HIP(hipGetLastError());
