@@ -23,6 +23,18 @@ def test_oracle_matches_fixture(yx, orc):
         assert o["rays"] == ent["rays"]
 
 
+def test_oracle_matches_small_fixture(yx, orc):
+    """The 64x64 fixture (tests/golden/plain_64.json): every view in all three
+    light modes."""
+    gold = mg.load(small=True)
+    s = yx.SVOData.build_synthetic(**yx.workloads.PLAIN_512)
+    assert mg.node_digest(s.nodes()) == gold["nodes_digest"]
+    for ent in gold["views"]:
+        o = orc.render(s.nodes(), s.GetRoot(), ent["pos"], ent["dir"], fov=gold["fov"], w=gold["w"], h=gold["h"],
+                       light_mode=ent["light_mode"], light=gold["light"], threads=4)
+        assert mg.digest(o) == ent["digest"], f"view {ent['view']} mode {ent['light_mode']}"
+
+
 def test_pipeline_oracle_matches_fixture(yx, oracle_mod):
     """The committed pipeline fixture (tests/golden/pipeline_256.json) pins the
     LOD trace, blur and shade restatement on the plain_512 scene."""

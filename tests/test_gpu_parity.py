@@ -211,3 +211,8 @@ def test_golden_fixture(yx, cuda):
         g = gpu_render(yx, cuda, s, ent["pos"], ent["dir"], gold["w"], gold["h"], light_mode=ent["light_mode"],
                        light=gold["light"])
         assert mg.digest(g) == ent["digest"], f"view {ent['view']}"
+    small = mg.load(small=True)
+    for ent in small["views"]:
+        g = gpu_render(yx, cuda, s, ent["pos"], ent["dir"], small["w"], small["h"], light_mode=ent["light_mode"],
+                       light=small["light"])
+        assert mg.digest(g) == ent["digest"], f"64x64 view {ent['view']} mode {ent['light_mode']}"
