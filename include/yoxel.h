@@ -119,9 +119,15 @@ int yx_scene_set_root(yx_scene* scene, uint32_t root);
  *   YX_RANGE_REBUILD      rebuild the canonical breadth-first tree on the host;
  *   YX_RANGE_REBUILD_GPU  the same tree, built on the GPU on `stream`.
  * Every method gives the same voxels and colours; the two rebuilds give the
- * same bytes. Pages whose bytes changed get a new version. YX_ERR_INVALID_ARG
- * for scenes without a recipe (loaded, or edited with yx_scene_write_nodes),
- * with nothing changed. */
+ * same bytes. Pages whose bytes changed get a new version.
+ * Scenes without a recipe (loaded, or edited with yx_scene_write_nodes) are
+ * edited in place only (YX_ERR_INVALID_ARG for the rebuilds), as trees: their
+ * solid is their leaves (a leaf slot covers its child cube) and FullNode
+ * cubes, levels count from the tree's deepest leaf level, and the voxels of
+ * each edit's box plus one are encoded again at that level: surface voxels
+ * as leaves, other solid voxels as FullNode (whole FullNode cubes collapse),
+ * the rest empty. Leaves keep their colour, grown voxels take the source's,
+ * and solid voxels that a CLEAR exposes take the CLEAR's `color`. */
 #define YX_BUILD_GROW  0
 #define YX_BUILD_CLEAR 1
 #define YX_RANGE_IN_PLACE    0
@@ -132,7 +138,8 @@ typedef struct yx_sphere_edit {
     int32_t level;  /* grid level of pos and radius, 1..depth (BuildRange's)  */
     float pos[3];   /* centre, in voxels of that level                         */
     float radius;   /* in voxels of that level, > 0                            */
-    uint32_t color; /* RGBA8 of grown voxels (SphereSource's colour)           */
+    uint32_t color; /* RGBA8 of grown voxels (SphereSource's colour); of voxels
+                       a CLEAR exposes in a scene without a recipe             */
 } yx_sphere_edit;
 int yx_scene_build_range(yx_scene* scene, const yx_sphere_edit* edits, uint32_t count, int32_t how,
                          void* stream);

@@ -392,3 +392,88 @@ def mixed_leaf_colors(desc, edits, xs, ys, zs):
     sel = touched[xs, ys, zs] & grow[xs, ys, zs]
     out[sel] = colmap[xs[sel], ys[sel], zs[sel]]
     return out
+
+
+FULL = 0x80000001
+
+
+def encode_tree(state, color, coarse=None):
+    """VoxNode tree of a dense voxel grid [x, y, z] of side 2^depth: state 0
+    empty, 1 solid inside (FullNode; whole cubes collapse), 2 leaf with
+    color. `coarse` {(level, ox, oy, oz): colour} puts a leaf slot at a node
+    cube (origin in voxels), covering it whole. Returns (nodes, root)."""
+    n = state.shape[0]
+    depth = n.bit_length() - 1
+    coarse = coarse or {}
+    rows = []
+
+    def slot(level, o):
+        size = n >> level
+        if (level, *o) in coarse:
+            return ("leaf", coarse[(level, *o)])
+        sub = state[o[0]:o[0] + size, o[1]:o[1] + size, o[2]:o[2] + size]
+        if level == depth:
+            st = int(sub[0, 0, 0])
+            return ("leaf", int(color[o])) if st == 2 else (("full", FULL) if st == 1 else ("empty", EMPTY))
+        if not sub.any() and not any(k[0] > level and all(o[a] <= k[1 + a] < o[a] + size for a in range(3))
+                                     for k in coarse):
+            return ("empty", EMPTY)
+        if (sub == 1).all():
+            return ("full", FULL)
+        half = size // 2
+        kids = [slot(level + 1, (o[0] + (c & 1) * half, o[1] + ((c >> 1) & 1) * half, o[2] + ((c >> 2) & 1) * half))
+                for c in range(8)]
+        row = np.full(10, EMPTY, np.uint32)
+        f = 0
+        for c, (kind, v) in enumerate(kids):
+            row[2 + c] = v
+            if kind == "leaf":
+                f |= 1 << c
+            elif kind in ("empty", "full"):
+                f |= 1 << (8 + c)
+        row[0] = f
+        row[1] = 0
+        rows.append(row)
+        return ("node", len(rows) - 1)
+
+    kind, v = slot(0, (0, 0, 0))
+    return (np.array(rows, np.uint32).reshape(-1, 10) if rows else np.zeros((0, 10), np.uint32)), \
+        (v if kind == "node" else (FULL if kind == "full" else EMPTY))
+
+
+def decode_tree(nodes, root, depth):
+    """(state, color) [x, y, z] of a tree: leaf slots fill their cubes with
+    state 2 and their colour, FullNode cubes state 1."""
+    n = 1 << depth
+    state = np.zeros((n, n, n), np.uint8)
+    color = np.zeros((n, n, n), np.uint32)
+
+    def walk(idx, level, o):
+        size = n >> level
+        row = nodes[idx]
+        f = int(row[0])
+        half = size // 2
+        for c in range(8):
+            co = (o[0] + (c & 1) * half, o[1] + ((c >> 1) & 1) * half, o[2] + ((c >> 2) & 1) * half)
+            ch = int(row[2 + c])
+            sl = tuple(slice(co[a], co[a] + half) for a in range(3))
+            if (f >> c) & 1:
+                state[sl] = 2
+                color[sl] = ch
+            elif ch == FULL:
+                state[sl] = 1
+            elif not ch & 0x80000000:
+                if level + 1 < depth:
+                    walk(ch, level + 1, co)
+
+    if root == FULL:
+        state[:] = 1
+    elif not root & 0x80000000:
+        walk(int(root), 0, (0, 0, 0))
+    return state, color
+
+
+def canonical_states(solid):
+    """Surface voxels (solid, a 6-neighbour empty or outside) 2, other solid 1."""
+    surf = surface(solid)
+    return np.where(surf, 2, np.where(solid, 1, 0)).astype(np.uint8)

@@ -163,3 +163,29 @@ def test_gpu_rebuild_random_sequences(yx, cuda, seed):
             batch = []
             assert c.info() == g.info()
             assert np.array_equal(c.nodes(), g.nodes()), f"seed {seed} after edit {k}"
+
+
+def test_tree_edits_sync_and_render(yx, cuda, orc):
+    """Edits of a loaded tree with FullNode interiors (qtview.py's flow: load,
+    BuildRange, UpdateScene): the GPU re-layout equals the host one, only the
+    edited pages go up, and the renders equal the oracle's."""
+    from test_scene_tree_edit import EDITS, check_edit, solid_scene
+    nodes, root, _, _ = solid_scene(7, 5)
+    s = yx.SVOData.from_nodes(nodes, root)
+    r = same_layout(yx, s, what="loaded tree")
+    scale = [(0, 7, (50.0, 80.0, 90.0), 9.0, 0xFF0000FF), (1, 7, (64.0, 64.0, 20.0), 14.0, 0xFF808080),
+             (0, 5, (3.0, 20.0, 8.0), 2.5, 0xFF00FF00)]
+    for k, e in enumerate(scale):
+        before = s.nodes()
+        check_edit(s, 7, e)
+        after = s.nodes()
+        exp = changed_pages(before, after)
+        pages, _ = r.CountChangedPages(s)
+        print(f"tree edit {k}: {exp} of {-(-after.shape[0] // 256)} pages", flush=True)
+        assert pages == exp and exp > 0
+        same_layout(yx, s, r, f"tree edit {k}")
+        g = render(yx, cuda, r, pos=(1.5, 1.4, 1.2), d=(-1.0, -0.9, -0.7))
+        o = orc.render(s.nodes(), s.GetRoot(), (1.5, 1.4, 1.2), (-1.0, -0.9, -0.7), fov=60.0, w=96, h=64,
+                       light_mode=2, light=(0.4, 0.3, 0.85))
+        assert np.array_equal(g["hit_id"].view(np.uint32), o["hit_id"])
+        assert np.array_equal(g["rgba"].view(np.uint32), o["rgba"])

@@ -250,13 +250,17 @@ def test_build_range_errors(yx):
     with pytest.raises(yx.YoxelError):
         s.BuildRanges([(0, 6, (10.0, 10.0, 10.0), 3.0, 0xFF0000FF), (0, 9, (1.0, 1.0, 1.0), 1.0, 0)])
     assert np.array_equal(s.nodes(), before)
-    # the recipe is gone once nodes were written by hand, or for a scene from nodes
+    # the recipe is gone once nodes were written by hand, and a scene from nodes
+    # has none: such scenes are edited in place only (as trees)
     s.write_nodes(0, before[0])
-    with pytest.raises(yx.YoxelError):
-        s.BuildRange(6, (1.0, 1.0, 1.0), 0, 1.0)
+    for how in (yx.SVOData.REBUILD, yx.SVOData.REBUILD_GPU):
+        with pytest.raises(yx.YoxelError):
+            s.BuildRange(6, (1.0, 1.0, 1.0), 0, 1.0, how=how, stream=0)
     t = yx.SVOData.from_nodes(before, 0)
     with pytest.raises(yx.YoxelError):
-        t.BuildRange(6, (1.0, 1.0, 1.0), 0, 1.0)
+        t.BuildRange(6, (1.0, 1.0, 1.0), 0, 1.0, how=yx.SVOData.REBUILD)
+    t.BuildRange(6, (40.0, 40.0, 50.0), 0, 3.0, 0xFF0000FF)
+    assert t.info()["num_leaves"] > s.info()["num_leaves"]
 
 
 def blob(shape, seed, scale=255):

@@ -1172,6 +1172,255 @@ extern "C" int yx_scene_build_synthetic(const yx_scene_desc* desc, yx_scene** ou
     return YX_OK;
 }
 
+// ---- edits of scenes without a recipe --------------------------------------------
+
+namespace {
+
+// A child slot: EMPTY, FULL (FullNode: solid, no surface), a leaf payload, or a
+// node id.
+struct Slot {
+    enum Kind : uint8_t { kEmptySlot, kFullSlot, kLeafSlot, kNodeSlot } kind;
+    uint32_t v;  // colour for kLeafSlot, id for kNodeSlot
+};
+
+inline Slot slot_of(const uint32_t* row, int c) {
+    const uint32_t ch = row[2 + c];
+    if ((row[0] >> c) & 1u) return Slot{Slot::kLeafSlot, ch};
+    if (ch == YX_FULL_NODE) return Slot{Slot::kFullSlot, ch};
+    if (ch & 0x80000000u) return Slot{Slot::kEmptySlot, YX_EMPTY_NODE};
+    return Slot{Slot::kNodeSlot, ch};
+}
+
+// Voxel states of a local grid: empty, solid inside, solid on the surface.
+enum : uint8_t { kVoxEmpty = 0, kVoxFull = 1, kVoxLeaf = 2 };
+
+// Dense state and colour of the voxels of the box [g0, g1] (finest level D).
+struct LocalGrid {
+    int g0[3], g1[3], ext[3];
+    std::vector<uint8_t> state;
+    std::vector<uint32_t> color;
+    size_t at(int i, int j, int k) const {
+        return (size_t)(i - g0[0]) + (size_t)ext[0] * ((size_t)(j - g0[1]) + (size_t)ext[1] * (size_t)(k - g0[2]));
+    }
+    bool inside(int i, int j, int k) const {
+        return i >= g0[0] && i <= g1[0] && j >= g0[1] && j <= g1[1] && k >= g0[2] && k <= g1[2];
+    }
+    void fill(const int o[3], int size, uint8_t st, uint32_t col) {
+        int a0[3], a1[3];
+        for (int a = 0; a < 3; ++a) {
+            a0[a] = std::max(o[a], g0[a]);
+            a1[a] = std::min(o[a] + size - 1, g1[a]);
+            if (a0[a] > a1[a]) return;
+        }
+        for (int k = a0[2]; k <= a1[2]; ++k)
+            for (int j = a0[1]; j <= a1[1]; ++j)
+                for (int i = a0[0]; i <= a1[0]; ++i) {
+                    const size_t x = at(i, j, k);
+                    state[x] = st;
+                    color[x] = col;
+                }
+    }
+};
+
+// Re-encodes the voxels of box [lo, hi] of a tree whose leaves are at most at
+// level `depth`: rewrites only the nodes whose cubes meet the box.
+struct TreeSplice {
+    Splice base;  // page marks, allocation and blanking
+    const LocalGrid& g;
+    const std::vector<uint8_t>& nstate;   // new state per grid voxel
+    const std::vector<uint32_t>& ncolor;  // and colour
+
+    int depth() const { return base.depth; }
+    bool meets(const int o[3], int size) const {
+        for (int a = 0; a < 3; ++a)
+            if (o[a] > base.hi[a] || o[a] + size - 1 < base.lo[a]) return false;
+        return true;
+    }
+    static bool is_leaf(const Slot& x) { return x.kind == Slot::kLeafSlot; }
+    void release_tree(uint32_t id) {
+        uint32_t row[10];
+        std::memcpy(row, &base.s.nodes[(size_t)id * 10], 40);
+        for (int c = 0; c < 8; ++c) {
+            const Slot k = slot_of(row, c);
+            if (k.kind == Slot::kNodeSlot) release_tree(k.v);
+            if (k.kind == Slot::kLeafSlot) base.leaf_delta -= 1;
+        }
+        base.release(id);
+    }
+
+    // slot `x` covering the cube at origin o of `level` (its children are
+    // voxels when level == depth - 1); returns the slot that replaces it
+    Slot visit(Slot x, int level, const int o[3]) {
+        const int size = 1 << (depth() - level);
+        if (!meets(o, size)) return x;
+        if (level == depth()) {
+            if (x.kind == Slot::kNodeSlot) release_tree(x.v);  // an empty chain below the leaf level
+            const size_t at = g.at(o[0], o[1], o[2]);
+            const uint8_t st = nstate[at];
+            const Slot y = st == kVoxLeaf ? Slot{Slot::kLeafSlot, ncolor[at]}
+                                          : (st == kVoxFull ? Slot{Slot::kFullSlot, YX_FULL_NODE}
+                                                            : Slot{Slot::kEmptySlot, YX_EMPTY_NODE});
+            base.leaf_delta += (int64_t)is_leaf(y) - (int64_t)is_leaf(x);
+            return y;
+        }
+        uint32_t old_row[10], row[10];
+        Slot kids[8];
+        if (x.kind == Slot::kNodeSlot) {
+            std::memcpy(old_row, &base.s.nodes[(size_t)x.v * 10], 40);
+            for (int c = 0; c < 8; ++c) kids[c] = slot_of(old_row, c);
+        } else {
+            // an empty, full or leaf cube seen as a node of 8 such children
+            for (int c = 0; c < 8; ++c) kids[c] = x;
+            if (x.kind == Slot::kLeafSlot) base.leaf_delta += 7;
+        }
+        const int half = size >> 1;
+        bool all_empty = true, all_full = true;
+        for (int c = 0; c < 8; ++c) {
+            const int co[3] = {o[0] + (c & 1) * half, o[1] + ((c >> 1) & 1) * half, o[2] + ((c >> 2) & 1) * half};
+            kids[c] = visit(kids[c], level + 1, co);
+            all_empty = all_empty && kids[c].kind == Slot::kEmptySlot;
+            all_full = all_full && kids[c].kind == Slot::kFullSlot;
+        }
+        if (all_empty || all_full) {
+            if (x.kind == Slot::kNodeSlot) base.release(x.v);
+            return all_empty ? Slot{Slot::kEmptySlot, YX_EMPTY_NODE} : Slot{Slot::kFullSlot, YX_FULL_NODE};
+        }
+        row[0] = 0;
+        row[1] = x.kind == Slot::kNodeSlot ? old_row[1] : 0u;
+        for (int c = 0; c < 8; ++c) {
+            row[2 + c] = kids[c].v;
+            if (kids[c].kind == Slot::kLeafSlot) row[0] |= 1u << c;
+            if (kids[c].kind == Slot::kEmptySlot || kids[c].kind == Slot::kFullSlot) row[0] |= 1u << (8 + c);
+        }
+        uint32_t id;
+        if (x.kind == Slot::kNodeSlot) {
+            id = x.v;
+            row[0] |= old_row[0] & ~0xFFFFu;  // keep the flag bits above the masks
+            if (std::memcmp(row, old_row, 40) == 0) return x;
+        } else {
+            id = base.alloc();
+        }
+        std::memcpy(&base.s.nodes[(size_t)id * 10], row, 40);
+        base.mark(id);
+        return Slot{Slot::kNodeSlot, id};
+    }
+};
+
+// Fills g with the solid of the subtree in slot x (cube at origin o of
+// `level`): leaf cubes and FullNode cubes are solid.
+void decode_into(const yx_scene& s, LocalGrid& g, Slot x, int level, int depth, const int o[3]) {
+    const int size = 1 << (depth - level);
+    for (int a = 0; a < 3; ++a)
+        if (o[a] > g.g1[a] || o[a] + size - 1 < g.g0[a]) return;
+    if (x.kind == Slot::kEmptySlot) return;
+    if (x.kind == Slot::kFullSlot) return g.fill(o, size, kVoxFull, 0u);
+    if (x.kind == Slot::kLeafSlot) return g.fill(o, size, kVoxLeaf, x.v);
+    if (level >= depth) return;  // a chain of empty nodes below the deepest leaf
+    const uint32_t* row = &s.nodes[(size_t)x.v * 10];
+    const int half = size >> 1;
+    for (int c = 0; c < 8; ++c) {
+        const int co[3] = {o[0] + (c & 1) * half, o[1] + ((c >> 1) & 1) * half, o[2] + ((c >> 2) & 1) * half};
+        decode_into(s, g, slot_of(row, c), level + 1, depth, co);
+    }
+}
+
+}  // namespace
+
+// Applies edits to a scene without a recipe in place. Its solid is its leaves
+// (a leaf slot covers its child cube) and its FullNode cubes; an edit unions
+// (GROW) or removes (CLEAR) the source's solid voxels, and the voxels of the
+// edit's box plus one are encoded again at the leaf level `depth`: surface
+// voxels (solid, with a 6-neighbour empty or outside the domain) as leaves,
+// other solid voxels as FullNode (whole FullNode cubes collapse), the rest
+// empty. A leaf keeps its colour; a grown voxel takes the source's; a solid
+// voxel that a CLEAR exposes takes the CLEAR's colour.
+static int edit_tree_in_place(yx_scene& s, const EditSet& set, int depth) {
+    const int n = 1 << depth;
+    for (size_t q = 0; q < set.edits.size(); ++q) {
+        const EditSphere& e = set.edits[q];
+        int lo[3] = {e.xlo, e.ylo, e.zlo}, hi[3] = {e.xhi, e.yhi, e.zhi};
+        bool empty = false;
+        for (int a = 0; a < 3; ++a) {
+            if (lo[a] > hi[a]) empty = true;
+            lo[a] = std::max(0, lo[a] - 1);
+            hi[a] = std::min(n - 1, hi[a] + 1);
+        }
+        if (empty) continue;
+        LocalGrid g;
+        size_t cells = 1;
+        for (int a = 0; a < 3; ++a) {
+            g.g0[a] = std::max(0, lo[a] - 1);
+            g.g1[a] = std::min(n - 1, hi[a] + 1);
+            g.ext[a] = g.g1[a] - g.g0[a] + 1;
+            cells *= (size_t)g.ext[a];
+        }
+        if (cells > ((size_t)1 << 28))
+            return fail(YX_ERR_RANGE, "edit %zu covers %zu voxels; a scene without a recipe takes at most 2^28 per edit",
+                        q, cells);
+        g.state.assign(cells, kVoxEmpty);
+        g.color.assign(cells, 0u);
+        const int o0[3] = {0, 0, 0};
+        const Slot root = s.root == YX_FULL_NODE ? Slot{Slot::kFullSlot, YX_FULL_NODE}
+                                                 : ((s.root & 0x80000000u) ? Slot{Slot::kEmptySlot, YX_EMPTY_NODE}
+                                                                           : Slot{Slot::kNodeSlot, s.root});
+        decode_into(s, g, root, 0, depth, o0);
+        // the edit, then the new encoding of the box
+        std::vector<uint8_t> solid(cells), nstate(cells, kVoxEmpty);
+        std::vector<uint32_t> ncolor(g.color);
+        for (int k = g.g0[2]; k <= g.g1[2]; ++k)
+            for (int j = g.g0[1]; j <= g.g1[1]; ++j)
+                for (int i = g.g0[0]; i <= g.g1[0]; ++i) {
+                    const size_t x = g.at(i, j, k);
+                    bool sol = g.state[x] != kVoxEmpty;
+                    uint32_t col = 0;
+                    if (i >= e.xlo && i <= e.xhi && j >= e.ylo && j <= e.yhi && k >= e.zlo && k <= e.zhi &&
+                        in_edit(set, e, (uint32_t)i, (uint32_t)j, (uint32_t)k, &col)) {
+                        sol = e.mode == YX_BUILD_GROW;
+                        if (sol) ncolor[x] = col;
+                        else ncolor[x] = 0;
+                        if (sol) g.state[x] = kVoxLeaf;  // grown: coloured
+                    }
+                    solid[x] = sol ? 1 : 0;
+                }
+        auto solid_at = [&](int i, int j, int k) -> bool {
+            if (i < 0 || j < 0 || k < 0 || i >= n || j >= n || k >= n) return false;
+            return solid[g.at(i, j, k)] != 0;
+        };
+        for (int k = lo[2]; k <= hi[2]; ++k)
+            for (int j = lo[1]; j <= hi[1]; ++j)
+                for (int i = lo[0]; i <= hi[0]; ++i) {
+                    const size_t x = g.at(i, j, k);
+                    if (!solid[x]) continue;
+                    const bool surf = !solid_at(i - 1, j, k) || !solid_at(i + 1, j, k) || !solid_at(i, j - 1, k) ||
+                                      !solid_at(i, j + 1, k) || !solid_at(i, j, k - 1) || !solid_at(i, j, k + 1);
+                    if (!surf) {
+                        nstate[x] = kVoxFull;
+                    } else {
+                        nstate[x] = kVoxLeaf;
+                        if (g.state[x] != kVoxLeaf) ncolor[x] = e.color;  // exposed by a CLEAR
+                    }
+                }
+        s.version += 1;
+        static const std::vector<uint64_t> no_codes;
+        static const std::vector<uint32_t> no_colors;
+        TreeSplice ts{Splice{s, depth, {lo[0], lo[1], lo[2]}, {hi[0], hi[1], hi[2]}, no_codes, no_colors, s.version},
+                      g, nstate, ncolor};
+        const Slot r = ts.visit(root, 0, o0);
+        if (r.kind == Slot::kNodeSlot) {
+            s.root = r.v;
+        } else if (r.kind == Slot::kLeafSlot) {
+            // a root that is one leaf cube: keep a node above it
+            throw std::logic_error("a leaf root");
+        } else {
+            s.root = r.v;
+        }
+        s.num_leaves = (uint64_t)((int64_t)s.num_leaves + ts.base.leaf_delta);
+    }
+    const size_t pages = (s.nodes.size() / 10 + kScenePageNodes - 1) / kScenePageNodes;
+    if (s.page_version.size() < pages) s.page_version.resize(pages, s.version);
+    return YX_OK;
+}
+
 // Applies the edits appended to s->edits from index `first` (volume data from
 // `vfirst`) by method `how`; on failure they are taken off again.
 static int apply_new_edits(yx_scene* s, size_t first, size_t vfirst, int32_t how, void* stream) {
@@ -1235,9 +1484,42 @@ static int apply_new_edits(yx_scene* s, size_t first, size_t vfirst, int32_t how
 static int check_range_call(yx_scene* s, int32_t how) {
     if (how != YX_RANGE_IN_PLACE && how != YX_RANGE_REBUILD && how != YX_RANGE_REBUILD_GPU)
         return fail(YX_ERR_INVALID_ARG, "unknown build-range method %d", how);
-    if (!s->has_recipe)
-        return fail(YX_ERR_INVALID_ARG, "the scene has no synthetic recipe (loaded, or edited node by node)");
+    if (!s->has_recipe && how != YX_RANGE_IN_PLACE)
+        return fail(YX_ERR_INVALID_ARG, "a scene without a synthetic recipe (loaded, or edited node by node) is "
+                                        "edited in place only");
     return YX_OK;
+}
+
+// Depth of the leaf level of a scene without a recipe (validating it if it
+// was edited node by node since).
+static int tree_depth(yx_scene* s, int* depth) {
+    if (s->depth_stale) {
+        yx_scene_info info;
+        const int rc = yx_scene_get_info(s, &info);
+        if (rc != YX_OK) return rc;
+    }
+    if (s->depth < 1) return fail(YX_ERR_INVALID_ARG, "the scene has no leaves, so no leaf level to edit at");
+    *depth = s->depth;
+    return YX_OK;
+}
+
+// Sphere or volume edits on a scene without a recipe (in place).
+template <class Add>
+static int edit_tree(yx_scene* s, Add&& add) {
+    int depth = 0;
+    int rc = tree_depth(s, &depth);
+    if (rc != YX_OK) return rc;
+    EditSet es;
+    try {
+        rc = add(depth, es);
+        if (rc != YX_OK) return rc;
+        rc = edit_tree_in_place(*s, es, depth);
+    } catch (const std::bad_alloc&) {
+        return fail(YX_ERR_OOM, "out of host memory editing the scene (the scene is partly edited)");
+    } catch (const std::exception& x) {
+        return fail(YX_ERR_RANGE, "%s (the scene is partly edited)", x.what());
+    }
+    return rc;
 }
 
 extern "C" int yx_scene_build_range(yx_scene* s, const yx_sphere_edit* edits, uint32_t count, int32_t how,
@@ -1246,6 +1528,8 @@ extern "C" int yx_scene_build_range(yx_scene* s, const yx_sphere_edit* edits, ui
     if (!s || (!edits && count)) return fail(YX_ERR_INVALID_ARG, "null argument");
     int rc = check_range_call(s, how);
     if (rc != YX_OK || count == 0) return rc;
+    if (!s->has_recipe)
+        return edit_tree(s, [&](int depth, EditSet& es) { return add_sphere_edits(edits, count, depth, es); });
     const size_t first = s->edits.edits.size(), vfirst = s->edits.vcol.size();
     try {
         rc = add_sphere_edits(edits, count, s->recipe.depth, s->edits);
@@ -1262,6 +1546,7 @@ extern "C" int yx_scene_build_range_volume(yx_scene* s, const yx_volume_source* 
     if (!s || !src) return fail(YX_ERR_INVALID_ARG, "null argument");
     int rc = check_range_call(s, how);
     if (rc != YX_OK) return rc;
+    if (!s->has_recipe) return edit_tree(s, [&](int depth, EditSet& es) { return add_volume_edit(*src, depth, es); });
     const size_t first = s->edits.edits.size(), vfirst = s->edits.vcol.size();
     try {
         rc = add_volume_edit(*src, s->recipe.depth, s->edits);
