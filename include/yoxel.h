@@ -152,6 +152,14 @@ int yx_scene_build_range(yx_scene* scene, const yx_sphere_edit* edits, uint32_t 
  * value < iso (inside == 0), coloured `color`; RawSource alpha != 0, coloured
  * by the voxel. GROW unions them, CLEAR subtracts them, in order with the
  * sphere edits, by the same methods. The data are copied. */
+/* Solid interior of a tree without a recipe whose solid is only its surface
+ * (a saved synthetic scene, for one): the empty space that a 6-connected flood
+ * from the domain's faces does not reach becomes FullNode, so that edits see
+ * a solid (yx_scene_build_range). Leaves and rendering are unchanged. Each
+ * empty child cube is one connected empty region and changes whole; *filled
+ * (may be NULL) receives how many. Up to depth 11 (two bit volumes of the
+ * leaf grid in host memory). No counterpart in the reference. */
+int yx_scene_fill_interior(yx_scene* scene, uint64_t* filled);
 #define YX_SOURCE_ISO 1
 #define YX_SOURCE_RAW 2
 typedef struct yx_volume_source {

@@ -189,3 +189,25 @@ def test_tree_edits_sync_and_render(yx, cuda, orc):
                        light_mode=2, light=(0.4, 0.3, 0.85))
         assert np.array_equal(g["hit_id"].view(np.uint32), o["hit_id"])
         assert np.array_equal(g["rgba"].view(np.uint32), o["rgba"])
+
+
+def test_filled_tree_renders_the_same(yx, cuda, orc):
+    """Filling a loaded surface-only tree changes no render (FullNode is not
+    hit), and its edits sync and render like any other."""
+    syn = yx.SVOData.build_synthetic(depth=8, num_spheres=24, radius_min=4.0, radius_max=22.0, heightfield=True,
+                                     seed=13)
+    s = yx.SVOData.from_nodes(syn.nodes(), syn.GetRoot())
+    r = same_layout(yx, s, what="surface-only tree")
+    g0 = render(yx, cuda, r, pos=(1.5, 1.4, 1.2), d=(-1.0, -0.9, -0.7))
+    assert s.fill_interior() > 0
+    same_layout(yx, s, r, "filled tree")
+    g1 = render(yx, cuda, r, pos=(1.5, 1.4, 1.2), d=(-1.0, -0.9, -0.7))
+    for k in g0:
+        assert np.array_equal(g0[k], g1[k]), k
+    s.BuildRange(8, (128.0, 128.0, 40.0), 1, 20.0, 0xFF808080)   # a crater with solid walls
+    same_layout(yx, s, r, "crater")
+    g2 = render(yx, cuda, r, pos=(1.5, 1.4, 1.2), d=(-1.0, -0.9, -0.7))
+    o = orc.render(s.nodes(), s.GetRoot(), (1.5, 1.4, 1.2), (-1.0, -0.9, -0.7), fov=60.0, w=96, h=64,
+                   light_mode=2, light=(0.4, 0.3, 0.85))
+    assert np.array_equal(g2["hit_id"].view(np.uint32), o["hit_id"])
+    assert np.array_equal(g2["rgba"].view(np.uint32), o["rgba"])

@@ -262,6 +262,7 @@ SIGNATURES = {
     "yx_voxtree_pack_bricks": (ctypes.c_int, [_P, _P, ctypes.c_int32, _P, ctypes.c_uint32, _P]),
     "yx_voxtree_classify": (ctypes.c_int, [_P, ctypes.c_int32, _P, _P, _P, _P, _P, ctypes.c_uint32,
                                            ctypes.POINTER(ctypes.c_uint32), _P]),
+    "yx_scene_fill_interior": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "yx_scene_build_range_volume": (ctypes.c_int, [_P, ctypes.POINTER(VolumeSource), ctypes.c_int32, _P]),
     "yx_scene_build_range": (ctypes.c_int, [_P, ctypes.POINTER(SphereEdit), ctypes.c_uint32, ctypes.c_int32, _P]),
     "yx_scene_write_nodes": (ctypes.c_int, [_P, ctypes.c_uint32, _P, ctypes.c_uint32]),

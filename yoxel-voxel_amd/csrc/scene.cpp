@@ -1421,6 +1421,151 @@ static int edit_tree_in_place(yx_scene& s, const EditSet& set, int depth) {
     return YX_OK;
 }
 
+static int tree_depth(yx_scene* s, int* depth);
+
+// ---- solid interior of a surface-only tree ---------------------------------------
+
+namespace {
+
+struct BitVol {
+    int n = 0;
+    std::vector<uint64_t> w;
+    void init(int side) {
+        n = side;
+        w.assign(((size_t)side * side * side + 63) / 64, 0ull);
+    }
+    size_t at(int x, int y, int z) const { return (size_t)x + (size_t)n * ((size_t)y + (size_t)n * (size_t)z); }
+    bool get(size_t i) const { return (w[i >> 6] >> (i & 63)) & 1ull; }
+    void set(size_t i) { w[i >> 6] |= 1ull << (i & 63); }
+    void fill(const int o[3], int size) {
+        for (int z = o[2]; z < o[2] + size; ++z)
+            for (int y = o[1]; y < o[1] + size; ++y)
+                for (int x = o[0]; x < o[0] + size; ++x) set(at(x, y, z));
+    }
+};
+
+void mark_solid(const yx_scene& s, BitVol& b, Slot x, int level, int depth, const int o[3]) {
+    const int size = 1 << (depth - level);
+    if (x.kind == Slot::kEmptySlot) return;
+    if (x.kind == Slot::kFullSlot || x.kind == Slot::kLeafSlot) return b.fill(o, size);
+    if (level >= depth) return;  // an empty chain below the deepest leaf
+    const uint32_t* row = &s.nodes[(size_t)x.v * 10];
+    const int half = size >> 1;
+    for (int c = 0; c < 8; ++c) {
+        const int co[3] = {o[0] + (c & 1) * half, o[1] + ((c >> 1) & 1) * half, o[2] + ((c >> 2) & 1) * half};
+        mark_solid(s, b, slot_of(row, c), level + 1, depth, co);
+    }
+}
+
+// 6-connected flood of the empty voxels reachable from the domain's faces,
+// by x-spans: a span is filled whole, then the rows y +- 1 and z +- 1 over it
+// are scanned for unfilled empty runs.
+void flood_outside(const BitVol& solid, BitVol& out) {
+    const int n = solid.n;
+    struct Span {
+        int x0, x1, y, z;
+    };
+    std::vector<Span> stack;
+    auto open = [&](int x, int y, int z) {
+        const size_t i = solid.at(x, y, z);
+        return !solid.get(i) && !out.get(i);
+    };
+    auto fill_from = [&](int x, int y, int z) {
+        int x0 = x, x1 = x;
+        while (x0 > 0 && open(x0 - 1, y, z)) --x0;
+        while (x1 < n - 1 && open(x1 + 1, y, z)) ++x1;
+        for (int k = x0; k <= x1; ++k) out.set(out.at(k, y, z));
+        stack.push_back(Span{x0, x1, y, z});
+    };
+    auto scan_row = [&](int x0, int x1, int y, int z) {
+        if (y < 0 || z < 0 || y >= n || z >= n) return;
+        for (int x = x0; x <= x1; ++x)
+            if (open(x, y, z)) fill_from(x, y, z);
+    };
+    auto drain = [&]() {
+        while (!stack.empty()) {
+            const Span sp = stack.back();
+            stack.pop_back();
+            scan_row(sp.x0, sp.x1, sp.y - 1, sp.z);
+            scan_row(sp.x0, sp.x1, sp.y + 1, sp.z);
+            scan_row(sp.x0, sp.x1, sp.y, sp.z - 1);
+            scan_row(sp.x0, sp.x1, sp.y, sp.z + 1);
+        }
+    };
+    // seeds: every open voxel on the six faces
+    for (int a = 0; a < n; ++a)
+        for (int b2 = 0; b2 < n; ++b2) {
+            const int pts[6][3] = {{0, a, b2}, {n - 1, a, b2}, {a, 0, b2}, {a, n - 1, b2}, {a, b2, 0}, {a, b2, n - 1}};
+            for (const auto& p : pts)
+                if (open(p[0], p[1], p[2])) {
+                    fill_from(p[0], p[1], p[2]);
+                    drain();
+                }
+        }
+}
+
+// Turns the empty slots whose cube the flood did not reach into FullNode.
+// An empty cube is one connected empty region, so one voxel decides it.
+uint64_t mark_enclosed(yx_scene& s, const BitVol& out, Splice& sp, uint32_t id, int level, int depth, const int o[3]) {
+    uint32_t* row = &s.nodes[(size_t)id * 10];
+    const int half = (1 << (depth - level)) >> 1;
+    uint64_t changed = 0;
+    bool dirty = false;
+    for (int c = 0; c < 8; ++c) {
+        const int co[3] = {o[0] + (c & 1) * half, o[1] + ((c >> 1) & 1) * half, o[2] + ((c >> 2) & 1) * half};
+        const Slot k = slot_of(row, c);
+        if (k.kind == Slot::kEmptySlot) {
+            if (!out.get(out.at(co[0], co[1], co[2]))) {
+                row[2 + c] = YX_FULL_NODE;
+                row[0] |= 1u << (8 + c);
+                dirty = true;
+                ++changed;
+            }
+        } else if (k.kind == Slot::kNodeSlot && level + 1 < depth) {
+            changed += mark_enclosed(s, out, sp, k.v, level + 1, depth, co);
+            row = &s.nodes[(size_t)id * 10];
+        }
+    }
+    if (dirty) sp.mark(id);
+    return changed;
+}
+
+}  // namespace
+
+extern "C" int yx_scene_fill_interior(yx_scene* s, uint64_t* filled) {
+    clear_error();
+    if (!s) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (s->has_recipe)
+        return fail(YX_ERR_INVALID_ARG, "a synthetic scene keeps its solid in its recipe; fill only trees without one");
+    int depth = 0;
+    int rc = tree_depth(s, &depth);
+    if (rc != YX_OK) return rc;
+    if (depth > 11) return fail(YX_ERR_RANGE, "depth %d: the dense flood handles up to 2048^3 voxels", depth);
+    uint64_t count = 0;
+    try {
+        BitVol solid, out;
+        solid.init(1 << depth);
+        out.init(1 << depth);
+        const int o0[3] = {0, 0, 0};
+        const Slot root = s->root == YX_FULL_NODE ? Slot{Slot::kFullSlot, YX_FULL_NODE}
+                                                  : ((s->root & 0x80000000u) ? Slot{Slot::kEmptySlot, YX_EMPTY_NODE}
+                                                                             : Slot{Slot::kNodeSlot, s->root});
+        mark_solid(*s, solid, root, 0, depth, o0);
+        flood_outside(solid, out);
+        if (root.kind == Slot::kNodeSlot) {
+            s->version += 1;
+            static const std::vector<uint64_t> no_codes;
+            static const std::vector<uint32_t> no_colors;
+            Splice sp{*s, depth, {0, 0, 0}, {0, 0, 0}, no_codes, no_colors, s->version};
+            count = mark_enclosed(*s, out, sp, root.v, 0, depth, o0);
+        }
+    } catch (const std::bad_alloc&) {
+        return fail(YX_ERR_OOM, "out of host memory for the %d-level flood", depth);
+    }
+    if (filled) *filled = count;
+    return YX_OK;
+}
+
 // Applies the edits appended to s->edits from index `first` (volume data from
 // `vfirst`) by method `how`; on failure they are taken off again.
 static int apply_new_edits(yx_scene* s, size_t first, size_t vfirst, int32_t how, void* stream) {

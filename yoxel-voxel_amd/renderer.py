@@ -117,6 +117,13 @@ class SVOData:
         st = ctypes.c_void_p(_stream(stream) if how == _lib.YX_RANGE_REBUILD_GPU else 0)
         check(lib().yx_scene_build_range(self._h, arr, len(edits), int(how), st))
 
+    def fill_interior(self):
+        """Mark the empty space enclosed by a surface-only tree as FullNode (see
+        yx_scene_fill_interior); returns how many empty cubes became solid."""
+        k = ctypes.c_uint64()
+        check(lib().yx_scene_fill_interior(self._h, ctypes.byref(k)))
+        return k.value
+
     def BuildRangeIso(self, level, pos, mode, values, iso, inside=True, color=0xFFC0B6C0, how=IN_PLACE,
                       stream=None):
         """BuildRange with an IsoSource (scene_gen.py:111-130): `values` uint8
