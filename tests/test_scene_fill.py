@@ -84,3 +84,14 @@ def test_fill_closes_a_cavity(yx):
     assert np.array_equal(st > 0, exp)
     assert (exp & ~shell).sum() > 1500  # the cavity
     assert np.array_equal(st == 2, state == 2)
+
+
+def test_fill_on_word_rows(yx):
+    """Depth 7: rows of two 64-bit words, runs across word edges, pockets."""
+    desc = dict(depth=7, num_spheres=60, radius_min=3.0, radius_max=14.0, heightfield=True, hf_octaves=4,
+                hf_amplitude=0.25, hf_base=0.3, seed=21)
+    syn = yx.SVOData.build_synthetic(**desc)
+    s = yx.SVOData.from_nodes(syn.nodes(), syn.GetRoot())
+    s.fill_interior()
+    st, _ = H.decode_tree(s.nodes(), s.GetRoot(), 7)
+    assert np.array_equal(st > 0, expected_solid(H.synthetic_inside(**desc)))

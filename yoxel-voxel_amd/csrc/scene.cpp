@@ -1459,28 +1459,101 @@ void mark_solid(const yx_scene& s, BitVol& b, Slot x, int level, int depth, cons
 
 // 6-connected flood of the empty voxels reachable from the domain's faces,
 // by x-spans: a span is filled whole, then the rows y +- 1 and z +- 1 over it
-// are scanned for unfilled empty runs.
+// are scanned for unfilled empty runs. Rows of 64 voxels or more are whole
+// 64-bit words, and spans are found and filled a word at a time.
 void flood_outside(const BitVol& solid, BitVol& out) {
     const int n = solid.n;
     struct Span {
         int x0, x1, y, z;
     };
     std::vector<Span> stack;
+    const bool words = n % 64 == 0;
+    const int wpr = n / 64;  // words per row
+    auto row_word = [&](int y, int z) { return ((size_t)y + (size_t)n * (size_t)z) * (size_t)wpr; };
+    auto open_bits = [&](size_t w) { return ~solid.w[w] & ~out.w[w]; };
     auto open = [&](int x, int y, int z) {
         const size_t i = solid.at(x, y, z);
         return !solid.get(i) && !out.get(i);
     };
+    // fills the open run through x of row (y, z), pushes it as a span
     auto fill_from = [&](int x, int y, int z) {
         int x0 = x, x1 = x;
-        while (x0 > 0 && open(x0 - 1, y, z)) --x0;
-        while (x1 < n - 1 && open(x1 + 1, y, z)) ++x1;
-        for (int k = x0; k <= x1; ++k) out.set(out.at(k, y, z));
+        if (!words) {
+            while (x0 > 0 && open(x0 - 1, y, z)) --x0;
+            while (x1 < n - 1 && open(x1 + 1, y, z)) ++x1;
+            for (int k = x0; k <= x1; ++k) out.set(out.at(k, y, z));
+        } else {
+            const size_t base = row_word(y, z);
+            // left end: highest closed bit below x, word by word
+            int w = x >> 6, b = x & 63;
+            for (;;) {
+                const uint64_t below = b >= 64 ? ~0ull : (b ? ((1ull << b) - 1ull) : 0ull);
+                const uint64_t closed = ~open_bits(base + w) & below;
+                if (closed) {
+                    x0 = (w << 6) + (63 - __builtin_clzll(closed)) + 1;
+                    break;
+                }
+                if (w == 0) {
+                    x0 = 0;
+                    break;
+                }
+                --w;
+                b = 64;
+                if (open_bits(base + w) == ~0ull) continue;  // whole word open: keep going
+                const uint64_t c2 = ~open_bits(base + w);
+                x0 = (w << 6) + (63 - __builtin_clzll(c2)) + 1;
+                break;
+            }
+            // right end: lowest closed bit above x
+            w = x >> 6;
+            b = x & 63;
+            for (;;) {
+                const uint64_t above = b == 63 ? 0ull : (~0ull << (b + 1));
+                const uint64_t closed = ~open_bits(base + w) & above;
+                if (closed) {
+                    x1 = (w << 6) + __builtin_ctzll(closed) - 1;
+                    break;
+                }
+                if (w == wpr - 1) {
+                    x1 = n - 1;
+                    break;
+                }
+                ++w;
+                b = -1;
+                if (open_bits(base + w) == ~0ull) {
+                    if (w == wpr - 1) {
+                        x1 = n - 1;
+                        break;
+                    }
+                    b = 63;  // whole word open: test the next one
+                    continue;
+                }
+                const uint64_t c2 = ~open_bits(base + w);
+                x1 = (w << 6) + __builtin_ctzll(c2) - 1;
+                break;
+            }
+            for (int ww = x0 >> 6; ww <= (x1 >> 6); ++ww) {
+                const int lo = std::max(x0, ww << 6) - (ww << 6), hi = std::min(x1, (ww << 6) + 63) - (ww << 6);
+                const uint64_t m = (hi == 63 ? ~0ull : ((1ull << (hi + 1)) - 1ull)) & (~0ull << lo);
+                out.w[base + ww] |= m;
+            }
+        }
         stack.push_back(Span{x0, x1, y, z});
     };
     auto scan_row = [&](int x0, int x1, int y, int z) {
         if (y < 0 || z < 0 || y >= n || z >= n) return;
-        for (int x = x0; x <= x1; ++x)
-            if (open(x, y, z)) fill_from(x, y, z);
+        if (!words) {
+            for (int x = x0; x <= x1; ++x)
+                if (open(x, y, z)) fill_from(x, y, z);
+            return;
+        }
+        const size_t base = row_word(y, z);
+        for (int ww = x0 >> 6; ww <= (x1 >> 6); ++ww) {
+            const int lo = std::max(x0, ww << 6) - (ww << 6), hi = std::min(x1, (ww << 6) + 63) - (ww << 6);
+            const uint64_t range = (hi == 63 ? ~0ull : ((1ull << (hi + 1)) - 1ull)) & (~0ull << lo);
+            uint64_t m;
+            while ((m = open_bits(base + ww) & range) != 0) fill_from((ww << 6) + __builtin_ctzll(m), y, z);
+        }
     };
     auto drain = [&]() {
         while (!stack.empty()) {
@@ -1492,16 +1565,28 @@ void flood_outside(const BitVol& solid, BitVol& out) {
             scan_row(sp.x0, sp.x1, sp.y, sp.z + 1);
         }
     };
-    // seeds: every open voxel on the six faces
+    // seeds: every open voxel on the six faces (x faces through the y, z rows)
     for (int a = 0; a < n; ++a)
         for (int b2 = 0; b2 < n; ++b2) {
-            const int pts[6][3] = {{0, a, b2}, {n - 1, a, b2}, {a, 0, b2}, {a, n - 1, b2}, {a, b2, 0}, {a, b2, n - 1}};
-            for (const auto& p : pts)
-                if (open(p[0], p[1], p[2])) {
-                    fill_from(p[0], p[1], p[2]);
-                    drain();
-                }
+            if (open(0, a, b2)) {
+                fill_from(0, a, b2);
+                drain();
+            }
+            if (open(n - 1, a, b2)) {
+                fill_from(n - 1, a, b2);
+                drain();
+            }
         }
+    for (int a = 0; a < n; ++a) {
+        scan_row(0, n - 1, 0, a);
+        drain();
+        scan_row(0, n - 1, n - 1, a);
+        drain();
+        scan_row(0, n - 1, a, 0);
+        drain();
+        scan_row(0, n - 1, a, n - 1);
+        drain();
+    }
 }
 
 // Turns the empty slots whose cube the flood did not reach into FullNode.
