@@ -175,7 +175,19 @@ struct SampleIter {
 // Steps taken per wave iteration; their parameter loads are issued together.
 constexpr int kCompBatch = 4;
 
-__device__ __forceinline__ float softplus(float x) { return x > 15.0f ? x : log1pf(expf(x)); }
+// Opacity of one sample with sigma = softplus(raw), without forming sigma:
+// alpha = 1 - exp(-sigma) = sigmoid(raw) and exp(-sigma) = sigmoid(-raw).
+// With n = exp(-|raw|) in (0, 1] and r = 1 / (1 + n) = sigmoid(|raw|), one
+// is r and the other n * r, so a sample costs one exponential and one
+// division (no log1p / expm1, no cancellation for either sign of raw).
+// alpha is also the chain factor dsigma/draw = sigmoid(raw) of the adjoint.
+__device__ __forceinline__ void opacity(float raw, float& alpha, float& keep) {
+    const float n = expf(-fabsf(raw));
+    const float r = 1.0f / (1.0f + n);
+    const float nr = n * r;
+    alpha = raw >= 0.0f ? r : nr;
+    keep = raw >= 0.0f ? nr : r;
+}
 
 // Thread t of a launch takes lane t % 64 of wave t / 64; each wave covers one
 // 8x8 pixel tile (tiles row-major per view, views in order) so its rays stay
@@ -219,9 +231,8 @@ __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
 #pragma unroll
         for (int j = 0; j < kCompBatch; ++j) {
             if (got[j]) {
-                const float sig = softplus(q[j].x);
-                const float e = expf(-sig);
-                const float al = -expm1f(-sig);
+                float al, e;
+                opacity(q[j].x, al, e);
                 const float wgt = T * al;
                 cr += wgt * q[j].y;
                 cg += wgt * q[j].z;
@@ -291,9 +302,8 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
         for (int j = 0; j < kCompBatch; ++j) {
             const uint64_t mask = __ballot(got[j]);
             if (got[j]) {
-                const float sig = softplus(q[j].x);
-                const float e = expf(-sig);
-                const float al = -expm1f(-sig);
+                float al, e;
+                opacity(q[j].x, al, e);
                 const float wgt = T * al;
                 pr += wgt * q[j].y;
                 pg += wgt * q[j].z;
@@ -301,11 +311,10 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
                 const float tn = T * e;
                 const float sr = fr - pr, sg = fg - pg, sb = fb - pb;
                 const float dsig = gr * (tn * q[j].y - sr) + gg * (tn * q[j].z - sg) + gb * (tn * q[j].w - sb);
-                const float sgm = 1.0f / (1.0f + expf(-q[j].x));
                 const uint32_t pos = cursor + lanes_rank(mask);
                 if (pos < a.total_records) {
                     a.keys[pos] = ix[j];
-                    a.vals[pos] = make_float4(dsig * sgm, wgt * gr, wgt * gg, wgt * gb);
+                    a.vals[pos] = make_float4(dsig * al, wgt * gr, wgt * gg, wgt * gb);
                 }
                 T = tn;
             }
