@@ -278,11 +278,12 @@ def composite_bench(yx, torch, dev, steps, with_cpu=False):
     nvox = params.shape[0]
     # algorithmic bytes: forward 16 B per sample + 20 B per ray out; backward
     # 16 B per sample recomputed, 36 B per ray in (dL/dC, C, offset, count), a
-    # 24 B record per sample written (key, id, float4), 3 radix passes reading
-    # and writing the 8 B (key, id) pairs, the reduce (24 B in per record),
-    # 16 B out per key and the zeroed gradient (16 B per voxel)
+    # 20 B record per sample written (key, float4), 2 radix passes over the 15
+    # brick-slot bits reading and writing the 8 B (key, id) pairs, the per-slot
+    # reduction (24 B in per record: key, id, float4) writing every voxel of
+    # every allocated brick (16 B per voxel)
     fwd_bytes = 16 * records + 20 * rays
-    bwd_bytes = 16 * records + 36 * rays + 24 * records + 3 * 16 * records + 24 * records + 16 * unique + 16 * nvox
+    bwd_bytes = 16 * records + 36 * rays + 20 * records + 2 * 16 * records + 24 * records + 16 * nvox
     res = {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "workload": "composite_512",
            "rays_per_step": rays, "ms_per_step": round(dt * 1e3, 3), "fwd_ms": round(fwd_ms, 3),
            "bwd_ms": round(bwd_ms, 3), "records": records, "unique_voxels": unique,

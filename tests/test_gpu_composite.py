@@ -61,6 +61,7 @@ def run_parity(yx, orc_c, torch, grid, brick, occupancy, seed, n_views, w, h, ns
 
     # voxels some ray sampled, 256 of them at random
     touched = np.nonzero(np.abs(g1n).sum(axis=1) > 0)[0]
+    assert touched.size <= unique, f"{touched.size} voxels with a gradient, {unique} distinct keys counted"
     rng = np.random.default_rng(seed)
     sel = rng.choice(touched, size=min(nsel, touched.size), replace=False)
     loss, ref, rays = orc_c.grad(v.table, v.params, grid, brick, views, w, h, target, sel, max_rays=4096,
@@ -85,6 +86,13 @@ def test_small_grid_parity(yx, oracle_mod, cuda):
 def test_uneven_bricks_and_dense_volume(yx, oracle_mod, cuda):
     # brick 4, most bricks allocated, odd resolution
     run_parity(yx, oracle_mod.CompositeOracle(), cuda, 32, 4, 0.9, 8, 3, 33, 17, nsel=64)
+
+
+@pytest.mark.parametrize("grid,brick,occupancy", [(16, 1, 0.5), (32, 2, 0.6), (64, 16, 0.5)])
+def test_brick_sizes(yx, oracle_mod, cuda, grid, brick, occupancy):
+    # bricks of 1 and 2 voxels go through the per-slot reduction with 1 and 8
+    # voxel sums per slot; 16^3 bricks through the whole-key sort and run sums
+    run_parity(yx, oracle_mod.CompositeOracle(), cuda, grid, brick, occupancy, 5, 2, 24, 20, nsel=64)
 
 
 def test_composite_512_full(yx, oracle_mod, cuda):

@@ -19,9 +19,12 @@
 //   dL/draw_k   = dL/dsigma_k * sigmoid(raw_k)
 // The backward recomputes the exact discrete sample sequence instead of
 // storing per-ray lists, writes one (voxel key, float4 gradient) record per
-// sample at the ray's exclusive-scan offset, radix-sorts the records by key
-// (with their ids, 0..n-1 from a buffer written once per size) and reduces
-// each key's run in sorted order: no atomics, deterministic.
+// sample at the ray's exclusive-scan offset, and radix-sorts the records
+// (with their ids, 0..n-1 from a buffer written once per size) by the brick
+// slot bits of the key. One wave per slot then sums each voxel's records in
+// emission order in LDS and writes the brick. Bricks above 8^3 sort by the
+// whole key and sum each key's run instead. No atomics on gradients; the
+// result is deterministic.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
@@ -355,7 +358,8 @@ __global__ void comp_zero_kernel(float4* g, size_t n) {
 // the wave holding its first record, reading on serially; waves that only
 // hold the continuation of an earlier run write nothing for it.
 __global__ __launch_bounds__(256) void comp_reduce_runs_kernel(const uint32_t* keys, const uint32_t* ids,
-                                                               const float4* vals, uint32_t n, float4* grad) {
+                                                               const float4* vals, uint32_t n, float4* grad,
+                                                               uint32_t nvox) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = (int)(threadIdx.x & 63);
     const bool in = i < n;
@@ -392,7 +396,105 @@ __global__ __launch_bounds__(256) void comp_reduce_runs_kernel(const uint32_t* k
         s.z += v.z;
         s.w += v.w;
     }
-    grad[key] = s;
+    if (key < nvox) grad[key] = s;  // a record no sample wrote holds no valid key
+}
+
+// Record ranges per brick slot after the records are sorted by slot (key >>
+// shift): start[s] is the first record of slot s, start[num_slots] = n. Thread
+// i looks at the boundary between records i - 1 and i and fills the starts of
+// every slot in between, so empty slots get the start of the next non-empty
+// one. Thread 0 also clears the distinct-voxel counter of the slot reduction.
+// The slot is read from the sorted bit range and clamped below num_slots, so
+// even records that no sample wrote (sample counts that do not match the
+// forward's) cannot index past start[num_slots].
+__global__ __launch_bounds__(256) void comp_slot_starts_kernel(const uint32_t* keys, uint32_t n, int shift,
+                                                               uint32_t slot_mask, uint32_t num_slots,
+                                                               uint32_t* start, uint32_t* distinct) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == 0) *distinct = 0u;
+    auto slot_of = [&](uint32_t k) { return min((k >> shift) & slot_mask, num_slots - 1u); };
+    const uint32_t s = i < n ? slot_of(keys[i]) : num_slots;
+    const uint32_t lo = i == 0 ? 0u : slot_of(keys[i - 1]) + 1u;
+    for (uint32_t q = lo; q <= s; ++q) start[q] = i;
+}
+
+__device__ __forceinline__ float4 shfl4(float4 v, int src) {
+    return make_float4(__shfl(v.x, src), __shfl(v.y, src), __shfl(v.z, src), __shfl(v.w, src));
+}
+
+// Sum the records of one brick slot per wave (records sorted by slot, stable,
+// so a slot's records are in emission order). The slot's 2^VB voxel sums live
+// in LDS. The wave takes 64 records at a time; lanes holding the same voxel
+// find each other with VB ballots, the lowest of them adds the group's values
+// in lane order and then adds that to the voxel's sum. Every sum is thus built
+// in emission order, whatever the schedule: the result is the same on every
+// call. The wave then writes the whole brick, untouched voxels as 0, and
+// counts its distinct voxels.
+template <int VB>
+__global__ __launch_bounds__(256) void comp_reduce_slots_kernel(const uint32_t* keys, const uint32_t* ids,
+                                                                const float4* vals, const uint32_t* start,
+                                                                uint32_t num_slots, float4* grad,
+                                                                uint32_t* distinct) {
+    constexpr int NV = 1 << VB;
+    constexpr int NW = (NV + 31) / 32;  // touched-voxel bit words
+    __shared__ float4 acc_all[4][NV];
+    __shared__ uint32_t seen_all[4][NW];
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const uint32_t slot = blockIdx.x * 4u + (uint32_t)w;
+    if (slot >= num_slots) return;  // whole wave; the waves of a block never synchronise
+    float4* acc = acc_all[w];
+    uint32_t* seen = seen_all[w];
+    for (int v = lane; v < NV; v += 64) acc[v] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    for (int v = lane; v < NW; v += 64) seen[v] = 0u;
+    const uint32_t b = start[slot], e = start[slot + 1];
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t base = b; base < e; base += 64u) {
+        const uint32_t i = base + (uint32_t)lane;
+        const bool in = i < e;
+        uint32_t v = 0;
+        float4 x = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (in) {
+            v = keys[i] & (uint32_t)(NV - 1);
+            x = vals[ids[i]];
+        }
+        uint64_t peers = __ballot(in);
+#pragma unroll
+        for (int bit = 0; bit < VB; ++bit) {
+            const bool on = (v >> bit) & 1u;
+            const uint64_t m = __ballot(on);
+            peers &= on ? m : ~m;
+        }
+        const bool leader = in && (peers & below) == 0;
+        uint64_t rest = leader ? (peers & ~(below | (1ull << lane))) : 0ull;
+        float4 s = x;
+        while (__ballot(rest != 0ull) != 0ull) {
+            const int src = rest != 0ull ? __ffsll((unsigned long long)rest) - 1 : lane;
+            const float4 y = shfl4(x, src);
+            if (rest != 0ull) {
+                s.x += y.x;
+                s.y += y.y;
+                s.z += y.z;
+                s.w += y.w;
+                rest &= rest - 1ull;
+            }
+        }
+        if (leader) {
+            float4 t = acc[v];
+            t.x += s.x;
+            t.y += s.y;
+            t.z += s.z;
+            t.w += s.w;
+            acc[v] = t;
+            atomicOr(&seen[v >> 5], 1u << (v & 31u));
+        }
+    }
+    float4* g = grad + (size_t)slot * NV;
+    for (int v = lane; v < NV; v += 64) g[v] = acc[v];
+    uint32_t cnt = 0;
+    for (int k = lane; k < NW; k += 64) cnt += (uint32_t)__builtin_popcount(seen[k]);
+    for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
+    if (lane == 0 && cnt) atomicAdd(distinct, cnt);
 }
 
 }  // namespace yx
@@ -411,6 +513,7 @@ struct yx_compositor {
     uint64_t last_records = 0;
     const uint32_t* sorted_keys = nullptr;  // the last backward's sorted keys (workspace)
     uint32_t* counter = nullptr;
+    bool counted = false;  // the last backward's slot reduction left the distinct count in `counter`
     uint32_t* d_iota = nullptr;  // record ids 0..iota_n-1, written once per size
     size_t iota_n = 0;
 };
@@ -475,7 +578,11 @@ extern "C" int yx_compositor_last_stats(const yx_compositor* c, uint64_t* record
     if (!c || !records || !unique_keys) return fail(YX_ERR_INVALID_ARG, "null argument");
     *records = c->last_records;
     *unique_keys = 0;
-    if (c->sorted_keys && c->last_records) {
+    if (c->counted && c->last_records) {
+        uint32_t runs = 0;
+        YX_HIP(hipMemcpy(&runs, c->counter, 4, hipMemcpyDeviceToHost));
+        *unique_keys = runs;
+    } else if (c->sorted_keys && c->last_records) {
         YX_HIP(hipMemset(c->counter, 0, 4));
         const uint32_t n = (uint32_t)c->last_records;
         hipLaunchKernelGGL(comp_count_runs_kernel, dim3(1024), dim3(256), 0, nullptr, c->sorted_keys, n, c->counter);
@@ -559,26 +666,37 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     if (total >= 0x7FFFFFFFull) return fail(YX_ERR_RANGE, "%llu records exceed 2^31", (unsigned long long)total);
     c->last_records = total;
 
-    // zero the gradient (voxels without samples keep 0)
-    {
+    // Bricks of up to 8^3 voxels are reduced per slot (records sorted by the
+    // slot bits of the key only); larger bricks sort the whole key and sum
+    // each key's run.
+    const int vox_bits = 3 * __builtin_ctz((unsigned)vol->brick);
+    const bool by_slot = vox_bits <= 9;
+    // zero the gradient (voxels without samples keep 0); the slot reduction
+    // writes every voxel itself
+    if (!by_slot || total == 0) {
         const size_t n4 = nvox;
         if (n4) hipLaunchKernelGGL(comp_zero_kernel, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st,
                                    (float4*)grad_params, n4);
         YX_HIP(hipGetLastError());
     }
     c->sorted_keys = nullptr;
+    c->counted = false;
     if (total == 0) return YX_OK;
 
-    // 2. workspace: keys, record ids, their sorted copies, values, library
-    //    temporaries. The sort moves (key, id) pairs only; the reduction
-    //    gathers each value once through the sorted ids.
+    // 2. workspace: keys, record ids, their sorted copies, values, slot
+    //    starts, library temporaries. The sort moves (key, id) pairs only; the
+    //    reduction gathers each value once through the sorted ids.
     int key_bits = 1;
     while (key_bits < 32 && ((uint64_t)1 << key_bits) < (uint64_t)nvox) ++key_bits;
+    const int begin_bit = by_slot ? vox_bits : 0;
+    const int end_bit = std::max(key_bits, begin_bit + 1);
     size_t sort_tmp = 0;
     YX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                              (uint32_t*)nullptr, (uint32_t*)nullptr, (int)total, 0, key_bits, st));
+                                              (uint32_t*)nullptr, (uint32_t*)nullptr, (int)total, begin_bit, end_bit,
+                                              st));
     const size_t kb = align256(total * 4), vb = align256(total * 16);
-    const size_t need = off_bytes + 3 * kb + vb + 256 + align256(std::max(sort_tmp, scan_tmp));
+    const size_t sb = by_slot ? align256(((size_t)vol->num_slots + 1) * 4) : 0;
+    const size_t need = off_bytes + 3 * kb + vb + sb + 256 + align256(std::max(sort_tmp, scan_tmp));
     if (c->iota_n < total) {
         if (c->d_iota) YX_HIP(hipFree(c->d_iota));
         c->d_iota = nullptr;
@@ -615,6 +733,8 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     p += kb;
     float4* vals = (float4*)p;
     p += vb;
+    uint32_t* starts = (uint32_t*)p;
+    p += sb;
     uint32_t* counter = (uint32_t*)p;
     p += 256;
     void* tmp = p;
@@ -629,14 +749,45 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     hipLaunchKernelGGL(comp_backward_kernel, dim3((unsigned)((n_waves * 64 + 255) / 256)), dim3(256), 0, st, a);
     YX_HIP(hipGetLastError());
 
-    // 4. sort (key, id) by voxel key (LSD radix, stable: a key's records stay
-    //    in emission order), 5. sum each key's run in that order
+    // 4. sort (key, id) by brick slot, or by the whole voxel key (LSD radix,
+    //    stable: records stay in emission order within a slot or key),
+    // 5. sum each voxel's records in that order
     YX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys_sorted, (const uint32_t*)c->d_iota, ids_sorted,
-                                              (int)total, 0, key_bits, st));
-    hipLaunchKernelGGL(comp_reduce_runs_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, keys_sorted,
-                       ids_sorted, vals, (uint32_t)total, (float4*)grad_params);
-    YX_HIP(hipGetLastError());
-    c->sorted_keys = keys_sorted;
+                                              (int)total, begin_bit, end_bit, st));
     c->counter = counter;
+    if (by_slot) {
+        const uint32_t ns = vol->num_slots;
+        const uint32_t slot_mask = (uint32_t)(((uint64_t)1 << (end_bit - begin_bit)) - 1u);
+        hipLaunchKernelGGL(comp_slot_starts_kernel, dim3((unsigned)((total + 1 + 255) / 256)), dim3(256), 0, st,
+                           keys_sorted, (uint32_t)total, vox_bits, slot_mask, ns, starts, counter);
+        YX_HIP(hipGetLastError());
+        const dim3 grid((unsigned)((ns + 3) / 4)), block(256);
+        float4* g = (float4*)grad_params;
+        switch (vox_bits) {
+            case 0:
+                hipLaunchKernelGGL(comp_reduce_slots_kernel<0>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
+                                   starts, ns, g, counter);
+                break;
+            case 3:
+                hipLaunchKernelGGL(comp_reduce_slots_kernel<3>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
+                                   starts, ns, g, counter);
+                break;
+            case 6:
+                hipLaunchKernelGGL(comp_reduce_slots_kernel<6>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
+                                   starts, ns, g, counter);
+                break;
+            default:
+                hipLaunchKernelGGL(comp_reduce_slots_kernel<9>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
+                                   starts, ns, g, counter);
+                break;
+        }
+        YX_HIP(hipGetLastError());
+        c->counted = true;
+    } else {
+        hipLaunchKernelGGL(comp_reduce_runs_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
+                           keys_sorted, ids_sorted, vals, (uint32_t)total, (float4*)grad_params, (uint32_t)nvox);
+        YX_HIP(hipGetLastError());
+        c->sorted_keys = keys_sorted;
+    }
     return YX_OK;
 }
