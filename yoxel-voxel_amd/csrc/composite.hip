@@ -407,16 +407,36 @@ __global__ __launch_bounds__(256) void comp_reduce_runs_kernel(const uint32_t* k
 // The slot is read from the sorted bit range and clamped below num_slots, so
 // even records that no sample wrote (sample counts that do not match the
 // forward's) cannot index past start[num_slots].
+// Thread t takes records 4t .. 4t + 3 (one 16-byte load; `keys` is 16-byte
+// aligned) and the boundary before each of them, up to record n.
 __global__ __launch_bounds__(256) void comp_slot_starts_kernel(const uint32_t* keys, uint32_t n, int shift,
                                                                uint32_t slot_mask, uint32_t num_slots,
                                                                uint32_t* start, uint32_t* distinct) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    if (i == 0) *distinct = 0u;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i0 = t * 4u;
+    if (i0 > n) return;
+    if (t == 0) *distinct = 0u;
     auto slot_of = [&](uint32_t k) { return min((k >> shift) & slot_mask, num_slots - 1u); };
-    const uint32_t s = i < n ? slot_of(keys[i]) : num_slots;
-    const uint32_t lo = i == 0 ? 0u : slot_of(keys[i - 1]) + 1u;
-    for (uint32_t q = lo; q <= s; ++q) start[q] = i;
+    uint32_t k4[4];
+    if (i0 + 4u <= n) {
+        const uint4 v = *reinterpret_cast<const uint4*>(keys + i0);
+        k4[0] = v.x;
+        k4[1] = v.y;
+        k4[2] = v.z;
+        k4[3] = v.w;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) k4[j] = i0 + (uint32_t)j < n ? keys[i0 + j] : 0u;
+    }
+    uint32_t lo = i0 == 0 ? 0u : slot_of(keys[i0 - 1]) + 1u;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint32_t i = i0 + (uint32_t)j;
+        if (i > n) break;
+        const uint32_t s = i < n ? slot_of(k4[j]) : num_slots;
+        for (uint32_t q = lo; q <= s; ++q) start[q] = i;
+        lo = s + 1u;
+    }
 }
 
 __device__ __forceinline__ float4 shfl4(float4 v, int src) {
@@ -758,7 +778,8 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     if (by_slot) {
         const uint32_t ns = vol->num_slots;
         const uint32_t slot_mask = (uint32_t)(((uint64_t)1 << (end_bit - begin_bit)) - 1u);
-        hipLaunchKernelGGL(comp_slot_starts_kernel, dim3((unsigned)((total + 1 + 255) / 256)), dim3(256), 0, st,
+        const size_t threads = total / 4 + 1;  // thread t covers records 4t .. 4t + 3, up to record `total`
+        hipLaunchKernelGGL(comp_slot_starts_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
                            keys_sorted, (uint32_t)total, vox_bits, slot_mask, ns, starts, counter);
         YX_HIP(hipGetLastError());
         const dim3 grid((unsigned)((ns + 3) / 4)), block(256);
