@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <type_traits>
 
 #include "internal.h"
 #include "trace_device.h"
@@ -44,6 +45,7 @@ struct CompArgs {
     const float* views;
     uint32_t num_slots;
     int n, lb, g;  // grid, log2 brick, bricks per axis
+    int lg;        // log2 of g
     int n_views, w, h;
     float* rgb;
     float* trans;
@@ -111,12 +113,17 @@ __device__ __forceinline__ int vox(float o, float d, float t, float nf, int n) {
 // parameters); `live` turns false at the end of the ray. One step per call
 // keeps the lanes of a wave in one flat loop: a lane crossing empty space does
 // not hold up the others' samples.
+// SMALL: at most 1024 bricks per axis (a power of two, 2^lg), so the brick
+// index (bz << 2 lg | by << lg | bx, < 2^30) fits 32 bits and costs two
+// shift-or instructions; otherwise it is formed with 64-bit multiplies.
+template <bool SMALL>
 struct SampleIter {
+    using BrickId = typename std::conditional<SMALL, uint32_t, int64_t>::type;
     CompRay r;
     int k;
     bool live;
 
-    int64_t cb;  // brick of the last table lookup (-1: none)
+    BrickId cb;  // brick of the last table lookup (all ones / -1: none)
     int cslot;   // its slot
 
     // Advance by one step without loading the sample's parameters: returns
@@ -135,7 +142,11 @@ struct SampleIter {
         const float t = (float)k * inv_n;
         const int vx = vox(r.ox, r.dx, t, nf, a.n), vy = vox(r.oy, r.dy, t, nf, a.n), vz = vox(r.oz, r.dz, t, nf, a.n);
         const int bx = vx >> a.lb, by = vy >> a.lb, bz = vz >> a.lb;
-        const int64_t bl = ((int64_t)bz * a.g + by) * a.g + bx;
+        BrickId bl;
+        if constexpr (SMALL)
+            bl = ((uint32_t)bz << (2 * a.lg)) | ((uint32_t)by << a.lg) | (uint32_t)bx;
+        else
+            bl = ((int64_t)bz * a.g + by) * a.g + bx;
         if (bl != cb) {  // consecutive samples mostly share a brick
             cb = bl;
             cslot = a.table[bl];
@@ -170,7 +181,7 @@ struct SampleIter {
     __device__ __forceinline__ void start(const CompArgs& a, int view, int x, int y, bool valid) {
         live = valid && comp_setup(a, view, x, y, r);
         k = r.k;
-        cb = -1;
+        cb = (BrickId)-1;
         cslot = -1;
     }
 };
@@ -211,13 +222,14 @@ __device__ __forceinline__ bool ray_of(const CompArgs& a, size_t t, int& view, i
     return true;
 }
 
+template <bool SMALL>
 __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
     int view, x, y;
     size_t ray;
     if (!ray_of(a, (size_t)blockIdx.x * 256 + threadIdx.x, view, x, y, ray)) return;
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
     uint32_t n = 0;
-    SampleIter it;
+    SampleIter<SMALL> it;
     it.start(a, view, x, y, true);
     // kCompBatch steps per lane per iteration, their loads issued together;
     // the wave-wide vote keeps the loop flat (without it the skip steps get
@@ -270,6 +282,7 @@ __global__ __launch_bounds__(256) void comp_wave_sums_kernel(CompArgs a, uint32_
 // exclusive scan of the per-wave sample counts gives its tile; within it the
 // lanes that emit in one iteration take consecutive slots (ballot rank), so
 // stores coalesce and the record order is fixed by the inputs alone.
+template <bool SMALL>
 __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
     const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
     const size_t wave = t >> 6;
@@ -277,8 +290,13 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
     int view = 0, x = 0, y = 0;
     size_t ray = 0;
     const bool valid = ray_of(a, t, view, x, y, ray);
-    SampleIter it;
-    it.start(a, view, x, y, valid);
+    // the forward's sample count: a ray without samples does not walk, and a
+    // ray stops right after its last sample instead of walking the empty
+    // bricks behind it (those steps emit nothing, so the records and their
+    // positions are the same)
+    uint32_t left = valid ? a.nsamp_in[ray] : 0u;
+    SampleIter<SMALL> it;
+    it.start(a, view, x, y, valid && left > 0u);
     float gr = 0.0f, gg = 0.0f, gb = 0.0f, fr = 0.0f, fg = 0.0f, fb = 0.0f;
     if (valid) {
         gr = a.grad_rgb[3 * ray];
@@ -297,7 +315,10 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
         uint32_t ix[kCompBatch];
         bool got[kCompBatch];
 #pragma unroll
-        for (int j = 0; j < kCompBatch; ++j) got[j] = it.next(a, ix[j]);
+        for (int j = 0; j < kCompBatch; ++j) {
+            got[j] = it.next(a, ix[j]);
+            if (got[j] && --left == 0u) it.live = false;
+        }
         float4 q[kCompBatch];
 #pragma unroll
         for (int j = 0; j < kCompBatch; ++j) q[j] = got[j] ? a.params[ix[j]] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -561,6 +582,7 @@ static CompArgs make_args(const yx_volume_dev* v, const float* views, int32_t n_
     a.n = v->grid;
     a.lb = __builtin_ctz((unsigned)v->brick);
     a.g = v->grid / v->brick;
+    a.lg = __builtin_ctz((unsigned)a.g);
     a.n_views = n_views;
     a.w = w;
     a.h = h;
@@ -625,8 +647,11 @@ extern "C" int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, 
     a.trans = trans;
     a.nsamp = nsamp;
     const size_t waves = (size_t)n_views * ((w + 7) / 8) * ((h + 7) / 8);
-    hipLaunchKernelGGL(comp_forward_kernel, dim3((unsigned)((waves * 64 + 255) / 256)), dim3(256), 0,
-                       (hipStream_t)stream, a);
+    const dim3 grid((unsigned)((waves * 64 + 255) / 256)), block(256);
+    if (a.g <= 1024)
+        hipLaunchKernelGGL(comp_forward_kernel<true>, grid, block, 0, (hipStream_t)stream, a);
+    else
+        hipLaunchKernelGGL(comp_forward_kernel<false>, grid, block, 0, (hipStream_t)stream, a);
     YX_HIP(hipGetLastError());
     return YX_OK;
 }
@@ -766,7 +791,11 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     a.total_records = (uint32_t)total;
     a.keys = keys;
     a.vals = vals;
-    hipLaunchKernelGGL(comp_backward_kernel, dim3((unsigned)((n_waves * 64 + 255) / 256)), dim3(256), 0, st, a);
+    const dim3 bgrid((unsigned)((n_waves * 64 + 255) / 256)), bblock(256);
+    if (a.g <= 1024)
+        hipLaunchKernelGGL(comp_backward_kernel<true>, bgrid, bblock, 0, st, a);
+    else
+        hipLaunchKernelGGL(comp_backward_kernel<false>, bgrid, bblock, 0, st, a);
     YX_HIP(hipGetLastError());
 
     // 4. sort (key, id) by brick slot, or by the whole voxel key (LSD radix,
