@@ -442,7 +442,9 @@ typedef struct yx_volume_dev {
     uint32_t num_slots;
 } yx_volume_dev;
 
-typedef struct yx_compositor yx_compositor; /* holds the backward's workspace */
+/* Holds the backward's workspace: calls on one compositor must not overlap
+ * (use one stream, or one compositor per stream). */
+typedef struct yx_compositor yx_compositor;
 int yx_compositor_create(yx_compositor** out);
 void yx_compositor_destroy(yx_compositor* c);
 
@@ -456,9 +458,10 @@ int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, const float
 /* Adjoint of the forward: given the forward's rgb and nsamp and dL/drgb,
  * writes dL/dparams (float4 per voxel slot: d raw, d r, d g, d b). Recomputes
  * the exact sample sequence, emits one (voxel key, gradient) record per
- * sample, sorts the records by key and sums each key's records in sorted
- * order: no atomics, deterministic. Synchronizes `stream` once to size its
- * workspace. */
+ * sample, sorts the records by brick slot (bricks up to 8^3; larger bricks by
+ * the whole key) and sums each voxel's records in emission order: no atomics
+ * on gradients, deterministic. nsamp must be the forward's. Synchronizes
+ * `stream` once to size its workspace. */
 int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol, const float* views, int32_t n_views,
                           int32_t width, int32_t height, const float* rgb, const uint32_t* nsamp,
                           const float* grad_rgb, float* grad_params, void* stream);
