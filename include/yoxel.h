@@ -465,7 +465,8 @@ int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, const float
 int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol, const float* views, int32_t n_views,
                           int32_t width, int32_t height, const float* rgb, const uint32_t* nsamp,
                           const float* grad_rgb, float* grad_params, void* stream);
-/* Records emitted by the last backward (sum of nsamp) and distinct keys. */
+/* Records emitted by the last backward (sum of nsamp) and distinct keys.
+ * Statistics only: waits for the device, then reads them. */
 int yx_compositor_last_stats(const yx_compositor* c, uint64_t* records, uint64_t* unique_keys);
 
 /* ---- brick pool manager (CuBrickPoolManager::allocMap,

@@ -620,6 +620,9 @@ extern "C" int yx_compositor_last_stats(const yx_compositor* c, uint64_t* record
     if (!c || !records || !unique_keys) return fail(YX_ERR_INVALID_ARG, "null argument");
     *records = c->last_records;
     *unique_keys = 0;
+    // the backward ran on the caller's stream, which may not synchronise with
+    // the null stream: wait for the device before reading its results
+    if (c->last_records) YX_HIP(hipDeviceSynchronize());
     if (c->counted && c->last_records) {
         uint32_t runs = 0;
         YX_HIP(hipMemcpy(&runs, c->counter, 4, hipMemcpyDeviceToHost));
