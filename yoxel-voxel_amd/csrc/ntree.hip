@@ -245,8 +245,10 @@ struct NtRay {
     bool first, inb;
     const uint8_t* brick;
     int bx, by, bz;
-    bool nc_valid[L > 0 ? L : 1];
-    int nck[L > 0 ? L : 1][3];
+    // per level the node last read: its cell (kx | ky << 10 | kz << 20; the
+    // coordinates stay below 2^9 for sizes up to 16384, so all ones means
+    // none), child masks and child base
+    uint32_t nck[L > 0 ? L : 1];
     uint32_t ncm_lo[L > 0 ? L : 1], ncm_hi[L > 0 ? L : 1], ncb[L > 0 ? L : 1];
 };
 
@@ -282,7 +284,7 @@ __device__ __forceinline__ bool nt_setup(const NtArgs& a, int x, int y, NtRay<L>
     r.bx = r.by = r.bz = 0;
     r.guard = 0;
 #pragma unroll
-    for (int l = 0; l < (L > 0 ? L : 1); ++l) r.nc_valid[l] = false;
+    for (int l = 0; l < (L > 0 ? L : 1); ++l) r.nck[l] = 0xFFFFFFFFu;
     const float lox = (0.0f - r.ox) * r.ix, hix = (1.0f - r.ox) * r.ix;
     const float loy = (0.0f - r.oy) * r.iy, hiy = (1.0f - r.oy) * r.iy;
     const float loz = (0.0f - r.oz) * r.iz, hiz = (1.0f - r.oz) * r.iz;
@@ -322,17 +324,15 @@ __device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_ro
         for (int l = 1; l <= L; ++l) {
             if (idx != kNtEmpty) {
                 const int sn = 3 + 2 * (L - l + 1);
-                const int kx = r.vx >> sn, ky = r.vy >> sn, kz = r.vz >> sn;
-                if (!(r.nc_valid[l - 1] && r.nck[l - 1][0] == kx && r.nck[l - 1][1] == ky && r.nck[l - 1][2] == kz)) {
+                const uint32_t key = (uint32_t)(r.vx >> sn) | ((uint32_t)(r.vy >> sn) << 10) |
+                                     ((uint32_t)(r.vz >> sn) << 20);
+                if (r.nck[l - 1] != key) {
                     const uint4 nd = a.nodes[idx];
                     ++r.nn;
                     r.ncm_lo[l - 1] = nd.x;
                     r.ncm_hi[l - 1] = nd.y;
                     r.ncb[l - 1] = nd.z;
-                    r.nck[l - 1][0] = kx;
-                    r.nck[l - 1][1] = ky;
-                    r.nck[l - 1][2] = kz;
-                    r.nc_valid[l - 1] = true;
+                    r.nck[l - 1] = key;
                 }
                 lS -= 2;
                 const int s = sn - 2;
