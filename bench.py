@@ -820,7 +820,7 @@ def brick_bench(yx, torch, frames, with_cpu):
            "mean_node_reads_per_ray": round(float(np.mean([r[0] for r in reads])) / (w * h), 3),
            "mean_voxel_reads_per_ray": round(float(np.mean([r[1] for r in reads])) / (w * h), 3),
            "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 5), "kernel": "nt_trace_kernel<3>",
+                        "frac": round(achieved / HBM_PEAK_GBS, 5), "kernel": "nt_trace_kernel<3, false>",
                         "algorithmic_bytes_per_launch": int(np.mean(algo))},
            "cpu_baseline": None}
     if with_cpu:

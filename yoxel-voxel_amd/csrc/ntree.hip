@@ -237,14 +237,15 @@ __device__ __forceinline__ int nt_follow(float o, float d, float t, float nf, in
 // State of one ray of the n-tree cast.
 template <int L>
 struct NtRay {
-    float ox, oy, oz, dx, dy, dz, ix, iy, iz, tx, t;
+    // the origin is the camera position of the launch (a.view[0..2], the
+    // same for every ray), read from the arguments where needed
+    float dx, dy, dz, ix, iy, iz, tx, t;
     int vx, vy, vz, guard;
     float ar, ag, ab, aa, thit;
     int hx, hy, hz;
     uint32_t nn, nv;
     bool first, inb;
-    const uint8_t* brick;
-    int bx, by, bz;
+    uint32_t brick;  // index of the brick the ray is in (its 512 voxels at bricks + 512 * brick)
     // per level the node last read: its cell (kx | ky << 10 | kz << 20; the
     // coordinates stay below 2^9 for sizes up to 16384, so all ones means
     // none), child masks and child base
@@ -268,9 +269,7 @@ __device__ __forceinline__ bool nt_setup(const NtArgs& a, int x, int y, NtRay<L>
     r.dx = nt_clamp_eps(dx / nrm);
     r.dy = nt_clamp_eps(dy / nrm);
     r.dz = nt_clamp_eps(dz / nrm);
-    r.ox = a.view[0];
-    r.oy = a.view[1];
-    r.oz = a.view[2];
+    const float ox = a.view[0], oy = a.view[1], oz = a.view[2];
     r.ix = 1.0f / r.dx;
     r.iy = 1.0f / r.dy;
     r.iz = 1.0f / r.dz;
@@ -280,23 +279,22 @@ __device__ __forceinline__ bool nt_setup(const NtArgs& a, int x, int y, NtRay<L>
     r.ar = r.ag = r.ab = r.aa = 0.0f;
     r.first = true;
     r.inb = false;
-    r.brick = a.bricks;
-    r.bx = r.by = r.bz = 0;
+    r.brick = 0;
     r.guard = 0;
 #pragma unroll
     for (int l = 0; l < (L > 0 ? L : 1); ++l) r.nck[l] = 0xFFFFFFFFu;
-    const float lox = (0.0f - r.ox) * r.ix, hix = (1.0f - r.ox) * r.ix;
-    const float loy = (0.0f - r.oy) * r.iy, hiy = (1.0f - r.oy) * r.iy;
-    const float loz = (0.0f - r.oz) * r.iz, hiz = (1.0f - r.oz) * r.iz;
+    const float lox = (0.0f - ox) * r.ix, hix = (1.0f - ox) * r.ix;
+    const float loy = (0.0f - oy) * r.iy, hiy = (1.0f - oy) * r.iy;
+    const float loz = (0.0f - oz) * r.iz, hiz = (1.0f - oz) * r.iz;
     float te = fmaxf(fmaxf(fmaxf(-INFINITY, fminf(lox, hix)), fminf(loy, hiy)), fminf(loz, hiz));
     r.tx = fminf(fminf(fminf(INFINITY, fmaxf(lox, hix)), fmaxf(loy, hiy)), fmaxf(loz, hiz));
     te = fmaxf(te, 0.0f);
     if (r.tx < 0.0f || te > r.tx) return false;
     const int N = a.size;
     const float nf = (float)N;
-    r.vx = min(max((int)floorf((r.ox + te * r.dx) * nf), 0), N - 1);
-    r.vy = min(max((int)floorf((r.oy + te * r.dy) * nf), 0), N - 1);
-    r.vz = min(max((int)floorf((r.oz + te * r.dz) * nf), 0), N - 1);
+    r.vx = min(max((int)floorf((ox + te * r.dx) * nf), 0), N - 1);
+    r.vy = min(max((int)floorf((oy + te * r.dy) * nf), 0), N - 1);
+    r.vz = min(max((int)floorf((oz + te * r.dz) * nf), 0), N - 1);
     r.t = te;
     return true;
 }
@@ -313,6 +311,7 @@ __device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_ro
                                            NtRay<L>& r) {
     const int N = a.size, R = a.R;
     const float nf = (float)N, invN = 1.0f / nf;
+    const float ox = a.view[0], oy = a.view[1], oz = a.view[2];
     const int sx = r.dx > 0.0f ? 1 : -1, sy = r.dy > 0.0f ? 1 : -1, sz = r.dz > 0.0f ? 1 : -1;
     const int ux = sx > 0 ? 1 : 0, uy = sy > 0 ? 1 : 0, uz = sz > 0 ? 1 : 0;
     constexpr int sr = 3 + 2 * L;  // log2 of the root cell side
@@ -352,19 +351,19 @@ __device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_ro
         if (idx == kNtEmpty) {
             // leave the empty cell of side 2^lS in one step
             const int cx = r.vx >> lS, cy = r.vy >> lS, cz = r.vz >> lS;
-            const float tX = ((float)((cx + ux) << lS) * invN - r.ox) * r.ix;
-            const float tY = ((float)((cy + uy) << lS) * invN - r.oy) * r.iy;
-            const float tZ = ((float)((cz + uz) << lS) * invN - r.oz) * r.iz;
+            const float tX = ((float)((cx + ux) << lS) * invN - ox) * r.ix;
+            const float tY = ((float)((cy + uy) << lS) * invN - oy) * r.iy;
+            const float tZ = ((float)((cz + uz) << lS) * invN - oz) * r.iz;
             const int ax = (tX <= tY && tX <= tZ) ? 0 : (tY <= tZ ? 1 : 2);
             const float tn = pick3(ax, tX, tY, tZ);
             if (tn > r.tx) return true;
             const int cs = 1 << lS;
             const int nx = ax == 0 ? (sx > 0 ? (cx + 1) << lS : (cx << lS) - 1)
-                                   : nt_follow(r.ox, r.dx, tn, nf, r.vx, sx, cx << lS, cs);
+                                   : nt_follow(ox, r.dx, tn, nf, r.vx, sx, cx << lS, cs);
             const int ny = ax == 1 ? (sy > 0 ? (cy + 1) << lS : (cy << lS) - 1)
-                                   : nt_follow(r.oy, r.dy, tn, nf, r.vy, sy, cy << lS, cs);
+                                   : nt_follow(oy, r.dy, tn, nf, r.vy, sy, cy << lS, cs);
             const int nz = ax == 2 ? (sz > 0 ? (cz + 1) << lS : (cz << lS) - 1)
-                                   : nt_follow(r.oz, r.dz, tn, nf, r.vz, sz, cz << lS, cs);
+                                   : nt_follow(oz, r.dz, tn, nf, r.vz, sz, cz << lS, cs);
             const int na = pick3(ax, nx, ny, nz);
             if (na < 0 || na >= N) return true;
             r.t = tn;
@@ -374,29 +373,29 @@ __device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_ro
             return false;
         }
         r.inb = true;
-        r.brick = a.bricks + (size_t)idx * 512u;
-        r.bx = r.vx >> 3;
-        r.by = r.vy >> 3;
-        r.bz = r.vz >> 3;
+        r.brick = idx;
     }
-    int px[4], py[4], pz[4];
+    // every valid position of the batch is in the current brick (the batch
+    // stops at the first step that leaves it): keep its offset in the brick,
+    // x + 8 y + 64 z, and the brick's corner once
+    const int basex = r.vx & ~7, basey = r.vy & ~7, basez = r.vz & ~7;
+    int off[4];
     float pt[4];
     bool valid[4];
     bool more = true, end = false;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         valid[j] = more;
-        px[j] = r.vx;
-        py[j] = r.vy;
-        pz[j] = r.vz;
+        off[j] = (r.vx & 7) + 8 * (r.vy & 7) + 64 * (r.vz & 7);
         pt[j] = r.t;
         if (more) {
-            const float tX = ((float)(r.vx + ux) * invN - r.ox) * r.ix;
-            const float tY = ((float)(r.vy + uy) * invN - r.oy) * r.iy;
-            const float tZ = ((float)(r.vz + uz) * invN - r.oz) * r.iz;
+            const float tX = ((float)(r.vx + ux) * invN - ox) * r.ix;
+            const float tY = ((float)(r.vy + uy) * invN - oy) * r.iy;
+            const float tZ = ((float)(r.vz + uz) * invN - oz) * r.iz;
             const int ax = (tX <= tY && tX <= tZ) ? 0 : (tY <= tZ ? 1 : 2);
             const float tn = pick3(ax, tX, tY, tZ);
-            const int na = pick3(ax, r.vx + sx, r.vy + sy, r.vz + sz);
+            const int oa = pick3(ax, r.vx, r.vy, r.vz);
+            const int na = oa + pick3(ax, sx, sy, sz);
             if (tn > r.tx || na < 0 || na >= N) {
                 end = true;
                 more = false;
@@ -405,13 +404,16 @@ __device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_ro
                 r.vx = ax == 0 ? na : r.vx;
                 r.vy = ax == 1 ? na : r.vy;
                 r.vz = ax == 2 ? na : r.vz;
-                if ((r.vx >> 3) != r.bx || (r.vy >> 3) != r.by || (r.vz >> 3) != r.bz) more = false;
+                // only the exit axis moved (by one): the ray left the brick iff
+                // that coordinate crossed a multiple of 8
+                if ((na >> 3) != (oa >> 3)) more = false;
             }
         }
     }
     uint32_t dd[4];
+    const uint8_t* brick = a.bricks + (size_t)r.brick * 512u;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) dd[j] = valid[j] ? r.brick[(px[j] & 7) + 8 * (py[j] & 7) + 64 * (pz[j] & 7)] : 0u;
+    for (int j = 0; j < 4; ++j) dd[j] = valid[j] ? brick[off[j]] : 0u;
     bool stop = false;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -434,9 +436,9 @@ __device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_ro
                 if (r.first) {
                     r.first = false;
                     r.thit = pt[j];
-                    r.hx = px[j];
-                    r.hy = py[j];
-                    r.hz = pz[j];
+                    r.hx = basex + (off[j] & 7);
+                    r.hy = basey + ((off[j] >> 3) & 7);
+                    r.hz = basez + (off[j] >> 6);
                 }
                 if (r.aa > 0.99f) {
                     stop = true;
@@ -449,7 +451,9 @@ __device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_ro
     return end;
 }
 
-template <int L>
+// COUNT: the caller asked for the node / voxel read counts (without them the
+// counters are dead and take no registers)
+template <int L, bool COUNT>
 __device__ __forceinline__ void nt_write(const NtArgs& a, size_t pix, const NtRay<L>& r) {
     if (a.rgba)
         a.rgba[pix] = (uint32_t)(nt_saturate(r.ar) * 255.0f) | ((uint32_t)(nt_saturate(r.ag) * 255.0f) << 8) |
@@ -461,8 +465,10 @@ __device__ __forceinline__ void nt_write(const NtArgs& a, size_t pix, const NtRa
         a.voxel[3 * pix + 1] = r.hy;
         a.voxel[3 * pix + 2] = r.hz;
     }
-    if (a.n_nodes) a.n_nodes[pix] = r.nn;
-    if (a.n_voxels) a.n_voxels[pix] = r.nv;
+    if constexpr (COUNT) {
+        if (a.n_nodes) a.n_nodes[pix] = r.nn;
+        if (a.n_voxels) a.n_voxels[pix] = r.nv;
+    }
 }
 
 __device__ __forceinline__ void nt_stage(const NtArgs& a, uint32_t* s_root, uint32_t* s_pal) {
@@ -473,7 +479,7 @@ __device__ __forceinline__ void nt_stage(const NtArgs& a, uint32_t* s_root, uint
 }
 
 // One lane per pixel; wave w covers 8x8 pixel tile w (row-major tiles).
-template <int L>
+template <int L, bool COUNT>
 __global__ __launch_bounds__(256) void nt_trace_kernel(NtArgs a) {
     __shared__ uint32_t s_root[kNtMaxRoot * kNtMaxRoot * kNtMaxRoot];
     __shared__ uint32_t s_pal[256];
@@ -488,7 +494,7 @@ __global__ __launch_bounds__(256) void nt_trace_kernel(NtArgs a) {
     if (nt_setup(a, x, y, r))
         while (!nt_iterate(a, s_root, s_pal, r)) {
         }
-    nt_write(a, (size_t)y * a.w + x, r);
+    nt_write<L, COUNT>(a, (size_t)y * a.w + x, r);
 }
 
 // Persistent waves over an atomic ray queue (ray ids in 8x8 pixel tiles).
@@ -514,7 +520,7 @@ __global__ __launch_bounds__(256) void nt_trace_persistent(NtArgs a) {
         const uint64_t waiting = __ballot(status == kIdle || status == kDone);
         if (waiting != 0 && (walking == 0 || __popcll(waiting) >= a.refill_min)) {
             if (status == kDone) {
-                nt_write(a, pix, r);
+                nt_write<L, true>(a, pix, r);
                 status = kIdle;
             }
             const uint64_t need = __ballot(status == kIdle);
@@ -849,13 +855,22 @@ extern "C" int yx_ntree_render(const yx_ntree* t, const float view[12], int32_t 
     const hipStream_t st = (hipStream_t)stream;
     if (!t->persistent) {
         const dim3 grid((waves + 3) / 4);
+        const bool count = a.n_nodes || a.n_voxels;
+#define YX_NT_CAST(L)                                                                \
+    do {                                                                             \
+        if (count)                                                                   \
+            hipLaunchKernelGGL((nt_trace_kernel<L, true>), grid, blk, 0, st, a);     \
+        else                                                                         \
+            hipLaunchKernelGGL((nt_trace_kernel<L, false>), grid, blk, 0, st, a);    \
+    } while (0)
         switch (t->levels) {
-            case 0: hipLaunchKernelGGL(nt_trace_kernel<0>, grid, blk, 0, st, a); break;
-            case 1: hipLaunchKernelGGL(nt_trace_kernel<1>, grid, blk, 0, st, a); break;
-            case 2: hipLaunchKernelGGL(nt_trace_kernel<2>, grid, blk, 0, st, a); break;
-            case 3: hipLaunchKernelGGL(nt_trace_kernel<3>, grid, blk, 0, st, a); break;
-            default: hipLaunchKernelGGL(nt_trace_kernel<4>, grid, blk, 0, st, a); break;
+            case 0: YX_NT_CAST(0); break;
+            case 1: YX_NT_CAST(1); break;
+            case 2: YX_NT_CAST(2); break;
+            case 3: YX_NT_CAST(3); break;
+            default: YX_NT_CAST(4); break;
         }
+#undef YX_NT_CAST
         YX_HIP(hipGetLastError());
         return YX_OK;
     }
