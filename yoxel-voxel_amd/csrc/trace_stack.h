@@ -107,10 +107,11 @@ __device__ __forceinline__ void write_final(const yx_outputs& o, uint32_t pix, u
 }
 
 // LDS bytes per lane of stack_kernel<BLOCK, W, ...> when `levels` levels are
-// pushed: W == 0 keeps every node record (8 B) and the cells (24 B) of even
-// levels; W > 0 keeps 32-byte rows for the levels older than the window.
+// pushed: W == 0 keeps every level's node record index (4 B) and the cells
+// (24 B) of even levels; W > 0 keeps 32-byte rows for the levels older than
+// the window.
 inline size_t stack_lane_bytes(int levels, int W) {
-    if (W == 0) return (size_t)8 * (size_t)levels + (size_t)24 * (size_t)((levels + 1) / 2);
+    if (W == 0) return (size_t)4 * (size_t)levels + (size_t)24 * (size_t)((levels + 1) / 2);
     return (size_t)32 * (size_t)(levels - W > 1 ? levels - W : 1);
 }
 
@@ -187,16 +188,19 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     bool lod_stop = false;  // LOD: the ray ended on a node
     uint32_t lod_rec = 0;   // LOD: that node's record
 
-    // W == 0: every level's node record ([levels][BLOCK] uint2), and the cell
-    // of even levels only ([(levels + 1) / 2][BLOCK] as uint4 + uint2). A pop to
-    // an odd level rebuilds its cell from the level above by one exact replay
-    // step, so a lane needs 8 B + 12 B per level instead of 32 B.
+    // W == 0: every level's node record index ([levels][BLOCK] uint32; a pop
+    // reads the record again, mostly from the cache that served it on the way
+    // down), and the cell of even levels only ([(levels + 1) / 2][BLOCK] as
+    // uint4 + uint2). A pop to an odd level rebuilds its cell from the level
+    // above by one exact replay step, so a lane needs 4 B + 12 B per level
+    // instead of 32 B.
     uint64_t ffcpath = 0;  // FindFirstChild result per level (W == 0)
-    uint2* sh_node = reinterpret_cast<uint2*>(sh_stk);
-    uint4* sh_ca = reinterpret_cast<uint4*>(sh_node + (size_t)a.levels * BLOCK);
+    uint32_t cur = 0;      // record index of `node` (W == 0)
+    uint32_t* sh_idx = reinterpret_cast<uint32_t*>(sh_stk);
+    uint4* sh_ca = reinterpret_cast<uint4*>(sh_idx + (size_t)a.levels * BLOCK);
     uint2* sh_cb = reinterpret_cast<uint2*>(sh_ca + (size_t)((a.levels + 1) >> 1) * BLOCK);
-    auto push0 = [&](int lv, const Cell& cc, uint2 nn) {
-        sh_node[(size_t)lv * BLOCK + tid] = nn;
+    auto push0 = [&](int lv, const Cell& cc, uint32_t ri) {
+        sh_idx[(size_t)lv * BLOCK + tid] = ri;
         if ((lv & 1) == 0) {
             const size_t r = (size_t)(lv >> 1) * BLOCK + tid;
             sh_ca[r] = make_uint4(__float_as_uint(cc.t1x), __float_as_uint(cc.t1y), __float_as_uint(cc.t1z),
@@ -205,7 +209,8 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
         }
     };
     auto pop0 = [&](int lv, Cell& cc, uint2& nn) {
-        nn = sh_node[(size_t)lv * BLOCK + tid];
+        cur = sh_idx[(size_t)lv * BLOCK + tid];
+        nn = rec_at(cur);
         const size_t r = (size_t)(lv >> 1) * BLOCK + tid;
         const uint4 r0 = sh_ca[r];
         const uint2 r1 = sh_cb[r];
@@ -377,6 +382,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                 if (ok) {
                     ++visits;
                     node = rec_at(root);
+                    if constexpr (W == 0) cur = root;
                     level = 0;
                     adv = false;
                     ch = find_first_child(c);
@@ -422,12 +428,13 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     win[0].c = c;
                     win[0].rec = node;
                 } else {
-                    push0(level, c, node);
+                    push0(level, c, cur);
                 }
                 chpath = set3(chpath, level, (uint32_t)ch);
                 const uint32_t below = (cm | lm) & ((1u << real) - 1u);
                 const uint32_t crec = node.x + (uint32_t)__builtin_popcount(below);
                 node = rec_at(crec);
+                if constexpr (W == 0) cur = crec;
                 ++level;
                 if constexpr (LOD) {
                     // c is this node's cell; 2^-level as a float is exact
