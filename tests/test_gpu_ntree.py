@@ -1,8 +1,9 @@
 """GPU parity of the brick n-tree: the GPU builder's arrays equal the CPU
 restatement's byte for byte (from a dense volume and from the fBm), and the
 DDA cast equals the restatement's per-pixel outputs exactly (RGBA8, t, first
-voxel, node and voxel read counts); at the full brick_4096 size the build is
-checked on sampled bricks and the cast on every 8th row of a 1080p frame."""
+voxel, node and voxel read counts), and a cast without the read counters
+equals the counted one; at the full brick_4096 size the build is checked on
+sampled bricks and the cast on every 8th row of a 1080p frame."""
 import numpy as np
 import pytest
 
@@ -32,6 +33,17 @@ def assert_cast_equal(g, o, rows=None, what=""):
                                  f"gpu {a[tuple(bad[0])]} cpu {b[tuple(bad[0])]}")
 
 
+def assert_uncounted_equal(torch, tree, view, w, h, pal, counted, what=""):
+    """A cast without the read counters (the kernel variant that leaves them
+    out) gives the same RGBA8, t and first voxel as the counted cast."""
+    dpal = torch.from_numpy(pal.view(np.int32)).cuda()
+    out = tree.render_tensors(view, w, h, dpal, outputs=("rgba", "t", "voxel"))
+    torch.cuda.synchronize()
+    for k, v in out.items():
+        a = v.cpu().numpy().view(DT[k])
+        assert np.array_equal(a, counted[k]), f"{what}: {k} differs without the read counters"
+
+
 def views_for(yx, w, h, seed=0):
     cams = yx.workloads.orbit_cameras(3, seed=seed) + [((0.41, 0.37, 0.55), (0.4, 1.0, -0.3)),
                                                          ((0.5, 0.5, 2.5), (0.0, 1e-3, -1.0))]
@@ -59,6 +71,8 @@ def test_dense_build_and_cast(yx, oracle_mod, cuda, size, levels):
             t.set_launch(persistent, refill)
             g = gpu_cast(cuda, t, view, w, h, pal)
             assert_cast_equal(g, c, what=f"size {size} levels {levels} view {i} launch {persistent}/{refill}")
+        t.set_launch(False, 16)
+        assert_uncounted_equal(cuda, t, view, w, h, pal, g, what=f"size {size} levels {levels} view {i}")
         assert (g["t"] >= 0).any()
 
 
@@ -128,6 +142,7 @@ def test_brick_4096_full(yx, oracle_mod, cuda):
     c = o.render(root, nodes, bricks, cfg["size"], cfg["levels"], view, pal, w, h, ystep=8, threads=16)
     rows = np.arange(0, h, 8)
     assert_cast_equal(g, c, rows=rows, what="brick_4096 1080p")
+    assert_uncounted_equal(torch, t, view, w, h, pal, g, what="brick_4096 1080p")
     hit = (g["t"] >= 0).mean()
     print(f"brick_4096 frame: {hit:.3f} of the rays reach a non-empty voxel", flush=True)
     assert 0.05 < hit
