@@ -44,6 +44,9 @@ class Oracle:
         l.oracle_trace_rays.restype = ctypes.c_int
         l.oracle_pixel_dirs.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _f32p]
         l.oracle_pixel_dirs.restype = None
+        l.oracle_shade_rays.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _f32p, _f32p,
+                                        ctypes.c_int, _f32p, _u32p, _u32p, _u8p, _f32p, _u32p]
+        l.oracle_shade_rays.restype = ctypes.c_int
         l.oracle_plant.restype = ctypes.c_int
         self.lib = l
 
@@ -131,6 +134,28 @@ class Oracle:
         if rc < 0:
             raise ValueError("oracle followed a child id out of range")
         return {"hit_id": hit, "voxel": vox, "t": t, "level": lvl, "t1": t1, "visits": vis}
+
+    def shade_rays(self, nodes, root, pos, dirs, light_mode=0, light=(0, 0, 1)):
+        """RenderRect's per-pixel work (trace, Lambert, shadow ray) for explicit
+        rays: dict of hit_id, rgba, shadow, t, visits."""
+        nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = dirs.shape[0]
+        if pos.shape[0] == 1 and n > 1:
+            pos = np.ascontiguousarray(np.repeat(pos, n, axis=0))
+        lt = np.ascontiguousarray(light, np.float32).reshape(3)
+        out = {"hit_id": np.zeros(n, np.uint32), "rgba": np.zeros(n, np.uint32),
+               "shadow": np.zeros(n, np.uint8), "t": np.zeros(n, np.float32),
+               "visits": np.zeros(n, np.uint32)}
+        rc = self.lib.oracle_shade_rays(self._p(nodes, _u32p), nodes.size // 10, int(root), n,
+                                        self._p(pos, _f32p), self._p(dirs, _f32p), int(light_mode),
+                                        self._p(lt, _f32p), self._p(out["hit_id"], _u32p),
+                                        self._p(out["rgba"], _u32p), self._p(out["shadow"], _u8p),
+                                        self._p(out["t"], _f32p), self._p(out["visits"], _u32p))
+        if rc < 0:
+            raise ValueError("oracle followed a child id out of range")
+        return out
 
     def plant(self):
         return self.lib.oracle_plant()

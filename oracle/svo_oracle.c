@@ -28,6 +28,11 @@
  * Pins (tests/test_oracle_pins.py): brute-force first-hit search on dense grids
  * of depth 1..5, closed-form rays, hand-derived tie cases, invariants, and
  * planted-mistake builds (ORACLE_PLANT) that the pins must reject.
+ * Shading pins (tests/test_oracle_shade_pins.py): axis rays and each of the six
+ * faces against the closed-form Lambert term, the entry face against a float64
+ * brute force of the hit box, and shadow-ray occlusion against an exhaustive
+ * float64 search from the stepped origin; plants 12-14 (light sign flipped,
+ * origin stepped into the hit voxel, ray cast away from the light) fail them.
  */
 #include <math.h>
 #include <stdint.h>
@@ -356,7 +361,13 @@ static void shade_pixel(const uint32_t* nodes, uint32_t num_nodes, uint32_t root
     if (light_mode == 0) {
         lam = fabsf(dir[e]);
     } else {
+        /* n = outward normal of the entry face: +e_e when the ray moves toward
+         * -e (dir[e] < 0), else -e_e; n.L = +-light[e] */
+#if ORACLE_PLANT == 12
+        lam = (dir[e] < 0) ? -light[e] : light[e]; /* planted: light sign flipped */
+#else
         lam = (dir[e] < 0) ? light[e] : -light[e];
+#endif
         lam = o_max(lam, 0.0f);
     }
     int lit = 1;
@@ -366,12 +377,24 @@ static void shade_pixel(const uint32_t* nodes, uint32_t num_nodes, uint32_t root
         float p[3];
         for (int i = 0; i < 3; ++i) p[i] = pos[i] + dir[i] * h.t;
         float half = ldexpf(1.0f, -(h.level + 1));
+#if ORACLE_PLANT == 13
+        if (dir[e] < 0) /* planted: origin stepped into the hit voxel */
+            p[e] = p[e] - half;
+        else
+            p[e] = p[e] + half;
+#else
         if (dir[e] < 0)
             p[e] = p[e] + half;
         else
             p[e] = p[e] - half;
+#endif
         trace_hit sh;
+#if ORACLE_PLANT == 14
+        const float away[3] = {-light[0], -light[1], -light[2]}; /* planted: ray cast away from the light */
+        lit = !trace_ray(nodes, num_nodes, root, p, away, 1, 0.0f, &sh, &o->visits, bad_id);
+#else
         lit = !trace_ray(nodes, num_nodes, root, p, light, 1, 0.0f, &sh, &o->visits, bad_id);
+#endif
         o->shadow = lit ? 0 : 1;
     }
     float k = lit ? 0.1f + 0.9f * lam : 0.1f;
@@ -445,6 +468,29 @@ int oracle_trace_rays(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, 
             out_level[i] = 0;
             for (int k = 0; k < 3; ++k) out_t1[3 * i + k] = 0.0f;
         }
+    }
+    return bad ? -1 : 0;
+}
+
+/* Shaded single rays with explicit origins and directions (for the pins on
+ * the Lambert term and the shadow rays): exactly the per-pixel work of
+ * RenderRect (shade_pixel) with the pixel direction replaced by dir[i].
+ * Outputs per ray: hit id (node*8+child or miss), RGBA8, shadow flag (0 lit,
+ * 1 occluded, 2 no shadow ray), t, hit level and node visits (primary +
+ * shadow). Returns -1 on a bad child id. */
+int oracle_shade_rays(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, int n, const float* pos,
+                      const float* dir, int light_mode, const float* light, uint32_t* out_hit,
+                      uint32_t* out_rgba, uint8_t* out_shadow, float* out_t, uint32_t* out_visits) {
+    int bad = 0;
+    for (int i = 0; i < n; ++i) {
+        pixel_out o;
+        shade_pixel(nodes, num_nodes, root, pos + 3 * i,
+                    f3_make(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]), light_mode, light, &o, &bad);
+        out_hit[i] = o.hit_id;
+        out_rgba[i] = o.rgba;
+        out_shadow[i] = o.shadow;
+        out_t[i] = o.t;
+        if (out_visits) out_visits[i] = o.visits;
     }
     return bad ? -1 : 0;
 }
