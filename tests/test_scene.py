@@ -96,6 +96,13 @@ def test_load_errors(yx, tmp_path):
     with pytest.raises(yx.YoxelError) as e:
         yx.SVOData().Load(hdr)
     assert e.value.code == -3
+    # a bare header that names 2^32 - 1 nodes (171 GB) is rejected from the
+    # file length, before any allocation
+    huge = tmp_path / "huge.vox"
+    huge.write_bytes(struct.pack("<4I", 0, 0, 0, 0xFFFFFFFF))
+    with pytest.raises(yx.YoxelError) as e:
+        yx.SVOData().Load(huge)
+    assert e.value.code == -3 and "short file" in str(e.value)
     # a child id beyond the node array
     bad = np.zeros((1, 10), np.uint32)
     bad[0, 2:] = 0x80000000

@@ -56,7 +56,22 @@ def camera_views(cams, fov, w, h):
     return out
 
 
+def _check(name, x, dtype, shape_tail=None):
+    """The C ABI takes raw device pointers: refuse tensors whose bytes would be
+    misread (wrong dtype, host memory, strided views)."""
+    import torch
+    if not isinstance(x, torch.Tensor) or x.dtype != dtype or not x.is_cuda or not x.is_contiguous():
+        got = (f"{x.dtype} on {x.device}, contiguous={x.is_contiguous()}" if isinstance(x, torch.Tensor)
+               else type(x).__name__)
+        raise TypeError(f"{name} must be a contiguous CUDA tensor of {dtype}, got {got}")
+    if shape_tail is not None and tuple(x.shape[1:]) != tuple(shape_tail):
+        raise ValueError(f"{name} must have shape [n, {', '.join(map(str, shape_tail))}], got {tuple(x.shape)}")
+
+
 def _vol_dev(table, params, grid, brick):
+    import torch
+    _check("params", params, torch.float32, (4,))
+    _check("table", table, torch.int32)
     n_slots = params.shape[0] // (brick ** 3)
     return _lib.VolumeDev(table.data_ptr(), params.data_ptr(), grid, brick, n_slots)
 
@@ -83,6 +98,7 @@ class Compositor:
         """params: [slots * brick^3, 4] float32 CUDA; views: [n, 12] float32 CUDA.
         Returns rgb [n, h, w, 3], transmittance [n, h, w], samples [n, h, w]."""
         import torch
+        _check("views", views, torch.float32, (12,))
         n = views.shape[0]
         dev = params.device
         rgb = torch.empty((n, h, w, 3), dtype=torch.float32, device=dev)
@@ -95,10 +111,15 @@ class Compositor:
 
     def backward(self, table, params, grid, brick, views, w, h, rgb, nsamp, grad_rgb):
         import torch
+        _check("views", views, torch.float32, (12,))
+        _check("rgb", rgb, torch.float32)
+        _check("nsamp", nsamp, torch.int32)
         n = views.shape[0]
-        grad = torch.empty_like(params)
         vol = _vol_dev(table, params, grid, brick)
-        g = grad_rgb.contiguous()
+        grad = torch.empty(params.shape, dtype=params.dtype, device=params.device)
+        g = grad_rgb.to(torch.float32).contiguous()
+        if g.shape != rgb.shape:
+            raise ValueError(f"grad_rgb shape {tuple(g.shape)} differs from rgb {tuple(rgb.shape)}")
         check(lib().yx_composite_backward(self._h, ctypes.byref(vol), views.data_ptr(), n, w, h, rgb.data_ptr(),
                                           nsamp.data_ptr(), g.data_ptr(), grad.data_ptr(), _stream()))
         return grad

@@ -51,7 +51,7 @@ struct yx_batch {
     uint32_t* d_counters = nullptr;
     int device = -1, num_cus = 0;
     size_t max_lds = 0;
-    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 32, 1, 0, 0, 0};
+    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 64, 1, 0, 0, 0};
     // graph of one frame
     bool use_graph = true;
     hipStream_t cap_stream = nullptr;
@@ -341,7 +341,7 @@ static int plan_launch(yx_batch* b, const yx_outputs& out, BatchLaunch& L) {
     const uint32_t tile = (uint32_t)(t.tile_w * t.tile_h);
     a.total = b->total_tiles * tile;
     a.nq = t.queues;
-    a.per_q = ((b->total_tiles + (uint32_t)t.queues - 1) / (uint32_t)t.queues) * tile;
+    a.per_q = queue_rays(b->total_tiles, tile, t.queues);
     a.refill_min = t.refill_min;
     a.levels = b->levels;
     a.jobs = b->d_jobs;

@@ -673,6 +673,10 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     const size_t rays = (size_t)n_views * w * h;
     const size_t nvox = (size_t)vol->num_slots * vol->brick * vol->brick * vol->brick;
     if (rays >= 0xFFFFFFFFull) return fail(YX_ERR_RANGE, "too many rays");
+    // the last call's statistics point into the workspace, which may be
+    // replaced below: forget them first
+    c->sorted_keys = nullptr;
+    c->counted = false;
 
     // 1. per-wave sample counts (8x8 ray tiles), exclusive scan -> each
     //    wave's record offset, and the total
@@ -727,8 +731,6 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
                                    (float4*)grad_params, n4);
         YX_HIP(hipGetLastError());
     }
-    c->sorted_keys = nullptr;
-    c->counted = false;
     if (total == 0) return YX_OK;
 
     // 2. workspace: keys, record ids, their sorted copies, values, slot

@@ -243,7 +243,7 @@ int tune_coordinate_descent(const yx_tuning& start, TimeFn time_ms, yx_tune_tria
         return YX_OK;
     };
     static const int tiles[][2] = {{8, 8}, {16, 4}, {4, 16}, {16, 8}, {8, 16}, {16, 16}, {32, 8}};
-    static const int refills[] = {8, 16, 24, 32, 40, 48};
+    static const int refills[] = {24, 32, 40, 48, 56, 64};
     static const int windows[] = {0, 2, 4};
     static const int blocks[] = {64, 128, 256};
     static const int queues[] = {1, 8, 32};

@@ -1828,6 +1828,19 @@ extern "C" int yx_scene_load_vox(const char* path, yx_scene** out) {
         return fail(YX_ERR_IO, "%s: short header", path);
     }
     const uint32_t size = hdr[3];
+    // the header's node count must fit the file before anything is allocated:
+    // a truncated or foreign file cannot ask for gigabytes
+    long here = ftell(f), end = -1;
+    if (here >= 0 && fseek(f, 0, SEEK_END) == 0) end = ftell(f);
+    if (end < 0 || fseek(f, here, SEEK_SET) != 0) {
+        fclose(f);
+        return fail(YX_ERR_IO, "%s: cannot seek", path);
+    }
+    if ((uint64_t)(end - here) < (uint64_t)size * 40u) {
+        fclose(f);
+        return fail(YX_ERR_IO, "%s: short file (%ld bytes of nodes, the header names %u nodes)", path,
+                    end - here, size);
+    }
     std::vector<uint32_t> nodes;
     try {
         nodes.resize((size_t)size * 10);

@@ -605,7 +605,10 @@ static int gpu_build(const yx_scene_desc* d, const EditSet& edits, yx_scene* s, 
         s->depth = 0;
         return YX_OK;
     }
-    if (total >= 0xFFFFFFF0ull) return fail(YX_ERR_RANGE, "%llu surface voxels", (unsigned long long)total);
+    // the sort and the per-level unique passes take int counts, and every
+    // level holds at most `total` codes: 2^31 - 1 bounds them all
+    if (total >= 0x7FFFFFFFull)
+        return fail(YX_ERR_RANGE, "%llu surface voxels (at most 2^31 - 2)", (unsigned long long)total);
     uint64_t *d_codes = nullptr, *d_sorted = nullptr;
     if ((rc = B.alloc(&d_codes, total)) != YX_OK || (rc = B.alloc(&d_sorted, total)) != YX_OK) return rc;
     hipLaunchKernelGGL(sb_surface_kernel, dim3((unsigned)((ncol + 127) / 128)), dim3(128), 0, st, a, d_offs,

@@ -107,6 +107,7 @@ using namespace yx;
 // ---- renderer object ------------------------------------------------------
 
 static constexpr int kMaxQueues = 64;
+static constexpr int kCounterWords = kMaxQueues + 1;
 
 struct yx_renderer {
     float pos[3] = {0.0f, 0.0f, 0.0f};
@@ -129,8 +130,10 @@ struct yx_renderer {
     int num_cus = 0;
     size_t max_lds = 0;  // dynamic LDS bytes a workgroup may use
     // measured best on MI355X for the 1080p depth-11 frame (stack kernel, one
-    // wave per workgroup, no register window, refill at 32 idle lanes)
-    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 32, 1, 0, 0, 0};
+    // wave per workgroup, no register window, refill at 64 idle lanes: a wave
+    // takes a whole 8x8 tile at once and traces its primary rays, then its
+    // shadow rays, in phase)
+    yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 64, 1, 0, 0, 0};
     // shading pipeline (SVORenderer::Render): per-record mean leaf colours,
     // settings with SVORenderer's defaults (demo/SVORenderer.cpp:6-16), and the
     // z / colour work buffers (m_zbuf[2], demo/SVORenderer.h:59)
@@ -176,12 +179,12 @@ static int ensure_device(yx_renderer* r) {
     int lds_max = 0;
     YX_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     r->max_lds = (size_t)lds_max;
-    hipError_t e = hipMalloc((void**)&r->d_counters, (kMaxQueues + 1) * sizeof(uint32_t));
+    hipError_t e = hipMalloc((void**)&r->d_counters, kCounterWords * sizeof(uint32_t));
     if (e != hipSuccess) {
         r->d_counters = nullptr;
         return fail(YX_ERR_OOM, "hipMalloc counters: %s", hipGetErrorString(e));
     }
-    YX_HIP(hipMemset(r->d_counters, 0, (kMaxQueues + 1) * sizeof(uint32_t)));
+    YX_HIP(hipMemset(r->d_counters, 0, kCounterWords * sizeof(uint32_t)));
     r->device = dev;
     r->num_cus = cus;
     return YX_OK;
@@ -262,8 +265,8 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
     const uint32_t tile = (uint32_t)(t.tile_w * t.tile_h);
     a.total = (uint32_t)a.tiles_x * (uint32_t)tiles_y * tile;
     a.nq = t.queues;
-    const uint32_t tiles_per_q = ((uint32_t)a.tiles_x * (uint32_t)tiles_y + t.queues - 1) / t.queues;
-    a.per_q = tiles_per_q * tile;
+    const uint32_t n_tiles = (uint32_t)a.tiles_x * (uint32_t)tiles_y;
+    a.per_q = queue_rays(n_tiles, tile, t.queues);  // stack kernel: interleaved tile groups
     a.refill_min = t.refill_min;
     // stack rows: levels 0 .. node_levels - 2 are pushed
     a.levels = std::max(1, r->node_levels - 1);
@@ -314,6 +317,8 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
         kernel = YX_KERNEL_REPLAY;
         block = 256;
     }
+    // the replay kernel's queues are contiguous bands of tiles
+    a.per_q = ((n_tiles + (uint32_t)t.queues - 1u) / (uint32_t)t.queues) * tile;
     a.levels = std::max(1, r->node_levels);
     block = block >= 512 ? 512 : 256;
     size_t lds = (size_t)a.levels * (size_t)block * sizeof(uint2);
@@ -742,8 +747,7 @@ extern "C" int yx_render_pipeline(yx_renderer* r, const yx_pipeline_outputs* out
     const uint32_t tile = (uint32_t)(t.tile_w * t.tile_h);
     a.total = (uint32_t)a.tiles_x * (uint32_t)tiles_y * tile;
     a.nq = t.queues;
-    const uint32_t tiles_per_q = ((uint32_t)a.tiles_x * (uint32_t)tiles_y + t.queues - 1) / t.queues;
-    a.per_q = tiles_per_q * tile;
+    a.per_q = queue_rays((uint32_t)a.tiles_x * (uint32_t)tiles_y, tile, t.queues);
     a.refill_min = t.refill_min;
     a.levels = std::max(1, r->node_levels - 1);
     a.lod = r->d_lod;

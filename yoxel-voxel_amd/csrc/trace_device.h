@@ -97,19 +97,36 @@ __device__ __forceinline__ int find_first_child(Cell& c) {
     return (bx ? 1 : 0) | (by ? 2 : 0) | (bz ? 4 : 0);
 }
 
-// GoNext, cell/spu/trace_spu.cpp:70-93 (argmin tie order of lines 75-78)
+// Exit axis of a cell: argmin(t2) with the tie order of GoNext,
+// cell/spu/trace_spu.cpp:75-78.
+__device__ __forceinline__ int exit_axis(const Cell& c) {
+    if (c.t2x > c.t2y) return (c.t2y < c.t2z) ? 1 : 2;
+    return (c.t2x < c.t2z) ? 0 : 2;
+}
+
+// GoNext, cell/spu/trace_spu.cpp:70-93: leave through the exit plane into the
+// sibling, or fail when the current child already lies on that side. Written
+// without branches on the axis (selects), so lanes of a wave that leave
+// through different planes run one instruction stream; the float operations
+// are the reference's: dt = t2 - t1, t1 = t2, t2 += dt on the exit axis.
 __device__ __forceinline__ bool go_next(int& ch, Cell& c) {
-    int e;
-    if (c.t2x > c.t2y)
-        e = (c.t2y < c.t2z) ? 1 : 2;
-    else
-        e = (c.t2x < c.t2z) ? 0 : 2;
-    const int mask = 1 << e;
+    const bool gt = c.t2x > c.t2y;
+    const bool ex = !gt && (c.t2x < c.t2z);
+    const bool ey = gt && (c.t2y < c.t2z);
+    const int mask = ex ? 1 : (ey ? 2 : 4);
     if (ch & mask) return false;
     ch ^= mask;
-    if (e == 0) { float dt = c.t2x - c.t1x; c.t1x = c.t2x; c.t2x += dt; }
-    else if (e == 1) { float dt = c.t2y - c.t1y; c.t1y = c.t2y; c.t2y += dt; }
-    else { float dt = c.t2z - c.t1z; c.t1z = c.t2z; c.t2z += dt; }
+    const float t1 = ex ? c.t1x : (ey ? c.t1y : c.t1z);
+    const float t2 = ex ? c.t2x : (ey ? c.t2y : c.t2z);
+    const float dt = t2 - t1;
+    const float n2 = t2 + dt;
+    const bool ez = !ex && !ey;
+    c.t1x = ex ? t2 : c.t1x;
+    c.t2x = ex ? n2 : c.t2x;
+    c.t1y = ey ? t2 : c.t1y;
+    c.t2y = ey ? n2 : c.t2y;
+    c.t1z = ez ? t2 : c.t1z;
+    c.t2z = ez ? n2 : c.t2z;
     return true;
 }
 

@@ -32,6 +32,23 @@ struct QueueCursor {
     int left;  // queues not yet found drained (wave-uniform)
 };
 
+// Queue q owns the groups of 2^kQueueGroupLog consecutive tiles whose group
+// index is q modulo the queue count, in order. Every queue therefore walks the
+// whole image from the first tile row to the last at about the same pace (the
+// waves of all queues share one region of the scene in the caches), while one
+// queue's workgroups - one XCD's, since workgroup b serves queue b % nq and
+// workgroups are dealt to the 8 XCDs in turn - write whole output lines: 16
+// tiles of 8 pixels span 128 pixels of a row, a 128-byte line even for the
+// one-byte shadow flags. One queue is the plain tile order.
+constexpr int kQueueGroupLog = 4;
+
+// Host side: rays per queue for `tiles` tiles of `tile` rays over nq queues.
+inline uint32_t queue_rays(uint32_t tiles, uint32_t tile, int nq) {
+    const uint32_t groups = (tiles + (1u << kQueueGroupLog) - 1u) >> kQueueGroupLog;
+    const uint32_t per_q = (groups + (uint32_t)nq - 1u) / (uint32_t)nq;
+    return (per_q << kQueueGroupLog) * tile;
+}
+
 // In a batched launch the ray id's tile is looked up in the job table (binary
 // search over the jobs' first tiles) and x, y are within that job's image.
 template <bool BATCH>
@@ -42,18 +59,20 @@ __device__ __forceinline__ bool take_ray(const PersistArgs& a, QueueCursor& qc, 
     uint32_t got = 0;
     if ((int)lane == leader) got = atomicAdd(&a.counters[qc.q], cnt);
     got = __shfl(got, leader);
-    const uint32_t qbeg = (uint32_t)qc.q * a.per_q;
-    const uint32_t qend = min(a.total, qbeg + a.per_q);
-    if (qbeg + got + cnt >= qend) {
+    const uint32_t q = (uint32_t)qc.q;
+    if (got + cnt >= a.per_q) {
         qc.q = (qc.q + 1 == a.nq) ? 0 : qc.q + 1;
         --qc.left;
     }
-    if (!idle || qbeg + got >= qend) return false;
-    const uint32_t my = qbeg + got + lanes_below(need);
-    if (my >= qend) return false;
+    if (!idle || got >= a.per_q) return false;
+    const uint32_t pos = got + lanes_below(need);  // ray position within queue q
+    if (pos >= a.per_q) return false;
     const int tl = a.tile_wl + a.tile_hl;
-    uint32_t tile = my >> tl;
-    const uint32_t within = my & ((1u << tl) - 1u);
+    const uint32_t tq = pos >> tl;  // tile position within the queue
+    uint32_t tile = ((((tq >> kQueueGroupLog) * (uint32_t)a.nq) + q) << kQueueGroupLog) |
+                    (tq & ((1u << kQueueGroupLog) - 1u));
+    if (tile >= (a.total >> tl)) return false;  // padding of the last group
+    const uint32_t within = pos & ((1u << tl) - 1u);
     int tiles_x = a.tiles_x, w = a.w, h = a.h;
     if constexpr (BATCH) {
         // job_tiles[lo] <= tile < job_tiles[hi]
@@ -196,6 +215,15 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     // instead of 32 B.
     uint64_t ffcpath = 0;  // FindFirstChild result per level (W == 0)
     uint32_t cur = 0;      // record index of `node` (W == 0)
+    // W == 0: bit l set when level l of the current path can still move to a
+    // sibling once its child is exhausted, i.e. when the exit axis of the
+    // level's cell (argmin t2, GoNext's tie order) is clear in its child slot.
+    // The recursion hands the child a copy of (t1, t2) and finds its own cell
+    // unchanged on return, so the bit, taken when the level descends, is the
+    // outcome of the GoNext the recursion runs there after the child returns.
+    // A failed GoNext therefore returns in one step to the deepest such level
+    // and advances there; the levels in between would have failed in turn.
+    uint32_t advmask = 0;
     uint32_t* sh_idx = reinterpret_cast<uint32_t*>(sh_stk);
     uint4* sh_ca = reinterpret_cast<uint4*>(sh_idx + (size_t)a.levels * BLOCK);
     uint2* sh_cb = reinterpret_cast<uint2*>(sh_ca + (size_t)((a.levels + 1) >> 1) * BLOCK);
@@ -429,6 +457,8 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     win[0].rec = node;
                 } else {
                     push0(level, c, cur);
+                    const uint32_t bit = 1u << level;
+                    advmask = (((uint32_t)ch >> exit_axis(c)) & 1u) ? (advmask & ~bit) : (advmask | bit);
                 }
                 chpath = set3(chpath, level, (uint32_t)ch);
                 const uint32_t below = (cm | lm) & ((1u << real) - 1u);
@@ -454,24 +484,36 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
         }
         if (go_next(ch, c)) {
             adv = false;
-        } else if (level == 0) {
+            continue;
+        }
+        // the current level is exhausted: W == 0 returns to the deepest level
+        // that can advance and advances there; W > 0 pops one level
+        uint32_t up = 0;
+        if constexpr (W == 0) {
+            up = advmask & ((1u << level) - 1u);
+        } else {
+            up = level;
+        }
+        if (up == 0) {
             if (status == sTraceP) {
                 status = sMissP;
             } else {
                 occluded = false;
                 status = sEndS;
             }
+        } else if constexpr (W == 0) {
+            level = 31 - __builtin_clz(up);
+            pop0(level, c, node);
+            ch = (int)bits3(chpath, level);
+            go_next(ch, c);  // succeeds: the level's exit axis is clear in ch
+            adv = false;
         } else {
             --level;
-            if constexpr (W > 0) {
-                c = win[0].c;
-                node = win[0].rec;
+            c = win[0].c;
+            node = win[0].rec;
 #pragma unroll
-                for (int k = 0; k + 1 < W; ++k) win[k] = win[k + 1];
-                if (level >= W) lds_load(level - W, win[W - 1].c, win[W - 1].rec);
-            } else {
-                pop0(level, c, node);
-            }
+            for (int k = 0; k + 1 < W; ++k) win[k] = win[k + 1];
+            if (level >= W) lds_load(level - W, win[W - 1].c, win[W - 1].rec);
             ch = (int)bits3(chpath, level);
         }
     }
