@@ -230,6 +230,16 @@ int yx_renderer_ray_basis(const yx_renderer* r, float basis[9]);
  * (demo/SVORenderer.cpp:95-149); the reference's full GPU pipeline is
  * yx_render_pipeline below. */
 int yx_render(yx_renderer* r, const yx_outputs* out, void* stream);
+/* yx_render without any device allocation. No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, serving
+ * loops that forbid hipMalloc). The scene must already be set (set_scene
+ * allocates at set-up). The persistent kernels' scratch (ray-queue heads and
+ * retired-wave count: 65 words, 260 bytes) is a fixed per-device array inside
+ * the library instead of renderer-owned memory; a tuning with more queues than
+ * it holds is refused with YX_ERR_RANGE. Outputs equal yx_render's. Every call
+ * on one GPU shares that array: calls that overlap on one GPU (other streams,
+ * other threads, other renderers) race and must be serialised by the caller. */
+int yx_render_noalloc(yx_renderer* r, const yx_outputs* out, void* stream);
 /* RenderFrame (cell/ppu_renderer.cpp:76-85): synchronous; returns a pointer to
  * a renderer-owned host RGBA8 buffer valid until the next call or resolution
  * change. Fails with YX_ERR_NO_SCENE when no scene is set (the reference

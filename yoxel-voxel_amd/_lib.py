@@ -245,6 +245,7 @@ SIGNATURES = {
     "yx_renderer_set_light": (ctypes.c_int, [_P, _F3, ctypes.c_int32]),
     "yx_renderer_ray_basis": (ctypes.c_int, [_P, _F3]),
     "yx_render": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), _P]),
+    "yx_render_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), _P]),
     "yx_render_frame": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "yx_renderer_scene_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "yx_renderer_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),

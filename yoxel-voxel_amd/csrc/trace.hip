@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "internal.h"
@@ -170,8 +171,9 @@ struct yx_renderer {
 // RendererBase::InitRayDir (cell/renderer_base.h:50-61).
 static void init_ray_dir(const yx_renderer* r, float* out) { camera_basis(r->dir, r->up, r->fov, r->w, r->h, out); }
 
-static int ensure_device(yx_renderer* r) {
-    if (r->device >= 0) return YX_OK;
+// Compute units and LDS limit of the current device (no allocation).
+static int query_device(yx_renderer* r) {
+    if (r->num_cus > 0) return YX_OK;
     int dev = 0;
     YX_HIP(hipGetDevice(&dev));
     int cus = 0;
@@ -179,6 +181,17 @@ static int ensure_device(yx_renderer* r) {
     int lds_max = 0;
     YX_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     r->max_lds = (size_t)lds_max;
+    r->num_cus = cus;
+    return YX_OK;
+}
+
+// query_device, plus the renderer's own queue counters.
+static int ensure_device(yx_renderer* r) {
+    if (r->device >= 0) return YX_OK;
+    int rc = query_device(r);
+    if (rc != YX_OK) return rc;
+    int dev = 0;
+    YX_HIP(hipGetDevice(&dev));
     hipError_t e = hipMalloc((void**)&r->d_counters, kCounterWords * sizeof(uint32_t));
     if (e != hipSuccess) {
         r->d_counters = nullptr;
@@ -186,7 +199,6 @@ static int ensure_device(yx_renderer* r) {
     }
     YX_HIP(hipMemset(r->d_counters, 0, kCounterWords * sizeof(uint32_t)));
     r->device = dev;
-    r->num_cus = cus;
     return YX_OK;
 }
 
@@ -215,7 +227,10 @@ static int launch_persistent(yx_renderer* r, const void* fn, void (*kern)(Persis
 #define YX_LAUNCH(KERNEL, BLOCK) \
     launch_persistent(r, (const void*)KERNEL, KERNEL, BLOCK, a, lds, stream)
 
-static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t stream) {
+// `counters`: the persistent kernels' queue heads and retired-wave count
+// (kCounterWords words, zero on entry; the kernels leave them zero); null
+// means the renderer's own, allocated on first use.
+static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t stream, uint32_t* counters = nullptr) {
     float basis[9];
     init_ray_dir(r, basis);
     if (r->tune.kernel == YX_KERNEL_PER_PIXEL || !r->d_recs) {
@@ -242,7 +257,7 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
         YX_HIP(hipGetLastError());
         return YX_OK;
     }
-    int rc = ensure_device(r);
+    int rc = counters ? query_device(r) : ensure_device(r);
     if (rc != YX_OK) return rc;
     PersistArgs a;
     std::memset(&a, 0, sizeof(a));
@@ -256,7 +271,7 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
     a.w = r->w;
     a.h = r->h;
     a.out = out;
-    a.counters = r->d_counters;
+    a.counters = counters ? counters : r->d_counters;
     const yx_tuning& t = r->tune;
     a.tile_wl = __builtin_ctz((unsigned)t.tile_w);
     a.tile_hl = __builtin_ctz((unsigned)t.tile_h);
@@ -533,6 +548,62 @@ extern "C" int yx_render(yx_renderer* r, const yx_outputs* out, void* stream) {
     if (!r || !out) return fail(YX_ERR_INVALID_ARG, "null argument");
     if (!r->has_scene) return fail(YX_ERR_NO_SCENE, "no scene set");
     return launch_frame(r, *out, (hipStream_t)stream);
+}
+
+// ---- allocation-free rendering -------------------------------------------
+// For callers that cannot allocate device memory (code under HIP graph
+// capture, serving loops that forbid it): the persistent kernels' only scratch
+// - the ray-queue heads and the retired-wave count, kScratchWords words
+// (260 bytes) - lives in this file-scope array instead of a renderer-owned
+// allocation. Each device holds its own copy from module load until the
+// process ends; every call on a device shares it. Host code uses the device
+// address that hipGetSymbolAddress returns for the current device, never the
+// array's name.
+static constexpr int kScratchWords = kCounterWords;
+static constexpr int kScratchDevices = 64;
+static __device__ __align__(256) uint32_t scratch[kScratchWords];
+static std::mutex g_scratch_mu;
+static uint32_t* g_scratch_ptr[kScratchDevices];  // per device, looked up once
+
+static int scratch_address(uint32_t** out) {
+    int dev = 0;
+    YX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices)
+        return fail(YX_ERR_RANGE, "device %d: the allocation-free path serves devices 0..%d", dev,
+                    kScratchDevices - 1);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(YX_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<uint32_t*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return YX_OK;
+}
+
+__global__ void zero_words_kernel(uint32_t* p, int n) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = 0u;
+}
+
+extern "C" int yx_render_noalloc(yx_renderer* r, const yx_outputs* out, void* stream) {
+    clear_error();
+    if (!r || !out) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (!r->has_scene) return fail(YX_ERR_NO_SCENE, "no scene set");
+    if (r->tune.queues + 1 > kScratchWords)
+        return fail(YX_ERR_RANGE, "%d ray queues need %d scratch words; the allocation-free path holds %d",
+                    r->tune.queues, r->tune.queues + 1, kScratchWords);
+    uint32_t* s = nullptr;
+    int rc = scratch_address(&s);
+    if (rc != YX_OK) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    // the queue heads must start at zero (the last call's kernels left them
+    // zero; a fill kernel rather than a memset node keeps the call capturable)
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(128), 0, st, s, kScratchWords);
+    YX_HIP(hipGetLastError());
+    return launch_frame(r, *out, st, s);
 }
 
 extern "C" int yx_render_frame(yx_renderer* r, const uint32_t** colors) {
