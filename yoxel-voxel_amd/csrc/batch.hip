@@ -349,7 +349,8 @@ static int plan_launch(yx_batch* b, const yx_outputs& out, BatchLaunch& L) {
     a.n_jobs = (int)b->jobs.size();
     a.tiles_x = 1;  // per job in the table
     a.w = a.h = 1;
-    const int win = t.window;
+    // W == 0 rows hold record indices below 2^29; larger pools use a window
+    const int win = (t.window == 0 && b->n_pool >= kStackMaxRecords) ? 2 : t.window;
     const size_t lane_bytes = stack_lane_bytes(b->levels, win);
     int block = t.block;
     size_t lds = lane_bytes * (size_t)block;

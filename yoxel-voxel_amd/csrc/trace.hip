@@ -290,7 +290,9 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
     if (kernel == YX_KERNEL_PERSISTENT) {
         // LDS stack rows per lane (stack_lane_bytes); shrink the workgroup, then
         // fall back to the replay kernel (same results) when they do not fit
-        const int win = t.window <= 0 ? 0 : (t.window <= 2 ? 2 : (t.window <= 4 ? 4 : 6));
+        // W == 0 rows hold record indices below 2^29; larger pools use a window
+        int win = t.window <= 0 ? 0 : (t.window <= 2 ? 2 : (t.window <= 4 ? 4 : 6));
+        if (win == 0 && r->n_recs >= kStackMaxRecords) win = 2;
         const size_t lane_bytes = stack_lane_bytes(a.levels, win);
         size_t lds = lane_bytes * (size_t)block;
         while (lds > r->max_lds && block > 64) {
@@ -827,6 +829,9 @@ extern "C" int yx_render_pipeline(yx_renderer* r, const yx_pipeline_outputs* out
     a.albedo = out->albedo ? out->albedo : r->d_albedo;
     a.hit_level = out->level;
     int block = std::min(256, std::max(64, t.block));
+    if (r->n_recs >= kStackMaxRecords)
+        return fail(YX_ERR_RANGE, "the LOD trace holds record indices below 2^29; the scene has %zu records",
+                    r->n_recs);
     const size_t lane_bytes = stack_lane_bytes(a.levels, 0);
     size_t lds = lane_bytes * (size_t)block;
     while (lds > r->max_lds && block > 64) {

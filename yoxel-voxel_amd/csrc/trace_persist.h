@@ -99,6 +99,19 @@ __device__ __forceinline__ void replay_axis(float& t1, float& t2, bool up, bool 
     t2 = n2;
 }
 
+// The same step when the FindFirstChild bits are not kept: they follow from
+// the node cell itself, as FindFirstChild computed them (tEnter = max(t1)
+// against each midpoint, the same float operations), so the result is
+// bit-identical.
+__device__ __forceinline__ void replay_level_ffc(Cell& s, uint32_t h) {
+    const float te = max3(s.t1x, s.t1y, s.t1z);
+    const float mx = 0.5f * (s.t1x + s.t2x), my = 0.5f * (s.t1y + s.t2y), mz = 0.5f * (s.t1z + s.t2z);
+    const uint32_t f = (te > mx ? 1u : 0u) | (te > my ? 2u : 0u) | (te > mz ? 4u : 0u);
+    replay_axis(s.t1x, s.t2x, (h & 1u) != 0, (f & 1u) != 0);
+    replay_axis(s.t1y, s.t2y, (h & 2u) != 0, (f & 2u) != 0);
+    replay_axis(s.t1z, s.t2z, (h & 4u) != 0, (f & 4u) != 0);
+}
+
 __device__ __forceinline__ void replay_level(Cell& s, uint32_t f, uint32_t h) {
     replay_axis(s.t1x, s.t2x, (h & 1u) != 0, (f & 1u) != 0);
     replay_axis(s.t1y, s.t2y, (h & 2u) != 0, (f & 2u) != 0);

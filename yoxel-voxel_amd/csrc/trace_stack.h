@@ -129,6 +129,9 @@ __device__ __forceinline__ void write_final(const yx_outputs& o, uint32_t pix, u
 // pushed: W == 0 keeps every level's node record index (4 B) and the cells
 // (24 B) of even levels; W > 0 keeps 32-byte rows for the levels older than
 // the window.
+// Record indices the W == 0 stack rows hold (index << 3 | child slot).
+constexpr size_t kStackMaxRecords = (size_t)1 << 29;
+
 inline size_t stack_lane_bytes(int levels, int W) {
     if (W == 0) return (size_t)4 * (size_t)levels + (size_t)24 * (size_t)((levels + 1) / 2);
     return (size_t)32 * (size_t)(levels - W > 1 ? levels - W : 1);
@@ -207,13 +210,13 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     bool lod_stop = false;  // LOD: the ray ended on a node
     uint32_t lod_rec = 0;   // LOD: that node's record
 
-    // W == 0: every level's node record index ([levels][BLOCK] uint32; a pop
-    // reads the record again, mostly from the cache that served it on the way
-    // down), and the cell of even levels only ([(levels + 1) / 2][BLOCK] as
-    // uint4 + uint2). A pop to an odd level rebuilds its cell from the level
-    // above by one exact replay step, so a lane needs 4 B + 12 B per level
-    // instead of 32 B.
-    uint64_t ffcpath = 0;  // FindFirstChild result per level (W == 0)
+    // W == 0: every level's node record index and child slot as one word,
+    // index << 3 | slot ([levels][BLOCK] uint32; a pop reads the record again,
+    // mostly from the cache that served it on the way down; record indices
+    // stay below 2^29), and the cell of even levels only ([(levels + 1) / 2]
+    // [BLOCK] as uint4 + uint2). A pop to an odd level rebuilds its cell from
+    // the level above by one exact replay step, so a lane needs 4 B + 12 B per
+    // level instead of 32 B.
     uint32_t cur = 0;      // record index of `node` (W == 0)
     // W == 0: bit l set when level l of the current path can still move to a
     // sibling once its child is exhausted, i.e. when the exit axis of the
@@ -227,8 +230,8 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
     uint32_t* sh_idx = reinterpret_cast<uint32_t*>(sh_stk);
     uint4* sh_ca = reinterpret_cast<uint4*>(sh_idx + (size_t)a.levels * BLOCK);
     uint2* sh_cb = reinterpret_cast<uint2*>(sh_ca + (size_t)((a.levels + 1) >> 1) * BLOCK);
-    auto push0 = [&](int lv, const Cell& cc, uint32_t ri) {
-        sh_idx[(size_t)lv * BLOCK + tid] = ri;
+    auto push0 = [&](int lv, const Cell& cc, uint32_t ri, int slot) {
+        sh_idx[(size_t)lv * BLOCK + tid] = (ri << 3) | (uint32_t)slot;
         if ((lv & 1) == 0) {
             const size_t r = (size_t)(lv >> 1) * BLOCK + tid;
             sh_ca[r] = make_uint4(__float_as_uint(cc.t1x), __float_as_uint(cc.t1y), __float_as_uint(cc.t1z),
@@ -236,8 +239,10 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
             sh_cb[r] = make_uint2(__float_as_uint(cc.t2y), __float_as_uint(cc.t2z));
         }
     };
-    auto pop0 = [&](int lv, Cell& cc, uint2& nn) {
-        cur = sh_idx[(size_t)lv * BLOCK + tid];
+    auto pop0 = [&](int lv, Cell& cc, uint2& nn, int& slot) {
+        const uint32_t v = sh_idx[(size_t)lv * BLOCK + tid];
+        cur = v >> 3;
+        slot = (int)(v & 7u);
         nn = rec_at(cur);
         const size_t r = (size_t)(lv >> 1) * BLOCK + tid;
         const uint4 r0 = sh_ca[r];
@@ -250,7 +255,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
         cc.t2z = __uint_as_float(r1.y);
         // odd level: the row holds the cell of level lv - 1, i.e. this level's
         // node interval; replay FindFirstChild / GoNext to the current child
-        if (lv & 1) replay_level(cc, bits3(ffcpath, lv), bits3(chpath, lv));
+        if (lv & 1) replay_level_ffc(cc, (uint32_t)slot);
     };
     auto lds_store = [&](int row, const Cell& cc, uint2 nn) {
         uint4* p = sh_stk + (size_t)(2 * row) * BLOCK + tid;
@@ -305,7 +310,11 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                 const uint2 leaf = rec_at(node.x + (uint32_t)__builtin_popcount(below));
                 int vx = 0, vy = 0, vz = 0;
                 for (int j = 0; j < level; ++j) {
-                    const uint32_t s = bits3(chpath, j) ^ flags;
+                    uint32_t s;
+                    if constexpr (W == 0)
+                        s = (sh_idx[(size_t)j * BLOCK + tid] & 7u) ^ flags;
+                    else
+                        s = bits3(chpath, j) ^ flags;
                     vx = (vx << 1) | (int)(s & 1u);
                     vy = (vy << 1) | (int)((s >> 1) & 1u);
                     vz = (vz << 1) | (int)((s >> 2) & 1u);
@@ -414,7 +423,6 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     level = 0;
                     adv = false;
                     ch = find_first_child(c);
-                    if constexpr (W == 0) ffcpath = set3(ffcpath, 0, (uint32_t)ch);
                     status = status == sStartP ? sTraceP : sTraceS;
                 } else if (status == sStartP) {
                     if constexpr (LOD)
@@ -455,12 +463,12 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     for (int k = W - 1; k > 0; --k) win[k] = win[k - 1];
                     win[0].c = c;
                     win[0].rec = node;
+                    chpath = set3(chpath, level, (uint32_t)ch);
                 } else {
-                    push0(level, c, cur);
+                    push0(level, c, cur, ch);
                     const uint32_t bit = 1u << level;
                     advmask = (((uint32_t)ch >> exit_axis(c)) & 1u) ? (advmask & ~bit) : (advmask | bit);
                 }
-                chpath = set3(chpath, level, (uint32_t)ch);
                 const uint32_t below = (cm | lm) & ((1u << real) - 1u);
                 const uint32_t crec = node.x + (uint32_t)__builtin_popcount(below);
                 node = rec_at(crec);
@@ -477,7 +485,6 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     }
                 }
                 ch = find_first_child(c);
-                if constexpr (W == 0) ffcpath = set3(ffcpath, level, (uint32_t)ch);
                 continue;
             }
             adv = true;
@@ -503,8 +510,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
             }
         } else if constexpr (W == 0) {
             level = 31 - __builtin_clz(up);
-            pop0(level, c, node);
-            ch = (int)bits3(chpath, level);
+            pop0(level, c, node, ch);
             go_next(ch, c);  // succeeds: the level's exit axis is clear in ch
             adv = false;
         } else {
