@@ -470,13 +470,25 @@ int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, const float
  * the exact sample sequence, emits one (voxel key, gradient) record per
  * sample, sorts the records by brick slot (bricks up to 8^3; larger bricks by
  * the whole key) and sums each voxel's records in emission order: no atomics
- * on gradients, deterministic. nsamp must be the forward's. Synchronizes
- * `stream` once to size its workspace. */
+ * on gradients, deterministic. nsamp must be the forward's.
+ * The record total (sum of nsamp) stays on the device. The workspace holds a
+ * capacity of records: it is sized with one wait on `stream` the first time a
+ * geometry (brick table and views pointers, sizes) is seen, unless
+ * yx_compositor_reserve set it; calls on a known geometry never wait (a
+ * training step can be captured in a graph). A call whose records exceed the
+ * capacity (its table was edited in place to hold more samples) writes a NaN
+ * gradient; the next call grows the workspace and yx_compositor_last_stats
+ * reports it. */
 int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol, const float* views, int32_t n_views,
                           int32_t width, int32_t height, const float* rgb, const uint32_t* nsamp,
                           const float* grad_rgb, float* grad_params, void* stream);
+/* Capacity of the backward's workspace in records (samples), so that no
+ * backward waits to size it. No reference counterpart (the reference has no
+ * adjoint). */
+int yx_compositor_reserve(yx_compositor* c, uint64_t records);
 /* Records emitted by the last backward (sum of nsamp) and distinct keys.
- * Statistics only: waits for the device, then reads them. */
+ * Statistics only: waits for the device, then reads them. YX_ERR_RANGE when
+ * the last backward overflowed its capacity. */
 int yx_compositor_last_stats(const yx_compositor* c, uint64_t* records, uint64_t* unique_keys);
 
 /* ---- brick pool manager (CuBrickPoolManager::allocMap,

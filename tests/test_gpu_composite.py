@@ -132,3 +132,40 @@ def test_autograd_function(yx, cuda):
             p -= lr * p.grad
         losses.append(float(l.detach()))
     assert all(b <= a for a, b in zip(losses, losses[1:])) and losses[-1] < 0.9 * losses[0]
+
+
+@pytest.mark.parametrize("brick", [8, 16])
+def test_backward_capacity(yx, cuda, brick):
+    """The record total stays on the device. A reserved capacity gives the same
+    bits as a backward that sized its own workspace; a capacity below the total
+    gives a NaN gradient and a reported error, and the next call grows the
+    workspace and is exact again. brick 8 reduces per slot, 16 by whole keys."""
+    torch = cuda
+    grid, w, h = 64, 40, 32
+    v, views, table, params, vd = setup_case(yx, torch, grid, brick, 0.4, 11, 3, w, h)
+    ref = yx.composite.Compositor()
+    rgb, _, ns = ref.forward(table, params, grid, brick, vd, w, h)
+    g_rgb = torch.sin(rgb * 7.0).contiguous()
+    g_ref = ref.backward(table, params, grid, brick, vd, w, h, rgb, ns, g_rgb)
+    torch.cuda.synchronize()
+    total = int(ns.to(torch.int64).sum())
+    assert ref.last_stats()[0] == total and total > 1000
+
+    exact = yx.composite.Compositor()
+    exact.reserve(total)
+    g = exact.backward(table, params, grid, brick, vd, w, h, rgb, ns, g_rgb)
+    torch.cuda.synchronize()
+    assert torch.equal(g, g_ref)
+
+    small = yx.composite.Compositor()
+    small.reserve(total // 2)
+    g = small.backward(table, params, grid, brick, vd, w, h, rgb, ns, g_rgb)
+    torch.cuda.synchronize()
+    assert torch.isnan(g).all(), "an overflowed backward must not return a gradient"
+    with pytest.raises(yx.YoxelError) as e:
+        small.last_stats()
+    assert e.value.code == yx._lib.YX_ERR_RANGE
+    g = small.backward(table, params, grid, brick, vd, w, h, rgb, ns, g_rgb)
+    torch.cuda.synchronize()
+    assert torch.equal(g, g_ref)
+    assert small.last_stats()[0] == total

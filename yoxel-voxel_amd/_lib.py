@@ -306,6 +306,7 @@ SIGNATURES = {
                                              ctypes.c_int32, _P, _P, _P, _P, _P]),
     "yx_compositor_last_stats": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64),
                                                 ctypes.POINTER(ctypes.c_uint64)]),
+    "yx_compositor_reserve": (ctypes.c_int, [_P, ctypes.c_uint64]),
     "yx_ntree_build_fbm": (ctypes.c_int, [ctypes.POINTER(NTreeDesc), ctypes.POINTER(_P), _P]),
     "yx_ntree_from_dense": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_P)]),
     "yx_ntree_get_info": (ctypes.c_int, [_P, ctypes.POINTER(NTreeInfo)]),

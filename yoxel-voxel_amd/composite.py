@@ -124,6 +124,10 @@ class Compositor:
                                           nsamp.data_ptr(), g.data_ptr(), grad.data_ptr(), _stream()))
         return grad
 
+    def reserve(self, records):
+        """Workspace capacity in records, so that no backward waits to size it."""
+        check(lib().yx_compositor_reserve(self._h, int(records)))
+
     def last_stats(self):
         r = ctypes.c_uint64()
         u = ctypes.c_uint64()

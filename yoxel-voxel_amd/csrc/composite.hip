@@ -19,14 +19,16 @@
 //   dL/draw_k   = dL/dsigma_k * sigmoid(raw_k)
 // The backward recomputes the exact discrete sample sequence instead of
 // storing per-ray lists, writes one (voxel key, float4 gradient) record per
-// sample at the ray's exclusive-scan offset, and radix-sorts the records
-// (with their ids, 0..n-1 from a buffer written once per size) by the brick
-// slot bits of the key. One wave per slot then sums each voxel's records in
-// emission order in LDS and writes the brick. Bricks above 8^3 sort by the
-// whole key and sum each key's run instead. No atomics on gradients; the
-// result is deterministic.
+// sample at the ray's exclusive-scan offset, and sorts the records' ids by
+// the brick slot bits of the key with a stable LSD radix sort of its own
+// (8-bit digits; per-block digit histograms, a scan, a scatter that ranks
+// equal digits in record order with wave ballots). One wave per slot then sums
+// each voxel's records in emission order in LDS and writes the brick. Bricks
+// above 8^3 sort by the whole key and sum each key's run instead. No atomics
+// on gradients; the result is deterministic. The record total stays on the
+// device: the workspace has a capacity, and a total beyond it is flagged and
+// turns the gradient into NaN instead of a silently wrong one.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -38,6 +40,10 @@
 #include "trace_device.h"
 
 namespace yx {
+
+// Words of the backward's small device block: record total, records used
+// (min(total, capacity)), overflow flag, distinct voxels.
+enum : int { kMiscTotal = 0, kMiscN = 1, kMiscOverflow = 2, kMiscDistinct = 3, kMiscWords = 64 };
 
 struct CompArgs {
     const int32_t* table;
@@ -348,9 +354,12 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
     }
 }
 
-// Statistics only (not on the gradient path): number of distinct sorted keys.
-__global__ __launch_bounds__(256) void comp_count_runs_kernel(const uint32_t* keys, uint32_t n, uint32_t* count) {
+// Statistics only (not on the gradient path): number of distinct sorted keys
+// among the *n_ptr records.
+__global__ __launch_bounds__(256) void comp_count_runs_kernel(const uint32_t* keys, const uint32_t* n_ptr,
+                                                              uint32_t* count) {
     __shared__ uint32_t part[4];
+    const uint32_t n = *n_ptr;
     uint32_t c = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         c += (i == 0 || keys[i - 1] != keys[i]) ? 1u : 0u;
@@ -358,13 +367,6 @@ __global__ __launch_bounds__(256) void comp_count_runs_kernel(const uint32_t* ke
     if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(count, part[0] + part[1] + part[2] + part[3]);
-}
-
-// 0, 1, 2, ...: the record ids the sort carries along with the keys (a
-// record's id is its position in emission order)
-__global__ void comp_iota_kernel(uint32_t* ids, size_t n) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) ids[i] = (uint32_t)i;
 }
 
 __global__ void comp_zero_kernel(float4* g, size_t n) {
@@ -379,9 +381,11 @@ __global__ void comp_zero_kernel(float4* g, size_t n) {
 // the wave holding its first record, reading on serially; waves that only
 // hold the continuation of an earlier run write nothing for it.
 __global__ __launch_bounds__(256) void comp_reduce_runs_kernel(const uint32_t* keys, const uint32_t* ids,
-                                                               const float4* vals, uint32_t n, float4* grad,
-                                                               uint32_t nvox) {
+                                                               const float4* vals, const uint32_t* n_ptr,
+                                                               float4* grad, uint32_t nvox) {
+    const uint32_t n = *n_ptr;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= n) return;  // whole block past the records
     const int lane = (int)(threadIdx.x & 63);
     const bool in = i < n;
     const uint32_t key = in ? keys[i] : 0xFFFFFFFFu;
@@ -424,19 +428,18 @@ __global__ __launch_bounds__(256) void comp_reduce_runs_kernel(const uint32_t* k
 // shift): start[s] is the first record of slot s, start[num_slots] = n. Thread
 // i looks at the boundary between records i - 1 and i and fills the starts of
 // every slot in between, so empty slots get the start of the next non-empty
-// one. Thread 0 also clears the distinct-voxel counter of the slot reduction.
-// The slot is read from the sorted bit range and clamped below num_slots, so
+// one. The slot is read from the sorted bit range and clamped below num_slots, so
 // even records that no sample wrote (sample counts that do not match the
 // forward's) cannot index past start[num_slots].
 // Thread t takes records 4t .. 4t + 3 (one 16-byte load; `keys` is 16-byte
 // aligned) and the boundary before each of them, up to record n.
-__global__ __launch_bounds__(256) void comp_slot_starts_kernel(const uint32_t* keys, uint32_t n, int shift,
-                                                               uint32_t slot_mask, uint32_t num_slots,
-                                                               uint32_t* start, uint32_t* distinct) {
+__global__ __launch_bounds__(256) void comp_slot_starts_kernel(const uint32_t* keys, const uint32_t* n_ptr,
+                                                               int shift, uint32_t slot_mask, uint32_t num_slots,
+                                                               uint32_t* start) {
+    const uint32_t n = *n_ptr;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t i0 = t * 4u;
     if (i0 > n) return;
-    if (t == 0) *distinct = 0u;
     auto slot_of = [&](uint32_t k) { return min((k >> shift) & slot_mask, num_slots - 1u); };
     uint32_t k4[4];
     if (i0 + 4u <= n) {
@@ -476,7 +479,7 @@ template <int VB>
 __global__ __launch_bounds__(256) void comp_reduce_slots_kernel(const uint32_t* keys, const uint32_t* ids,
                                                                 const float4* vals, const uint32_t* start,
                                                                 uint32_t num_slots, float4* grad,
-                                                                uint32_t* distinct) {
+                                                                uint32_t* distinct, const uint32_t* overflow) {
     constexpr int NV = 1 << VB;
     constexpr int NW = (NV + 31) / 32;  // touched-voxel bit words
     __shared__ float4 acc_all[4][NV];
@@ -531,11 +534,221 @@ __global__ __launch_bounds__(256) void comp_reduce_slots_kernel(const uint32_t* 
         }
     }
     float4* g = grad + (size_t)slot * NV;
+    if (*overflow) {  // records were dropped: no gradient rather than a wrong one
+        const float qn = __builtin_nanf("");
+        for (int v = lane; v < NV; v += 64) g[v] = make_float4(qn, qn, qn, qn);
+        return;
+    }
     for (int v = lane; v < NV; v += 64) g[v] = acc[v];
     uint32_t cnt = 0;
     for (int k = lane; k < NW; k += 64) cnt += (uint32_t)__builtin_popcount(seen[k]);
     for (int off = 32; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off);
     if (lane == 0 && cnt) atomicAdd(distinct, cnt);
+}
+
+// ---- scan and stable radix sort of the adjoint records ---------------------
+// Block-wide exclusive scan of one value per thread (256 threads); *total gets
+// the block's sum. Uses sh4[4]; ends with a barrier so it can be called again.
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh4, uint32_t& total) {
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    uint32_t inc = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t o = __shfl_up(inc, (unsigned)off);
+        if (lane >= off) inc += o;
+    }
+    if (lane == 63) sh4[w] = inc;
+    __syncthreads();
+    uint32_t before = 0, tot = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t x = sh4[k];
+        before += k < w ? x : 0u;
+        tot += x;
+    }
+    total = tot;
+    __syncthreads();
+    return before + inc - v;
+}
+
+constexpr int kScanThreads = 256;
+constexpr int kScanRounds = 16;
+constexpr uint32_t kScanChunk = kScanThreads * kScanRounds;  // items per block
+
+// partial[b] = sum of in[b * kScanChunk, ...) (coalesced rounds of 256 items).
+__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* in, uint32_t n,
+                                                                   uint32_t* partial) {
+    __shared__ uint32_t sh4[4];
+    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x;
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanRounds; ++k) {
+        const uint32_t i = base + (uint32_t)k * kScanThreads;
+        s += i < n ? in[i] : 0u;
+    }
+    uint32_t tot;
+    block_excl_scan(s, sh4, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// In-place exclusive scan of the np block sums (one block, chunks of 256 with
+// a carry); *total (if not null) receives the grand total.
+__global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* partial, uint32_t np,
+                                                                     uint32_t* total) {
+    __shared__ uint32_t sh4[4];
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < np; c += kScanThreads) {
+        const uint32_t i = c + threadIdx.x;
+        const uint32_t v = i < np ? partial[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, sh4, tot);
+        if (i < np) partial[i] = carry + ex;
+        carry += tot;
+    }
+    if (threadIdx.x == 0 && total) *total = carry;
+}
+
+// out[i] = partial[block] + exclusive prefix of in within the block's chunk
+// (out may alias in).
+__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* in, uint32_t n,
+                                                                  const uint32_t* partial, uint32_t* out) {
+    __shared__ uint32_t sh4[4];
+    uint32_t carry = partial[blockIdx.x];
+    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x;
+    for (int k = 0; k < kScanRounds; ++k) {
+        const uint32_t i = base + (uint32_t)k * kScanThreads;
+        const uint32_t v = i < n ? in[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_excl_scan(v, sh4, tot);
+        if (i < n) out[i] = carry + ex;
+        carry += tot;
+    }
+}
+
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = 16;
+constexpr uint32_t kSortTile = kSortThreads * kSortRounds;  // records per block
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+// Lanes of the wave holding the same digit (of `bits` bits) as this lane,
+// among the lanes with `in` set.
+__device__ __forceinline__ uint64_t digit_peers(bool in, uint32_t d, int bits) {
+    uint64_t peers = __ballot(in);
+    for (int b = 0; b < bits; ++b) {
+        const bool on = (d >> b) & 1u;
+        const uint64_t m = __ballot(on);
+        peers &= on ? m : ~m;
+    }
+    return peers;
+}
+
+// Digit histogram of block b's records [b * kSortTile, ...) below *n, stored
+// digit-major (hist[d * nb + b]) so that one exclusive scan gives every
+// (digit, block) its first output position. Blocks past the end store zeros.
+// Neighbouring records mostly share a digit (records in emission order share
+// brick slots), so each run of equal digits among a wave's lanes is counted
+// with one LDS add by its first lane; per-lane adds to one counter would
+// serialise.
+__global__ __launch_bounds__(kSortThreads) void rs_upsweep_kernel(const uint32_t* keys, const uint32_t* n_ptr,
+                                                                  int shift, int bits, uint32_t* hist,
+                                                                  uint32_t nb) {
+    __shared__ uint32_t h[kRadix];
+    for (int d = (int)threadIdx.x; d < kRadix; d += kSortThreads) h[d] = 0u;
+    __syncthreads();
+    const uint32_t n = *n_ptr;
+    const uint32_t mask = (1u << bits) - 1u;
+    const uint32_t base = blockIdx.x * kSortTile + threadIdx.x;
+    const int lane = (int)(threadIdx.x & 63);
+    const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
+    if (blockIdx.x * kSortTile < n) {
+        for (int k = 0; k < kSortRounds; ++k) {
+            const uint32_t i = base + (uint32_t)k * kSortThreads;
+            const bool in = i < n;  // the lanes in range are a prefix of the wave
+            const uint32_t d = in ? (keys[i] >> shift) & mask : 0u;
+            const uint32_t prev = __shfl_up(d, 1u);
+            const bool head = in && (lane == 0 || prev != d);
+            const uint64_t heads = __ballot(head);
+            const int n_in = __popcll(__ballot(in));
+            if (head) {
+                const uint64_t nxt = heads & above;
+                const int end = nxt ? __ffsll((unsigned long long)nxt) - 1 : n_in;
+                atomicAdd(&h[d], (uint32_t)(end - lane));
+            }
+        }
+    }
+    __syncthreads();
+    for (int d = (int)threadIdx.x; d < kRadix; d += kSortThreads) hist[(size_t)d * nb + blockIdx.x] = h[d];
+}
+
+// Stable scatter of block b's records by digit. `offs` is the scanned
+// histogram. Rounds take 256 consecutive records; a record's place among the
+// block's records of its digit is the count from earlier rounds, from earlier
+// waves of its round, and from earlier lanes of its wave (ballot rank), so
+// equal digits keep their record order. ids_in == null: ids are positions.
+__global__ __launch_bounds__(kSortThreads) void rs_downsweep_kernel(const uint32_t* keys_in, const uint32_t* ids_in,
+                                                                    uint32_t* keys_out, uint32_t* ids_out,
+                                                                    const uint32_t* n_ptr, int shift, int bits,
+                                                                    const uint32_t* offs, uint32_t nb) {
+    __shared__ uint32_t run[kRadix];
+    __shared__ uint32_t wc_all[2][4][kRadix];  // per round parity, per wave
+    const uint32_t n = *n_ptr;
+    if (blockIdx.x * kSortTile >= n) return;  // whole block
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const uint32_t mask = (1u << bits) - 1u;
+    for (int d = (int)threadIdx.x; d < kRadix; d += kSortThreads) {
+        run[d] = offs[(size_t)d * nb + blockIdx.x];
+        for (int q = 0; q < 4; ++q) wc_all[0][q][d] = wc_all[1][q][d] = 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t base = blockIdx.x * kSortTile + threadIdx.x;
+    for (int k = 0; k < kSortRounds; ++k) {
+        const uint32_t i = base + (uint32_t)k * kSortThreads;
+        const bool in = i < n;
+        uint32_t key = 0, id = 0, d = 0;
+        if (in) {
+            key = keys_in[i];
+            id = ids_in ? ids_in[i] : i;
+            d = (key >> shift) & mask;
+        }
+        // the per-wave counts alternate between two buffers by round, so the
+        // next round's counts never meet this round's clearing
+        uint32_t (*wc)[kRadix] = wc_all[k & 1];
+        const uint64_t peers = digit_peers(in, d, bits);
+        if (in && (peers & below) == 0) wc[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (in) {
+            uint32_t pos = run[d] + (uint32_t)__popcll(peers & below);
+            for (int q = 0; q < w; ++q) pos += wc[q][d];
+            keys_out[pos] = key;
+            ids_out[pos] = id;
+        }
+        __syncthreads();
+        for (int dd = (int)threadIdx.x; dd < kRadix; dd += kSortThreads) {
+            run[dd] += wc[0][dd] + wc[1][dd] + wc[2][dd] + wc[3][dd];
+            wc[0][dd] = wc[1][dd] = wc[2][dd] = wc[3][dd] = 0u;
+        }
+    }
+}
+
+// Record total against the workspace capacity: n = min(total, cap), the
+// overflow flag, and the distinct-voxel counter cleared.
+__global__ void comp_check_kernel(uint32_t* misc, uint32_t cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t total = misc[kMiscTotal];
+    misc[kMiscN] = total <= cap ? total : cap;
+    misc[kMiscOverflow] = total > cap ? 1u : 0u;
+    misc[kMiscDistinct] = 0u;
+}
+
+// After an overflow, the whole-key path's gradient is NaN (the per-slot path
+// writes NaN bricks itself).
+__global__ void comp_poison_kernel(const uint32_t* overflow, float4* g, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && *overflow) {
+        const float qn = __builtin_nanf("");
+        g[i] = make_float4(qn, qn, qn, qn);
+    }
 }
 
 }  // namespace yx
@@ -551,12 +764,15 @@ using namespace yx;
 struct yx_compositor {
     void* ws = nullptr;  // one device allocation, carved into the buffers below
     size_t ws_bytes = 0;
-    uint64_t last_records = 0;
+    uint64_t cap = 0;           // records the workspace holds
+    bool reserved = false;      // cap set by yx_compositor_reserve
+    uint64_t sig[8] = {};       // geometry of the call cap was sized for
+    uint32_t* misc = nullptr;   // kMiscWords device words (inside ws)
+    uint32_t* h_misc = nullptr; // pinned copy of misc after each backward
+    uint32_t* d_probe = nullptr;  // sizing: the record total of a new geometry
     const uint32_t* sorted_keys = nullptr;  // the last backward's sorted keys (workspace)
-    uint32_t* counter = nullptr;
-    bool counted = false;  // the last backward's slot reduction left the distinct count in `counter`
-    uint32_t* d_iota = nullptr;  // record ids 0..iota_n-1, written once per size
-    size_t iota_n = 0;
+    bool counted = false;  // the last backward's slot reduction counted its distinct voxels
+    bool ran = false;      // a backward has run since the workspace was laid out
 };
 
 static int check_vol(const yx_volume_dev* v, int32_t n_views, int32_t w, int32_t h) {
@@ -611,29 +827,45 @@ extern "C" int yx_compositor_create(yx_compositor** out) {
 extern "C" void yx_compositor_destroy(yx_compositor* c) {
     if (!c) return;
     if (c->ws) (void)hipFree(c->ws);
-    if (c->d_iota) (void)hipFree(c->d_iota);
+    if (c->h_misc) (void)hipHostFree(c->h_misc);
+    if (c->d_probe) (void)hipFree(c->d_probe);
     delete c;
+}
+
+extern "C" int yx_compositor_reserve(yx_compositor* c, uint64_t records) {
+    clear_error();
+    if (!c) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (records >= 0x7FFFFFFFull) return fail(YX_ERR_RANGE, "%llu records exceed 2^31", (unsigned long long)records);
+    if (records > c->cap) c->cap = records;
+    c->reserved = true;
+    return YX_OK;
 }
 
 extern "C" int yx_compositor_last_stats(const yx_compositor* c, uint64_t* records, uint64_t* unique_keys) {
     clear_error();
     if (!c || !records || !unique_keys) return fail(YX_ERR_INVALID_ARG, "null argument");
-    *records = c->last_records;
+    *records = 0;
     *unique_keys = 0;
+    if (!c->ran || !c->misc) return YX_OK;
     // the backward ran on the caller's stream, which may not synchronise with
     // the null stream: wait for the device before reading its results
-    if (c->last_records) YX_HIP(hipDeviceSynchronize());
-    if (c->counted && c->last_records) {
-        uint32_t runs = 0;
-        YX_HIP(hipMemcpy(&runs, c->counter, 4, hipMemcpyDeviceToHost));
-        *unique_keys = runs;
-    } else if (c->sorted_keys && c->last_records) {
-        YX_HIP(hipMemset(c->counter, 0, 4));
-        const uint32_t n = (uint32_t)c->last_records;
-        hipLaunchKernelGGL(comp_count_runs_kernel, dim3(1024), dim3(256), 0, nullptr, c->sorted_keys, n, c->counter);
+    YX_HIP(hipDeviceSynchronize());
+    uint32_t m[kMiscWords];
+    YX_HIP(hipMemcpy(m, c->misc, sizeof(m), hipMemcpyDeviceToHost));
+    if (m[kMiscOverflow])
+        return fail(YX_ERR_RANGE, "the last backward had %u records, beyond the workspace's %llu: its gradient is NaN",
+                    m[kMiscTotal], (unsigned long long)c->cap);
+    *records = m[kMiscTotal];
+    if (c->counted) {
+        *unique_keys = m[kMiscDistinct];
+    } else if (c->sorted_keys && m[kMiscN]) {
+        uint32_t* cnt = c->misc + kMiscDistinct;
+        YX_HIP(hipMemset(cnt, 0, 4));
+        hipLaunchKernelGGL(comp_count_runs_kernel, dim3(1024), dim3(256), 0, nullptr, c->sorted_keys,
+                           c->misc + kMiscN, cnt);
         YX_HIP(hipGetLastError());
         uint32_t runs = 0;
-        YX_HIP(hipMemcpy(&runs, c->counter, 4, hipMemcpyDeviceToHost));
+        YX_HIP(hipMemcpy(&runs, cnt, 4, hipMemcpyDeviceToHost));
         *unique_keys = runs;
     }
     return YX_OK;
@@ -661,6 +893,52 @@ extern "C" int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, 
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Exclusive scan of n words (out may alias in) on `st`; partial holds
+// ceil(n / kScanChunk) words; *total (device, may be null) gets the sum.
+static int scan_u32(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* partial, uint32_t* total,
+                    hipStream_t st) {
+    const uint32_t nb = (n + kScanChunk - 1) / kScanChunk;
+    if (nb) hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, st, in, n, partial);
+    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, partial, nb, total);
+    if (nb) hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, st, in, n, partial, out);
+    YX_HIP(hipGetLastError());
+    return YX_OK;
+}
+
+// The backward's workspace for `cap` records, n_waves ray tiles and
+// num_slots brick slots (byte offsets, 256-aligned).
+struct CompLayout {
+    size_t misc, sums, offs, partial, keys, vals, kA, iA, kB, iB, hist, starts, bytes;
+    uint32_t nb;  // sort blocks
+};
+
+static CompLayout comp_layout(uint64_t cap, size_t n_waves, uint32_t num_slots) {
+    CompLayout L;
+    L.nb = (uint32_t)std::max<uint64_t>(1, (cap + kSortTile - 1) / kSortTile);
+    const size_t hist = (size_t)kRadix * L.nb;
+    const size_t partial = std::max((n_waves + kScanChunk - 1) / kScanChunk, (hist + kScanChunk - 1) / kScanChunk) + 1;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o += align256(bytes);
+        return at;
+    };
+    L.misc = take(kMiscWords * 4);
+    L.sums = take(n_waves * 4);
+    L.offs = take(n_waves * 4);
+    L.partial = take(partial * 4);
+    L.keys = take(cap * 4);
+    L.vals = take(cap * 16);
+    L.kA = take(cap * 4);
+    L.iA = take(cap * 4);
+    L.kB = take(cap * 4);
+    L.iB = take(cap * 4);
+    L.hist = take(hist * 4);
+    L.starts = take(((size_t)num_slots + 1) * 4);
+    L.bytes = o;
+    return L;
+}
+
 extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol, const float* views,
                                      int32_t n_views, int32_t w, int32_t h, const float* rgb, const uint32_t* nsamp,
                                      const float* grad_rgb, float* grad_params, void* stream) {
@@ -678,171 +956,199 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     c->sorted_keys = nullptr;
     c->counted = false;
 
-    // 1. per-wave sample counts (8x8 ray tiles), exclusive scan -> each
-    //    wave's record offset, and the total
     const size_t n_waves = (size_t)n_views * ((w + 7) / 8) * ((h + 7) / 8);
     CompArgs a = make_args(vol, views, n_views, w, h);
     a.nsamp_in = nsamp;
     a.n_waves = (uint32_t)n_waves;
-    size_t scan_tmp = 0;
-    YX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                            (int)n_waves, st));
-    const size_t wb = align256(n_waves * sizeof(uint32_t));
-    const size_t off_bytes = 2 * wb;
-    const size_t need0 = off_bytes + align256(scan_tmp) + 256;
-    if (c->ws_bytes < need0) {
+    const dim3 wgrid((unsigned)((n_waves * 64 + 255) / 256)), wblock(256);
+
+    // 0. capacity. A previous call whose records overflowed (read from its
+    //    pinned copy of the device words; a copy still in flight reads as no
+    //    overflow) grows it to that call's total. A call on a new geometry
+    //    (table, views, sizes) is sized with one host wait, unless the caller
+    //    reserved a capacity; the same geometry always has the same samples.
+    if (c->h_misc && c->h_misc[kMiscOverflow]) {
+        const uint64_t t = c->h_misc[kMiscTotal];
+        c->cap = std::max<uint64_t>(c->cap, t + t / 4);
+        c->h_misc[kMiscOverflow] = 0;
+    }
+    const uint64_t sig[8] = {(uint64_t)(uintptr_t)vol->table, (uint64_t)(uintptr_t)views, (uint64_t)n_views,
+                             (uint64_t)w, (uint64_t)h, (uint64_t)vol->grid, (uint64_t)vol->brick,
+                             (uint64_t)vol->num_slots};
+    if (!c->reserved && (c->cap == 0 || std::memcmp(sig, c->sig, sizeof(sig)) != 0)) {
+        const size_t pw = align256(n_waves * 4) / 4;
+        const size_t words = pw + (n_waves + kScanChunk - 1) / kScanChunk + 2;
+        if (c->d_probe) YX_HIP(hipFree(c->d_probe));
+        c->d_probe = nullptr;
+        hipError_t e = hipMalloc((void**)&c->d_probe, words * 4);
+        if (e != hipSuccess) {
+            c->d_probe = nullptr;
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) for sizing: %s", words * 4, hipGetErrorString(e));
+        }
+        uint32_t* sums = c->d_probe;
+        uint32_t* part = c->d_probe + pw;
+        uint32_t* tot = part + (n_waves + kScanChunk - 1) / kScanChunk + 1;
+        hipLaunchKernelGGL(comp_wave_sums_kernel, wgrid, wblock, 0, st, a, sums, (uint32_t)n_waves);
+        rc = scan_u32(sums, (uint32_t)n_waves, sums, part, tot, st);
+        if (rc != YX_OK) return rc;
+        uint32_t total = 0;
+        YX_HIP(hipMemcpyAsync(&total, tot, 4, hipMemcpyDeviceToHost, st));
+        YX_HIP(hipStreamSynchronize(st));
+        c->cap = std::max<uint64_t>(c->cap, (uint64_t)total + total / 4 + 64);
+        std::memcpy(c->sig, sig, sizeof(sig));
+    }
+    if (c->cap >= 0x7FFFFFFFull) return fail(YX_ERR_RANGE, "%llu records exceed 2^31", (unsigned long long)c->cap);
+    const uint32_t cap = (uint32_t)std::max<uint64_t>(c->cap, 1);
+    const CompLayout L = comp_layout(cap, n_waves, vol->num_slots);
+    if (c->ws_bytes < L.bytes) {
         if (c->ws) YX_HIP(hipFree(c->ws));
         c->ws = nullptr;
         c->ws_bytes = 0;
-        hipError_t e = hipMalloc(&c->ws, need0);
-        if (e != hipSuccess) return fail(YX_ERR_OOM, "hipMalloc(%zu) workspace: %s", need0, hipGetErrorString(e));
-        c->ws_bytes = need0;
+        c->misc = nullptr;
+        c->ran = false;
+        hipError_t e = hipMalloc(&c->ws, L.bytes);
+        if (e != hipSuccess) {
+            c->ws = nullptr;
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) workspace: %s", L.bytes, hipGetErrorString(e));
+        }
+        c->ws_bytes = L.bytes;
     }
-    auto wave_offsets = [&]() -> int {
-        uint32_t* sums = (uint32_t*)c->ws;
-        uint32_t* offs = (uint32_t*)((char*)c->ws + wb);
-        hipLaunchKernelGGL(comp_wave_sums_kernel, dim3((unsigned)((n_waves * 64 + 255) / 256)), dim3(256), 0, st, a,
-                           sums, (uint32_t)n_waves);
-        YX_HIP(hipGetLastError());
-        YX_HIP(hipcub::DeviceScan::ExclusiveSum((char*)c->ws + off_bytes, scan_tmp, sums, offs, (int)n_waves, st));
-        return YX_OK;
-    };
-    rc = wave_offsets();
+    if (!c->h_misc) {
+        hipError_t e = hipHostMalloc((void**)&c->h_misc, kMiscWords * 4, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            c->h_misc = nullptr;
+            return fail(YX_ERR_OOM, "hipHostMalloc(%d): %s", kMiscWords * 4, hipGetErrorString(e));
+        }
+        std::memset(c->h_misc, 0, kMiscWords * 4);
+    }
+    char* base = (char*)c->ws;
+    uint32_t* misc = (uint32_t*)(base + L.misc);
+    c->misc = misc;
+    uint32_t* sums = (uint32_t*)(base + L.sums);
+    uint32_t* offs = (uint32_t*)(base + L.offs);
+    uint32_t* partial = (uint32_t*)(base + L.partial);
+    uint32_t* keys = (uint32_t*)(base + L.keys);
+    float4* vals = (float4*)(base + L.vals);
+    uint32_t* kbuf[2] = {(uint32_t*)(base + L.kA), (uint32_t*)(base + L.kB)};
+    uint32_t* ibuf[2] = {(uint32_t*)(base + L.iA), (uint32_t*)(base + L.iB)};
+    uint32_t* hist = (uint32_t*)(base + L.hist);
+    uint32_t* starts = (uint32_t*)(base + L.starts);
+    uint32_t* n_dev = misc + kMiscN;
+
+    // 1. per-wave sample counts (8x8 ray tiles), exclusive scan -> each wave's
+    //    record offset; the total against the capacity
+    hipLaunchKernelGGL(comp_wave_sums_kernel, wgrid, wblock, 0, st, a, sums, (uint32_t)n_waves);
+    YX_HIP(hipGetLastError());
+    rc = scan_u32(sums, (uint32_t)n_waves, offs, partial, misc + kMiscTotal, st);
     if (rc != YX_OK) return rc;
-    uint32_t* offsets = (uint32_t*)((char*)c->ws + wb);
-    uint32_t last[2];
-    YX_HIP(hipMemcpyAsync(&last[0], offsets + n_waves - 1, 4, hipMemcpyDeviceToHost, st));
-    YX_HIP(hipMemcpyAsync(&last[1], (uint32_t*)c->ws + n_waves - 1, 4, hipMemcpyDeviceToHost, st));
-    YX_HIP(hipStreamSynchronize(st));
-    const uint64_t total = (uint64_t)last[0] + last[1];
-    if (total >= 0x7FFFFFFFull) return fail(YX_ERR_RANGE, "%llu records exceed 2^31", (unsigned long long)total);
-    c->last_records = total;
+    hipLaunchKernelGGL(comp_check_kernel, dim3(1), dim3(64), 0, st, misc, cap);
+    YX_HIP(hipGetLastError());
 
     // Bricks of up to 8^3 voxels are reduced per slot (records sorted by the
     // slot bits of the key only); larger bricks sort the whole key and sum
     // each key's run.
     const int vox_bits = 3 * __builtin_ctz((unsigned)vol->brick);
     const bool by_slot = vox_bits <= 9;
-    // zero the gradient (voxels without samples keep 0); the slot reduction
-    // writes every voxel itself
-    if (!by_slot || total == 0) {
-        const size_t n4 = nvox;
-        if (n4) hipLaunchKernelGGL(comp_zero_kernel, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st,
-                                   (float4*)grad_params, n4);
+    if (!by_slot && nvox) {  // voxels without samples keep 0 (the slot reduction writes every voxel)
+        hipLaunchKernelGGL(comp_zero_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, st,
+                           (float4*)grad_params, nvox);
         YX_HIP(hipGetLastError());
     }
-    if (total == 0) return YX_OK;
-
-    // 2. workspace: keys, record ids, their sorted copies, values, slot
-    //    starts, library temporaries. The sort moves (key, id) pairs only; the
-    //    reduction gathers each value once through the sorted ids.
     int key_bits = 1;
     while (key_bits < 32 && ((uint64_t)1 << key_bits) < (uint64_t)nvox) ++key_bits;
     const int begin_bit = by_slot ? vox_bits : 0;
     const int end_bit = std::max(key_bits, begin_bit + 1);
-    size_t sort_tmp = 0;
-    YX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, sort_tmp, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                              (uint32_t*)nullptr, (uint32_t*)nullptr, (int)total, begin_bit, end_bit,
-                                              st));
-    const size_t kb = align256(total * 4), vb = align256(total * 16);
-    const size_t sb = by_slot ? align256(((size_t)vol->num_slots + 1) * 4) : 0;
-    const size_t need = off_bytes + 3 * kb + vb + sb + 256 + align256(std::max(sort_tmp, scan_tmp));
-    if (c->iota_n < total) {
-        if (c->d_iota) YX_HIP(hipFree(c->d_iota));
-        c->d_iota = nullptr;
-        c->iota_n = 0;
-        const size_t cap = total + total / 4;
-        hipError_t e = hipMalloc((void**)&c->d_iota, cap * 4);
-        if (e != hipSuccess) {
-            c->d_iota = nullptr;
-            return fail(YX_ERR_OOM, "hipMalloc(%zu) record ids: %s", cap * 4, hipGetErrorString(e));
-        }
-        hipLaunchKernelGGL(comp_iota_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, c->d_iota, cap);
-        YX_HIP(hipGetLastError());
-        c->iota_n = cap;
-    }
-    if (c->ws_bytes < need) {
-        // grow, then recompute the offsets into the new block
-        if (c->ws) YX_HIP(hipFree(c->ws));
-        c->ws = nullptr;
-        c->ws_bytes = 0;
-        const size_t grow = need + need / 4;
-        hipError_t e = hipMalloc(&c->ws, grow);
-        if (e != hipSuccess) return fail(YX_ERR_OOM, "hipMalloc(%zu) workspace: %s", grow, hipGetErrorString(e));
-        c->ws_bytes = grow;
-        rc = wave_offsets();
-        if (rc != YX_OK) return rc;
-        offsets = (uint32_t*)((char*)c->ws + wb);
-    }
-    char* p = (char*)c->ws + off_bytes;
-    uint32_t* keys = (uint32_t*)p;
-    p += kb;
-    uint32_t* keys_sorted = (uint32_t*)p;
-    p += kb;
-    uint32_t* ids_sorted = (uint32_t*)p;
-    p += kb;
-    float4* vals = (float4*)p;
-    p += vb;
-    uint32_t* starts = (uint32_t*)p;
-    p += sb;
-    uint32_t* counter = (uint32_t*)p;
-    p += 256;
-    void* tmp = p;
 
-    // 3. emit records (one wave = one 8x8 ray tile)
+    // 2. emit records (one wave = one 8x8 ray tile) at their offsets
     a.rgb_in = rgb;
     a.grad_rgb = grad_rgb;
-    a.offsets = offsets;
-    a.total_records = (uint32_t)total;
+    a.offsets = offs;
+    a.total_records = cap;
     a.keys = keys;
     a.vals = vals;
-    const dim3 bgrid((unsigned)((n_waves * 64 + 255) / 256)), bblock(256);
     if (a.g <= 1024)
-        hipLaunchKernelGGL(comp_backward_kernel<true>, bgrid, bblock, 0, st, a);
+        hipLaunchKernelGGL(comp_backward_kernel<true>, wgrid, wblock, 0, st, a);
     else
-        hipLaunchKernelGGL(comp_backward_kernel<false>, bgrid, bblock, 0, st, a);
+        hipLaunchKernelGGL(comp_backward_kernel<false>, wgrid, wblock, 0, st, a);
     YX_HIP(hipGetLastError());
 
-    // 4. sort (key, id) by brick slot, or by the whole voxel key (LSD radix,
-    //    stable: records stay in emission order within a slot or key),
-    // 5. sum each voxel's records in that order
-    YX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp, sort_tmp, keys, keys_sorted, (const uint32_t*)c->d_iota, ids_sorted,
-                                              (int)total, begin_bit, end_bit, st));
-    c->counter = counter;
+    // 3. record ids sorted by the slot bits (or the whole key), stable: LSD
+    //    passes of 8-bit digits, each a digit histogram per block, a scan of
+    //    the histograms and a scatter.
+    auto radix_sort = [&](const uint32_t* keys_in, const uint32_t* n_ptr, int b0, int b1, const uint32_t** kout,
+                          const uint32_t** iout) -> int {
+        const uint32_t* kin = keys_in;
+        const uint32_t* iin = nullptr;  // the first pass's ids are the positions
+        int pass = 0;
+        for (int sh = b0; sh < b1; sh += kRadixBits, ++pass) {
+            const int bits = std::min(kRadixBits, b1 - sh);
+            hipLaunchKernelGGL(rs_upsweep_kernel, dim3(L.nb), dim3(kSortThreads), 0, st, kin, n_ptr, sh, bits, hist,
+                               L.nb);
+            YX_HIP(hipGetLastError());
+            const int e = scan_u32(hist, (uint32_t)((size_t)kRadix * L.nb), hist, partial, nullptr, st);
+            if (e != YX_OK) return e;
+            hipLaunchKernelGGL(rs_downsweep_kernel, dim3(L.nb), dim3(kSortThreads), 0, st, kin, iin,
+                               kbuf[pass & 1], ibuf[pass & 1], n_ptr, sh, bits, hist, L.nb);
+            YX_HIP(hipGetLastError());
+            kin = kbuf[pass & 1];
+            iin = ibuf[pass & 1];
+        }
+        *kout = kin;
+        *iout = iin;
+        return YX_OK;
+    };
+
+    // 4. sum each voxel's records in emission order
+    const uint32_t* keys_sorted = nullptr;
+    const uint32_t* ids_sorted = nullptr;
+    rc = radix_sort(keys, n_dev, begin_bit, end_bit, &keys_sorted, &ids_sorted);
+    if (rc != YX_OK) return rc;
     if (by_slot) {
         const uint32_t ns = vol->num_slots;
         const uint32_t slot_mask = (uint32_t)(((uint64_t)1 << (end_bit - begin_bit)) - 1u);
-        const size_t threads = total / 4 + 1;  // thread t covers records 4t .. 4t + 3, up to record `total`
+        const size_t threads = (size_t)cap / 4 + 1;  // thread t covers records 4t .. 4t + 3, up to record n
         hipLaunchKernelGGL(comp_slot_starts_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st,
-                           keys_sorted, (uint32_t)total, vox_bits, slot_mask, ns, starts, counter);
+                           keys_sorted, n_dev, vox_bits, slot_mask, ns, starts);
         YX_HIP(hipGetLastError());
-        const dim3 grid((unsigned)((ns + 3) / 4)), block(256);
-        float4* g = (float4*)grad_params;
-        switch (vox_bits) {
-            case 0:
-                hipLaunchKernelGGL(comp_reduce_slots_kernel<0>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
-                                   starts, ns, g, counter);
-                break;
-            case 3:
-                hipLaunchKernelGGL(comp_reduce_slots_kernel<3>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
-                                   starts, ns, g, counter);
-                break;
-            case 6:
-                hipLaunchKernelGGL(comp_reduce_slots_kernel<6>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
-                                   starts, ns, g, counter);
-                break;
-            default:
-                hipLaunchKernelGGL(comp_reduce_slots_kernel<9>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
-                                   starts, ns, g, counter);
-                break;
+        if (ns) {
+            const dim3 grid((unsigned)((ns + 3) / 4)), block(256);
+            float4* g = (float4*)grad_params;
+            uint32_t* distinct = misc + kMiscDistinct;
+            const uint32_t* over = misc + kMiscOverflow;
+            switch (vox_bits) {
+                case 0:
+                    hipLaunchKernelGGL(comp_reduce_slots_kernel<0>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
+                                       starts, ns, g, distinct, over);
+                    break;
+                case 3:
+                    hipLaunchKernelGGL(comp_reduce_slots_kernel<3>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
+                                       starts, ns, g, distinct, over);
+                    break;
+                case 6:
+                    hipLaunchKernelGGL(comp_reduce_slots_kernel<6>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
+                                       starts, ns, g, distinct, over);
+                    break;
+                default:
+                    hipLaunchKernelGGL(comp_reduce_slots_kernel<9>, grid, block, 0, st, keys_sorted, ids_sorted, vals,
+                                       starts, ns, g, distinct, over);
+                    break;
+            }
+            YX_HIP(hipGetLastError());
         }
-        YX_HIP(hipGetLastError());
         c->counted = true;
     } else {
-        hipLaunchKernelGGL(comp_reduce_runs_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st,
-                           keys_sorted, ids_sorted, vals, (uint32_t)total, (float4*)grad_params, (uint32_t)nvox);
+        hipLaunchKernelGGL(comp_reduce_runs_kernel, dim3((unsigned)(((size_t)cap + 255) / 256)), dim3(256), 0, st,
+                           keys_sorted, ids_sorted, vals, n_dev, (float4*)grad_params, (uint32_t)nvox);
+        YX_HIP(hipGetLastError());
+        if (nvox)
+            hipLaunchKernelGGL(comp_poison_kernel, dim3((unsigned)((nvox + 255) / 256)), dim3(256), 0, st,
+                               misc + kMiscOverflow, (float4*)grad_params, nvox);
         YX_HIP(hipGetLastError());
         c->sorted_keys = keys_sorted;
     }
+    // 5. the device words to pinned host memory: the next call sees an
+    //    overflow without waiting for this one
+    YX_HIP(hipMemcpyAsync(c->h_misc, misc, kMiscWords * 4, hipMemcpyDeviceToHost, st));
+    c->ran = true;
     return YX_OK;
 }
