@@ -221,6 +221,24 @@ int yx_renderer_set_resolution(yx_renderer* r, int32_t width, int32_t height);
 int yx_renderer_get_resolution(const yx_renderer* r, int32_t* width, int32_t* height);
 int yx_renderer_set_fov(yx_renderer* r, float fov_deg); /* horizontal, default 70 */
 int yx_renderer_set_light(yx_renderer* r, const float dir[3], int32_t mode);
+/* How leaf payloads (VoxData) are read when shading yx_render /
+ * yx_render_frame / yx_render_noalloc frames:
+ *   YX_PAYLOAD_RGBA8 (default): RGBA8 colour (r in the low byte), Lambert over
+ *     the entry face of the hit cell;
+ *   YX_PAYLOAD_C16N16: the reference's colour16 + normal16 (VoxData,
+ *     reaction/report/main.tex:38,65; UnpackVoxData / UnpackColor /
+ *     UnpackNormal, demo/SVORenderer.cpp:181-185, bodies missing). Layout
+ *     defined here: bits 0-15 RGB565 (red in 11-15, widened by bit
+ *     replication); bits 16-31 a normal quantized after Baptista, "Higher
+ *     Accuracy Quantized Normals" (the thesis's citation): bits 15/14/13 of the
+ *     half the signs of x/y/z, bits 7-12 u and 0-6 v of the triangle
+ *     u + v <= 126 (points with u >= 64 folded to 127 - u, 127 - v); normal =
+ *     normalize(u, v, 126 - u - v) with the signs. Lambert over that normal;
+ *     shadow rays as for RGBA8.
+ * The shading pipeline (yx_render_pipeline) and batches read RGBA8. */
+#define YX_PAYLOAD_RGBA8 0
+#define YX_PAYLOAD_C16N16 1
+int yx_renderer_set_payload(yx_renderer* r, int32_t format);
 /* Ray directions of the current camera (RendererBase::InitRayDir,
  * cell/renderer_base.h:50-61): dir0, du, dv as 9 floats. */
 int yx_renderer_ray_basis(const yx_renderer* r, float basis[9]);

@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, as the checker / baseline. Never a product path.
 """
 import ctypes
+import os
 import pathlib
 import subprocess
 import threading
@@ -11,7 +12,9 @@ import threading
 import numpy as np
 
 HERE = pathlib.Path(__file__).resolve().parent
-LIB = HERE / "liboracle.so"
+# YORACLE_LIB names another build of the restatement (the sanitized build of
+# tests/sanitize/run.py); the default is the in-tree one.
+LIB = pathlib.Path(os.environ["YORACLE_LIB"]) if os.environ.get("YORACLE_LIB") else HERE / "liboracle.so"
 
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -39,13 +42,20 @@ class Oracle:
                                          ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int, _f32p, _u32p, _i32p, _f32p, _u32p, _u8p, _u32p]
         l.oracle_render_rows.restype = ctypes.c_longlong
+        l.oracle_render_rows_fmt.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, _f32p, _f32p,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, _f32p, ctypes.c_int, _u32p, _i32p, _f32p, _u32p, _u8p,
+                                             _u32p]
+        l.oracle_render_rows_fmt.restype = ctypes.c_longlong
+        l.oracle_decode_c16n16.argtypes = [ctypes.c_uint32, _u32p, _f32p]
+        l.oracle_decode_c16n16.restype = None
         l.oracle_trace_rays.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _f32p,
                                         _f32p, ctypes.c_int, _u32p, _i32p, _f32p, _i32p, _f32p, _u32p]
         l.oracle_trace_rays.restype = ctypes.c_int
         l.oracle_pixel_dirs.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, _f32p]
         l.oracle_pixel_dirs.restype = None
         l.oracle_shade_rays.argtypes = [_u32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _f32p, _f32p,
-                                        ctypes.c_int, _f32p, _u32p, _u32p, _u8p, _f32p, _u32p]
+                                        ctypes.c_int, _f32p, ctypes.c_int, _u32p, _u32p, _u8p, _f32p, _u32p]
         l.oracle_shade_rays.restype = ctypes.c_int
         l.oracle_plant.restype = ctypes.c_int
         self.lib = l
@@ -69,10 +79,11 @@ class Oracle:
         return out
 
     def render(self, nodes, root, pos, direction, up=(0, 0, 1), fov=70.0, w=64, h=64, light_mode=0,
-               light=(0, 0, 1), rows=None, threads=1):
+               light=(0, 0, 1), rows=None, threads=1, payload=0):
         """RenderRect over rows (default all) into full-frame arrays. `threads`
         splits the rows into bands, as TreadedRenderer (cell/ppu_renderer.cpp:
-        121-144) does. Returns a dict of outputs plus 'rays' traced."""
+        121-144) does. payload 1 reads leaves as colour16 + normal16. Returns a
+        dict of outputs plus 'rays' traced."""
         nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
         nn = nodes.size // 10
         basis = self.init_ray_dir(direction, up, fov, w, h)
@@ -95,9 +106,9 @@ class Oracle:
 
         def run(k):
             a, b = bands[k]
-            rays[k] = self.lib.oracle_render_rows(
+            rays[k] = self.lib.oracle_render_rows_fmt(
                 self._p(nodes, _u32p), nn, int(root), self._p(p, _f32p), self._p(basis, _f32p), w, h, a, b,
-                int(light_mode), self._p(lt, _f32p), self._p(out["hit_id"], _u32p),
+                int(light_mode), self._p(lt, _f32p), int(payload), self._p(out["hit_id"], _u32p),
                 self._p(out["voxel"], _i32p), self._p(out["t"], _f32p), self._p(out["rgba"], _u32p),
                 self._p(out["shadow"], _u8p), self._p(out["visits"], _u32p))
 
@@ -135,7 +146,14 @@ class Oracle:
             raise ValueError("oracle followed a child id out of range")
         return {"hit_id": hit, "voxel": vox, "t": t, "level": lvl, "t1": t1, "visits": vis}
 
-    def shade_rays(self, nodes, root, pos, dirs, light_mode=0, light=(0, 0, 1)):
+    def decode_c16n16(self, leaf):
+        """A colour16 + normal16 payload: (RGBA8, unit normal float32[3])."""
+        rgba = np.zeros(1, np.uint32)
+        n = np.zeros(3, np.float32)
+        self.lib.oracle_decode_c16n16(int(leaf) & 0xFFFFFFFF, self._p(rgba, _u32p), self._p(n, _f32p))
+        return int(rgba[0]), n
+
+    def shade_rays(self, nodes, root, pos, dirs, light_mode=0, light=(0, 0, 1), payload=0):
         """RenderRect's per-pixel work (trace, Lambert, shadow ray) for explicit
         rays: dict of hit_id, rgba, shadow, t, visits."""
         nodes = np.ascontiguousarray(nodes, np.uint32).reshape(-1)
@@ -150,7 +168,7 @@ class Oracle:
                "visits": np.zeros(n, np.uint32)}
         rc = self.lib.oracle_shade_rays(self._p(nodes, _u32p), nodes.size // 10, int(root), n,
                                         self._p(pos, _f32p), self._p(dirs, _f32p), int(light_mode),
-                                        self._p(lt, _f32p), self._p(out["hit_id"], _u32p),
+                                        self._p(lt, _f32p), int(payload), self._p(out["hit_id"], _u32p),
                                         self._p(out["rgba"], _u32p), self._p(out["shadow"], _u8p),
                                         self._p(out["t"], _f32p), self._p(out["visits"], _u32p))
         if rc < 0:

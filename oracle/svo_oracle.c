@@ -336,8 +336,40 @@ typedef struct {
     uint32_t visits;
 } pixel_out;
 
+/* The reference's VoxData as colour16 + normal16 (reaction/report/main.tex:
+ * 38,65; the unpacking bodies of demo/SVORenderer.cpp:181-185 are missing).
+ * DEFINED HERE: bits 0-15 RGB565 (red in 11-15), widened by bit replication;
+ * bits 16-31 a normal after Baptista, "Higher Accuracy Quantized Normals":
+ * bits 15/14/13 the signs of x/y/z, bits 7-12 u and 0-6 v on the triangle
+ * u + v <= 126, codes with u + v > 126 folded from (127 - u, 127 - v); the
+ * normal is (u, v, 126 - u - v) with the signs, normalised. */
+static uint32_t rgba_from_565(uint32_t c) {
+    uint32_t r5 = (c >> 11) & 31u, g6 = (c >> 5) & 63u, b5 = c & 31u;
+    uint32_t r = (r5 << 3) | (r5 >> 2), g = (g6 << 2) | (g6 >> 4), b = (b5 << 3) | (b5 >> 2);
+    return r | (g << 8) | (b << 16) | 0xff000000u;
+}
+
+static f3 normal_from_16(uint32_t q) {
+    int u = (int)((q >> 7) & 63u), v = (int)(q & 127u);
+    if (u + v > 126) {
+        u = 127 - u;
+        v = 127 - v;
+    }
+#if ORACLE_PLANT == 15
+    f3 p = f3_make((float)v, (float)u, (float)(126 - u - v)); /* planted: u and v swapped */
+#else
+    f3 p = f3_make((float)u, (float)v, (float)(126 - u - v));
+#endif
+    float len = sqrtf(f3_dot(p, p));
+    f3 n = f3_make(p.x / len, p.y / len, p.z / len);
+    if (q & 0x8000u) n.x = -n.x;
+    if (q & 0x4000u) n.y = -n.y;
+    if (q & 0x2000u) n.z = -n.z;
+    return n;
+}
+
 static void shade_pixel(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, const float* pos,
-                        f3 dirv, int light_mode, const float* light, pixel_out* o, int* bad_id) {
+                        f3 dirv, int light_mode, const float* light, int payload, pixel_out* o, int* bad_id) {
     float dir[3] = {dirv.x, dirv.y, dirv.z};
     o->hit_id = O_MISS;
     o->vox[0] = o->vox[1] = o->vox[2] = -1;
@@ -370,6 +402,14 @@ static void shade_pixel(const uint32_t* nodes, uint32_t num_nodes, uint32_t root
 #endif
         lam = o_max(lam, 0.0f);
     }
+    uint32_t leaf = nodes[(size_t)10 * h.node + 2 + (uint32_t)h.child];
+    uint32_t color = leaf;
+    if (payload == 1) { /* the leaf's own colour and normal */
+        f3 n = normal_from_16(leaf >> 16);
+        f3 L = light_mode == 0 ? f3_make(-dir[0], -dir[1], -dir[2]) : f3_make(light[0], light[1], light[2]);
+        lam = o_max(f3_dot(n, L), 0.0f);
+        color = rgba_from_565(leaf & 0xffffu);
+    }
     int lit = 1;
     if (light_mode == 2 && lam > 0.0f) {
         /* shadow ray: from the hit point, stepped half a cell back across the
@@ -398,10 +438,9 @@ static void shade_pixel(const uint32_t* nodes, uint32_t num_nodes, uint32_t root
         o->shadow = lit ? 0 : 1;
     }
     float k = lit ? 0.1f + 0.9f * lam : 0.1f;
-    uint32_t leaf = nodes[(size_t)10 * h.node + 2 + (uint32_t)h.child];
     uint32_t out = 0;
     for (int ch = 0; ch < 3; ++ch) {
-        float v = (float)((leaf >> (8 * ch)) & 0xffu) * k + 0.5f;
+        float v = (float)((color >> (8 * ch)) & 0xffu) * k + 0.5f;
         out |= ((uint32_t)v & 0xffu) << (8 * ch);
     }
     out |= 0xffu << 24;
@@ -411,10 +450,11 @@ static void shade_pixel(const uint32_t* nodes, uint32_t num_nodes, uint32_t root
 /* ---- RenderRect (cell/ppu_renderer.cpp:43-70) over rows [y0, y1) --------- */
 /* Outputs are full-frame arrays (row-major); any may be NULL. Returns the number
  * of rays traced (primary + shadow), or -1 if a child id was out of range. */
-long long oracle_render_rows(const uint32_t* nodes, uint32_t num_nodes, uint32_t root,
-                             const float* pos, const float* basis, int w, int h, int y0, int y1,
-                             int light_mode, const float* light, uint32_t* hit_id, int32_t* voxel,
-                             float* t, uint32_t* rgba, uint8_t* shadow, uint32_t* visits) {
+long long oracle_render_rows_fmt(const uint32_t* nodes, uint32_t num_nodes, uint32_t root,
+                                 const float* pos, const float* basis, int w, int h, int y0, int y1,
+                                 int light_mode, const float* light, int payload, uint32_t* hit_id,
+                                 int32_t* voxel, float* t, uint32_t* rgba, uint8_t* shadow,
+                                 uint32_t* visits) {
     long long rays = 0;
     int bad = 0;
     if (y1 > h) y1 = h;
@@ -422,7 +462,7 @@ long long oracle_render_rows(const uint32_t* nodes, uint32_t num_nodes, uint32_t
         for (int x = 0; x < w; ++x) {
             size_t offs = (size_t)y * (size_t)w + (size_t)x;
             pixel_out o;
-            shade_pixel(nodes, num_nodes, root, pos, pixel_dir(basis, x, y), light_mode, light, &o,
+            shade_pixel(nodes, num_nodes, root, pos, pixel_dir(basis, x, y), light_mode, light, payload, &o,
                         &bad);
             rays += 1 + (o.shadow < 2 ? 1 : 0);
             if (hit_id) hit_id[offs] = o.hit_id;
@@ -438,6 +478,24 @@ long long oracle_render_rows(const uint32_t* nodes, uint32_t num_nodes, uint32_t
         }
     }
     return bad ? -1 : rays;
+}
+
+/* RGBA8 payloads (the default format). */
+long long oracle_render_rows(const uint32_t* nodes, uint32_t num_nodes, uint32_t root,
+                             const float* pos, const float* basis, int w, int h, int y0, int y1,
+                             int light_mode, const float* light, uint32_t* hit_id, int32_t* voxel,
+                             float* t, uint32_t* rgba, uint8_t* shadow, uint32_t* visits) {
+    return oracle_render_rows_fmt(nodes, num_nodes, root, pos, basis, w, h, y0, y1, light_mode, light, 0,
+                                  hit_id, voxel, t, rgba, shadow, visits);
+}
+
+/* One colour16 + normal16 payload decoded: RGBA8 and the unit normal. */
+void oracle_decode_c16n16(uint32_t leaf, uint32_t* rgba, float* normal) {
+    f3 n = normal_from_16(leaf >> 16);
+    *rgba = rgba_from_565(leaf & 0xffffu);
+    normal[0] = n.x;
+    normal[1] = n.y;
+    normal[2] = n.z;
 }
 
 /* Single rays with explicit origins and directions (for the pins). Directions
@@ -479,13 +537,13 @@ int oracle_trace_rays(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, 
  * 1 occluded, 2 no shadow ray), t, hit level and node visits (primary +
  * shadow). Returns -1 on a bad child id. */
 int oracle_shade_rays(const uint32_t* nodes, uint32_t num_nodes, uint32_t root, int n, const float* pos,
-                      const float* dir, int light_mode, const float* light, uint32_t* out_hit,
+                      const float* dir, int light_mode, const float* light, int payload, uint32_t* out_hit,
                       uint32_t* out_rgba, uint8_t* out_shadow, float* out_t, uint32_t* out_visits) {
     int bad = 0;
     for (int i = 0; i < n; ++i) {
         pixel_out o;
         shade_pixel(nodes, num_nodes, root, pos + 3 * i,
-                    f3_make(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]), light_mode, light, &o, &bad);
+                    f3_make(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]), light_mode, light, payload, &o, &bad);
         out_hit[i] = o.hit_id;
         out_rgba[i] = o.rgba;
         out_shadow[i] = o.shadow;

@@ -4,10 +4,13 @@ Loads the in-tree library built by build.py and fails loudly when it is
 missing: there is no fallback path.
 """
 import ctypes
+import os
 import pathlib
 
 _PKG = pathlib.Path(__file__).resolve().parent
-LIB_PATH = _PKG / "lib" / "libyoxel.so"
+# YOXEL_LIB names another build of the same library (the host-sanitized
+# build that tests/sanitize/run.py makes); the default is the in-tree build.
+LIB_PATH = pathlib.Path(os.environ["YOXEL_LIB"]) if os.environ.get("YOXEL_LIB") else _PKG / "lib" / "libyoxel.so"
 
 YX_OK = 0
 YX_ERR_INVALID_ARG = -1
@@ -25,6 +28,8 @@ YX_MISS = 0xFFFFFFFF
 YX_LIGHT_HEADLIGHT = 0
 YX_LIGHT_SUN = 1
 YX_LIGHT_SUN_SHADOW = 2
+YX_PAYLOAD_RGBA8 = 0
+YX_PAYLOAD_C16N16 = 1
 
 _ERRNAMES = {
     YX_ERR_INVALID_ARG: "YX_ERR_INVALID_ARG",
@@ -246,6 +251,7 @@ SIGNATURES = {
     "yx_renderer_ray_basis": (ctypes.c_int, [_P, _F3]),
     "yx_render": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), _P]),
     "yx_render_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), _P]),
+    "yx_renderer_set_payload": (ctypes.c_int, [_P, ctypes.c_int32]),
     "yx_render_frame": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "yx_renderer_scene_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "yx_renderer_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),

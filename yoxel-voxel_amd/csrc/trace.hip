@@ -29,6 +29,7 @@ struct FrameArgs {
     float basis[9];
     float light[3];  // unit direction toward the light, as given
     int light_mode;
+    int payload;  // YX_PAYLOAD_*
     int w, h;
     yx_outputs out;
 };
@@ -67,6 +68,12 @@ __global__ __launch_bounds__(256) void frame_kernel(FrameArgs a) {
             lam = (de < 0) ? le : -le;
             lam = dmax(lam, 0.0f);
         }
+        uint32_t color = leaf.x;
+        if (a.payload == YX_PAYLOAD_C16N16) {  // the leaf's own colour and normal
+            lam = a.light_mode == YX_LIGHT_HEADLIGHT ? lambert_c16n16(leaf.x, -d.x, -d.y, -d.z)
+                                                     : lambert_c16n16(leaf.x, a.light[0], a.light[1], a.light[2]);
+            color = rgba_from_565(leaf.x & 0xffffu);
+        }
         bool lit = true;
         if (a.light_mode == YX_LIGHT_SUN_SHADOW && lam > 0.0f) {
             float p[3];
@@ -87,7 +94,7 @@ __global__ __launch_bounds__(256) void frame_kernel(FrameArgs a) {
             shadow = lit ? 0 : 1;
         }
         const float k = lit ? 0.1f + 0.9f * lam : 0.1f;
-        rgba = shade_rgba(leaf.x, k);
+        rgba = shade_rgba(color, k);
     }
     if (a.out.hit_id) a.out.hit_id[offs] = hit_id;
     if (a.out.voxel) {
@@ -118,6 +125,7 @@ struct yx_renderer {
     int w = 640, h = 480;              // RendererBase::SetResolution(640, 480)
     float light[3] = {0.0f, 0.0f, 1.0f};
     int light_mode = YX_LIGHT_HEADLIGHT;
+    int payload = YX_PAYLOAD_RGBA8;  // how leaf payloads are read when shading
     bool has_scene = false;
     uint2* d_recs = nullptr;
     size_t n_recs = 0;
@@ -243,6 +251,7 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
         }
         std::memcpy(a.basis, basis, sizeof(basis));
         a.light_mode = r->light_mode;
+        a.payload = r->payload;
         a.w = r->w;
         a.h = r->h;
         a.out = out;
@@ -268,6 +277,7 @@ static int launch_frame(yx_renderer* r, const yx_outputs& out, hipStream_t strea
     }
     std::memcpy(a.basis, basis, sizeof(basis));
     a.light_mode = r->light_mode;
+    a.payload = r->payload;
     a.w = r->w;
     a.h = r->h;
     a.out = out;
@@ -535,6 +545,15 @@ extern "C" int yx_renderer_set_light(yx_renderer* r, const float dir[3], int32_t
         return fail(YX_ERR_INVALID_ARG, "unknown light mode %d", mode);
     for (int i = 0; i < 3; ++i) r->light[i] = dir[i];
     r->light_mode = mode;
+    return YX_OK;
+}
+
+extern "C" int yx_renderer_set_payload(yx_renderer* r, int32_t format) {
+    clear_error();
+    if (!r) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (format != YX_PAYLOAD_RGBA8 && format != YX_PAYLOAD_C16N16)
+        return fail(YX_ERR_INVALID_ARG, "payload format %d", format);
+    r->payload = format;
     return YX_OK;
 }
 

@@ -150,6 +150,54 @@ __device__ __forceinline__ uint32_t shade_rgba(uint32_t leaf, float k) {
     return out;
 }
 
+// ---- the reference's leaf payload: VoxData = colour16 + normal16 ----------
+// (reaction/report/main.tex:38,65; unpacked by UnpackVoxData / UnpackColor /
+// UnpackNormal, demo/SVORenderer.cpp:181-185, whose bodies are missing). The
+// bit layout is DEFINED HERE:
+//   bits 0-15:  colour, RGB565 (red in bits 11-15), widened to 8 bits by bit
+//               replication, alpha 255;
+//   bits 16-31: normal, quantized as in Baptista, "Higher Accuracy Quantized
+//               Normals" (the thesis's citation): bits 15, 14, 13 of the half
+//               are the signs of x, y, z; bits 7-12 (u) and 0-6 (v) a point of
+//               the triangle u + v <= 126 of (|x|, |y|) / (|x| + |y| + |z|) *
+//               126, points with u >= 64 folded to (127 - u, 127 - v); the
+//               normal is (u, v, 126 - u - v) with the signs, normalised.
+// Every float operation is written out in order so the oracle's restatement
+// gives the same bits.
+__device__ __forceinline__ uint32_t rgba_from_565(uint32_t c) {
+    const uint32_t r5 = (c >> 11) & 31u, g6 = (c >> 5) & 63u, b5 = c & 31u;
+    const uint32_t r = (r5 << 3) | (r5 >> 2), g = (g6 << 2) | (g6 >> 4), b = (b5 << 3) | (b5 >> 2);
+    return r | (g << 8) | (b << 16) | 0xff000000u;
+}
+
+__device__ __forceinline__ void normal_from_16(uint32_t q, float& nx, float& ny, float& nz) {
+    int u = (int)((q >> 7) & 63u), v = (int)(q & 127u);
+    if (u + v > 126) {
+        u = 127 - u;
+        v = 127 - v;
+    }
+    const float x = (float)u, y = (float)v, z = (float)(126 - u - v);
+    float n2 = 0.0f;
+    n2 += x * x;
+    n2 += y * y;
+    n2 += z * z;
+    const float len = __builtin_sqrtf(n2);
+    nx = (q & 0x8000u) ? -(x / len) : x / len;
+    ny = (q & 0x4000u) ? -(y / len) : y / len;
+    nz = (q & 0x2000u) ? -(z / len) : z / len;
+}
+
+// Lambert term max(0, n.L) of a colour16 + normal16 leaf toward unit L.
+__device__ __forceinline__ float lambert_c16n16(uint32_t leaf, float lx, float ly, float lz) {
+    float nx, ny, nz;
+    normal_from_16(leaf >> 16, nx, ny, nz);
+    float d = 0.0f;
+    d += nx * lx;
+    d += ny * ly;
+    d += nz * lz;
+    return dmax(d, 0.0f);
+}
+
 struct HitInfo {
     uint32_t leaf_rec;  // record index of the hit leaf
     int child;          // real child slot

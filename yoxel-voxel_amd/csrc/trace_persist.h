@@ -53,6 +53,7 @@ struct PersistArgs {
     float basis[9];
     float light[3];
     int light_mode;
+    int payload;  // YX_PAYLOAD_*: how leaf payloads are shaded (single scene)
     int w, h;
     yx_outputs out;
     uint32_t* counters;  // [0, nq): queue heads; [nq]: retired-wave count
@@ -287,6 +288,12 @@ __global__ __launch_bounds__(BLOCK) void persist_kernel(PersistArgs a) {
                         lam = dmax((de < 0) ? le : -le, 0.0f);
                     }
                     color = leaf.x;
+                    if (a.payload == YX_PAYLOAD_C16N16) {  // the leaf's own colour and normal
+                        lam = a.light_mode == YX_LIGHT_HEADLIGHT
+                                  ? lambert_c16n16(leaf.x, -dx, -dy, -dz)
+                                  : lambert_c16n16(leaf.x, a.light[0], a.light[1], a.light[2]);
+                        color = rgba_from_565(leaf.x & 0xffffu);
+                    }
                     if (a.light_mode == YX_LIGHT_SUN_SHADOW && lam > 0.0f) {
                         // hit point stepped half a cell back across the entry
                         // face (a - b == a + (-b) exactly)

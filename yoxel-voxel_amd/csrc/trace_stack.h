@@ -345,6 +345,11 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     lam = dmax((de < 0) ? le : -le, 0.0f);
                 }
                 color = leaf.x;
+                if (!BATCH && a.payload == YX_PAYLOAD_C16N16) {  // the leaf's own colour and normal
+                    lam = light_mode == YX_LIGHT_HEADLIGHT ? lambert_c16n16(leaf.x, -dx, -dy, -dz)
+                                                           : lambert_c16n16(leaf.x, light[0], light[1], light[2]);
+                    color = rgba_from_565(leaf.x & 0xffffu);
+                }
                 if (light_mode == YX_LIGHT_SUN_SHADOW && lam > 0.0f) {
                     // hit point stepped half a cell back across the entry face
                     const float half = __builtin_bit_cast(float, (uint32_t)(127 - (level + 2)) << 23);

@@ -225,6 +225,11 @@ class SVORenderer:
     def SetLight(self, direction, mode=_lib.YX_LIGHT_SUN_SHADOW):
         check(lib().yx_renderer_set_light(self._h, f3(direction), int(mode)))
 
+    def SetPayloadFormat(self, fmt):
+        """YX_PAYLOAD_RGBA8 (default) or YX_PAYLOAD_C16N16 (the reference's
+        colour16 + normal16 VoxData, layout in include/yoxel.h)."""
+        check(lib().yx_renderer_set_payload(self._h, int(fmt)))
+
     def get_tuning(self):
         t = _lib.Tuning()
         check(lib().yx_renderer_get_tuning(self._h, ctypes.byref(t)))
