@@ -239,7 +239,7 @@ def main():
                      "kernel": f"stack_kernel<{tuning['block']}, {tuning['window']}>",
                      "kernel_ms": round(avg_kernel_s * 1e3, 4),
                      "algorithmic_bytes_per_launch": int(timed_bytes),
-                     "frames": "the timed steps (orbit frame k % %d)" % nframes,
+                     "frames": f"the timed steps (orbit frame k mod {nframes})",
                      "all_frames": all_frames},
         "frame_ms": {"median": round(float(np.median(kern_ms)), 4), "min": round(float(np.min(kern_ms)), 4),
                      "max": round(float(np.max(kern_ms)), 4)},

@@ -206,8 +206,11 @@ static inline bool in_edit(const EditSet& set, const EditSphere& e, uint32_t i, 
 static inline uint32_t edited_color(uint64_t code, uint64_t seed, const EditSet* edits, size_t limit) {
     if (edits && !edits->empty()) {
         const uint32_t i = compact3(code), j = compact3(code >> 1), k = compact3(code >> 2);
+        // newest first; the pointer never moves below the cell's first id (an
+        // empty cell of an empty id list has none at all)
         const uint32_t* b = edits->begin((int)i, (int)j);
-        for (const uint32_t* p = edits->end((int)i, (int)j); p-- != b;) {
+        for (const uint32_t* p = edits->end((int)i, (int)j); p != b;) {
+            --p;
             if (*p >= limit) continue;
             const EditSphere& e = edits->edits[*p];
             if ((int)i < e.xlo || (int)i > e.xhi || (int)j < e.ylo || (int)j > e.yhi) continue;
