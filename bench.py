@@ -122,7 +122,7 @@ def main():
     visits = torch.empty((h, w), dtype=torch.int32, device=dev)
 
     # untimed counting pass: rays and algorithmic bytes of every frame
-    rays_f, bytes_f = [], []
+    rays_f, bytes_f, hits_f, unlit_f = [], [], [], []
     for pos, d in cams:
         r.SetViewPos(pos)
         r.SetViewDir(d)
@@ -134,6 +134,8 @@ def main():
         nvis = int(visits.to(torch.int64).sum().item())
         rays_f.append(w * h + nshadow)
         bytes_f.append(8 * nvis + 4 * nhit + OUT_BYTES_PER_PIXEL * w * h)
+        hits_f.append(nhit)
+        unlit_f.append(nhit - nshadow if mode == yx.YX_LIGHT_SUN_SHADOW else 0)
     print(f"[bench] counting pass done: mean rays/frame {np.mean(rays_f):.0f}", file=sys.stderr, flush=True)
 
     stream = torch.cuda.current_stream()
@@ -233,7 +235,10 @@ def main():
         "config": {"workload": args.workload, "depth": desc["depth"], "resolution": [w, h],
                    "frames": nframes, "rays": "primary+shadow" if mode == 2 else "primary",
                    "nodes": info["num_nodes"], "leaves": info["num_leaves"],
-                   "mean_rays_per_frame": round(float(np.mean(rays_f)), 1), "tuning": tuning},
+                   "mean_rays_per_frame": round(float(np.mean(rays_f)), 1),
+                   "mean_hits_per_frame": round(float(np.mean(hits_f)), 1),
+                   # hits whose face turns away from the light (Lambert term 0) get no shadow ray
+                   "mean_hits_without_shadow_ray": round(float(np.mean(unlit_f)), 1), "tuning": tuning},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.workload),
                      "kernel": f"stack_kernel<{tuning['block']}, {tuning['window']}>",

@@ -239,6 +239,17 @@ int yx_renderer_set_light(yx_renderer* r, const float dir[3], int32_t mode);
 #define YX_PAYLOAD_RGBA8 0
 #define YX_PAYLOAD_C16N16 1
 int yx_renderer_set_payload(yx_renderer* r, int32_t format);
+/* SVORenderer::DumpTraceData (demo/SVORenderer.cpp:158-192), its decoding
+ * step: for each of `n` pixels of a frame's hit_id output (a host copy), the
+ * hit leaf's VoxData (node hit_id / 8, child hit_id % 8 of `scene`) read as
+ * `payload`. color: RGBA8 (C16N16: the RGB565 colour widened as above, alpha
+ * 255); normal: 3 floats per pixel (C16N16: the stored normal; RGBA8 stores
+ * none and gives 0, 0, 0). Misses (YX_MISS) give zeros, as the reference skips
+ * them. Either output may be NULL. YX_ERR_FORMAT for an id that names no leaf
+ * of the scene. The reference also writes a distance file that it leaves at
+ * zero (its fill line is commented out); the hit distance is yx_outputs.t. */
+int yx_scene_decode_hits(const yx_scene* scene, const uint32_t* hit_id, uint64_t n, int32_t payload,
+                         uint32_t* color, float* normal);
 /* Ray directions of the current camera (RendererBase::InitRayDir,
  * cell/renderer_base.h:50-61): dir0, du, dv as 9 floats. */
 int yx_renderer_ray_basis(const yx_renderer* r, float basis[9]);

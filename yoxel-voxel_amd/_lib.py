@@ -252,6 +252,7 @@ SIGNATURES = {
     "yx_render": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), _P]),
     "yx_render_noalloc": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), _P]),
     "yx_renderer_set_payload": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "yx_scene_decode_hits": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.c_int32, _P, _P]),
     "yx_render_frame": (ctypes.c_int, [_P, ctypes.POINTER(_P)]),
     "yx_renderer_scene_bytes": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
     "yx_renderer_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),

@@ -13,12 +13,12 @@
 // -> scan (rank of each new brick) -> free-slot enumeration (scan over the
 // used map) -> assign, so the result does not depend on thread timing.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <new>
 
 #include "internal.h"
+#include "prims.h"
 
 #define BP_HIP(call)                                                                           \
     do {                                                                                       \
@@ -182,8 +182,8 @@ static int bp_alloc(T** p, size_t n) {
 }
 
 static int bp_scan_space(yx_brick_pool* p, uint32_t n, hipStream_t st) {
-    size_t bytes = 0;
-    BP_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, p->d_isfree, p->d_fpos, (int)std::max(n, 1u), st));
+    (void)st;
+    const size_t bytes = scan_workspace_bytes(std::max(n, 1u));
     if (bytes <= p->scan_bytes) return YX_OK;
     if (p->scan_tmp) BP_HIP(hipFree(p->scan_tmp));
     p->scan_tmp = nullptr;
@@ -238,16 +238,11 @@ extern "C" int yx_brick_pool_create(const yx_volume_dev* vol, void* stream, yx_b
     // used slots: count them with a reduction over the used map
     uint32_t* d_cnt = nullptr;
     if ((rc = bp_alloc(&d_cnt, 1)) != YX_OK) return give_up(rc);
-    size_t bytes = 0;
-    (void)hipcub::DeviceReduce::Sum(nullptr, bytes, p->d_used, d_cnt, (int)std::max(vol->num_slots, 1u), st);
-    void* tmp = nullptr;
     uint32_t cnt = 0;
     const bool ok = vol->num_slots == 0 ||
-                    (hipMalloc(&tmp, std::max<size_t>(bytes, 1)) == hipSuccess &&
-                     hipcub::DeviceReduce::Sum(tmp, bytes, p->d_used, d_cnt, (int)vol->num_slots, st) == hipSuccess &&
+                    (reduce_sum_u32(p->d_used, vol->num_slots, d_cnt, p->scan_tmp, p->scan_bytes, st) == YX_OK &&
                      hipMemcpyAsync(&cnt, d_cnt, 4, hipMemcpyDeviceToHost, st) == hipSuccess &&
                      hipStreamSynchronize(st) == hipSuccess);
-    if (tmp) (void)hipFree(tmp);
     (void)hipFree(d_cnt);
     if (!ok) return give_up(fail(YX_ERR_HIP, "counting used slots"));
     p->used_count = cnt;
@@ -294,12 +289,13 @@ extern "C" int yx_brick_pool_alloc(yx_brick_pool* p, const int32_t* bricks, uint
         return fail(YX_ERR_RANGE, "brick index outside the %u bricks of the table", p->nb);
     }
     hipLaunchKernelGGL(bp_flag_kernel, bp_grid(n), dim3(256), 0, st, bricks, n, p->d_claim, table, p->d_flag);
-    BP_HIP(hipcub::DeviceScan::ExclusiveSum(p->scan_tmp, p->scan_bytes, p->d_flag, p->d_rank, (int)n, st));
+    if ((rc = scan_excl_u32(p->d_flag, p->d_rank, n, nullptr, p->scan_tmp, p->scan_bytes, st)) != YX_OK) return rc;
     // free slots in increasing order
     uint32_t last[2] = {0, 0}, free_last[2] = {0, 0};
     if (cap) {
         hipLaunchKernelGGL(bp_free_flag_kernel, bp_grid(cap), dim3(256), 0, st, p->d_used, cap, p->d_isfree);
-        BP_HIP(hipcub::DeviceScan::ExclusiveSum(p->scan_tmp, p->scan_bytes, p->d_isfree, p->d_fpos, (int)cap, st));
+        if ((rc = scan_excl_u32(p->d_isfree, p->d_fpos, cap, nullptr, p->scan_tmp, p->scan_bytes, st)) != YX_OK)
+            return rc;
         hipLaunchKernelGGL(bp_free_list_kernel, bp_grid(cap), dim3(256), 0, st, p->d_isfree, p->d_fpos, cap,
                            p->d_freelist);
         BP_HIP(hipMemcpyAsync(&free_last[0], p->d_fpos + (cap - 1), 4, hipMemcpyDeviceToHost, st));

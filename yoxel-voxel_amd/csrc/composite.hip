@@ -20,7 +20,7 @@
 // The backward recomputes the exact discrete sample sequence instead of
 // storing per-ray lists, writes one (voxel key, float4 gradient) record per
 // sample at the ray's exclusive-scan offset, and sorts the records' ids by
-// the brick slot bits of the key with a stable LSD radix sort of its own
+// the brick slot bits of the key with the stable LSD radix sort of prims.hip
 // (8-bit digits; per-block digit histograms, a scan, a scatter that ranks
 // equal digits in record order with wave ballots). One wave per slot then sums
 // each voxel's records in emission order in LDS and writes the brick. Bricks
@@ -37,6 +37,7 @@
 #include <type_traits>
 
 #include "internal.h"
+#include "prims.h"
 #include "trace_device.h"
 
 namespace yx {
@@ -546,191 +547,6 @@ __global__ __launch_bounds__(256) void comp_reduce_slots_kernel(const uint32_t* 
     if (lane == 0 && cnt) atomicAdd(distinct, cnt);
 }
 
-// ---- scan and stable radix sort of the adjoint records ---------------------
-// Block-wide exclusive scan of one value per thread (256 threads); *total gets
-// the block's sum. Uses sh4[4]; ends with a barrier so it can be called again.
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh4, uint32_t& total) {
-    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    uint32_t inc = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        const uint32_t o = __shfl_up(inc, (unsigned)off);
-        if (lane >= off) inc += o;
-    }
-    if (lane == 63) sh4[w] = inc;
-    __syncthreads();
-    uint32_t before = 0, tot = 0;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t x = sh4[k];
-        before += k < w ? x : 0u;
-        tot += x;
-    }
-    total = tot;
-    __syncthreads();
-    return before + inc - v;
-}
-
-constexpr int kScanThreads = 256;
-constexpr int kScanRounds = 16;
-constexpr uint32_t kScanChunk = kScanThreads * kScanRounds;  // items per block
-
-// partial[b] = sum of in[b * kScanChunk, ...) (coalesced rounds of 256 items).
-__global__ __launch_bounds__(kScanThreads) void scan_reduce_kernel(const uint32_t* in, uint32_t n,
-                                                                   uint32_t* partial) {
-    __shared__ uint32_t sh4[4];
-    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x;
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanRounds; ++k) {
-        const uint32_t i = base + (uint32_t)k * kScanThreads;
-        s += i < n ? in[i] : 0u;
-    }
-    uint32_t tot;
-    block_excl_scan(s, sh4, tot);
-    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
-}
-
-// In-place exclusive scan of the np block sums (one block, chunks of 256 with
-// a carry); *total (if not null) receives the grand total.
-__global__ __launch_bounds__(kScanThreads) void scan_partials_kernel(uint32_t* partial, uint32_t np,
-                                                                     uint32_t* total) {
-    __shared__ uint32_t sh4[4];
-    uint32_t carry = 0;
-    for (uint32_t c = 0; c < np; c += kScanThreads) {
-        const uint32_t i = c + threadIdx.x;
-        const uint32_t v = i < np ? partial[i] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan(v, sh4, tot);
-        if (i < np) partial[i] = carry + ex;
-        carry += tot;
-    }
-    if (threadIdx.x == 0 && total) *total = carry;
-}
-
-// out[i] = partial[block] + exclusive prefix of in within the block's chunk
-// (out may alias in).
-__global__ __launch_bounds__(kScanThreads) void scan_apply_kernel(const uint32_t* in, uint32_t n,
-                                                                  const uint32_t* partial, uint32_t* out) {
-    __shared__ uint32_t sh4[4];
-    uint32_t carry = partial[blockIdx.x];
-    const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x;
-    for (int k = 0; k < kScanRounds; ++k) {
-        const uint32_t i = base + (uint32_t)k * kScanThreads;
-        const uint32_t v = i < n ? in[i] : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_excl_scan(v, sh4, tot);
-        if (i < n) out[i] = carry + ex;
-        carry += tot;
-    }
-}
-
-constexpr int kSortThreads = 256;
-constexpr int kSortRounds = 16;
-constexpr uint32_t kSortTile = kSortThreads * kSortRounds;  // records per block
-constexpr int kRadixBits = 8;
-constexpr int kRadix = 1 << kRadixBits;
-
-// Lanes of the wave holding the same digit (of `bits` bits) as this lane,
-// among the lanes with `in` set.
-__device__ __forceinline__ uint64_t digit_peers(bool in, uint32_t d, int bits) {
-    uint64_t peers = __ballot(in);
-    for (int b = 0; b < bits; ++b) {
-        const bool on = (d >> b) & 1u;
-        const uint64_t m = __ballot(on);
-        peers &= on ? m : ~m;
-    }
-    return peers;
-}
-
-// Digit histogram of block b's records [b * kSortTile, ...) below *n, stored
-// digit-major (hist[d * nb + b]) so that one exclusive scan gives every
-// (digit, block) its first output position. Blocks past the end store zeros.
-// Neighbouring records mostly share a digit (records in emission order share
-// brick slots), so each run of equal digits among a wave's lanes is counted
-// with one LDS add by its first lane; per-lane adds to one counter would
-// serialise.
-__global__ __launch_bounds__(kSortThreads) void rs_upsweep_kernel(const uint32_t* keys, const uint32_t* n_ptr,
-                                                                  int shift, int bits, uint32_t* hist,
-                                                                  uint32_t nb) {
-    __shared__ uint32_t h[kRadix];
-    for (int d = (int)threadIdx.x; d < kRadix; d += kSortThreads) h[d] = 0u;
-    __syncthreads();
-    const uint32_t n = *n_ptr;
-    const uint32_t mask = (1u << bits) - 1u;
-    const uint32_t base = blockIdx.x * kSortTile + threadIdx.x;
-    const int lane = (int)(threadIdx.x & 63);
-    const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
-    if (blockIdx.x * kSortTile < n) {
-        for (int k = 0; k < kSortRounds; ++k) {
-            const uint32_t i = base + (uint32_t)k * kSortThreads;
-            const bool in = i < n;  // the lanes in range are a prefix of the wave
-            const uint32_t d = in ? (keys[i] >> shift) & mask : 0u;
-            const uint32_t prev = __shfl_up(d, 1u);
-            const bool head = in && (lane == 0 || prev != d);
-            const uint64_t heads = __ballot(head);
-            const int n_in = __popcll(__ballot(in));
-            if (head) {
-                const uint64_t nxt = heads & above;
-                const int end = nxt ? __ffsll((unsigned long long)nxt) - 1 : n_in;
-                atomicAdd(&h[d], (uint32_t)(end - lane));
-            }
-        }
-    }
-    __syncthreads();
-    for (int d = (int)threadIdx.x; d < kRadix; d += kSortThreads) hist[(size_t)d * nb + blockIdx.x] = h[d];
-}
-
-// Stable scatter of block b's records by digit. `offs` is the scanned
-// histogram. Rounds take 256 consecutive records; a record's place among the
-// block's records of its digit is the count from earlier rounds, from earlier
-// waves of its round, and from earlier lanes of its wave (ballot rank), so
-// equal digits keep their record order. ids_in == null: ids are positions.
-__global__ __launch_bounds__(kSortThreads) void rs_downsweep_kernel(const uint32_t* keys_in, const uint32_t* ids_in,
-                                                                    uint32_t* keys_out, uint32_t* ids_out,
-                                                                    const uint32_t* n_ptr, int shift, int bits,
-                                                                    const uint32_t* offs, uint32_t nb) {
-    __shared__ uint32_t run[kRadix];
-    __shared__ uint32_t wc_all[2][4][kRadix];  // per round parity, per wave
-    const uint32_t n = *n_ptr;
-    if (blockIdx.x * kSortTile >= n) return;  // whole block
-    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
-    const uint32_t mask = (1u << bits) - 1u;
-    for (int d = (int)threadIdx.x; d < kRadix; d += kSortThreads) {
-        run[d] = offs[(size_t)d * nb + blockIdx.x];
-        for (int q = 0; q < 4; ++q) wc_all[0][q][d] = wc_all[1][q][d] = 0u;
-    }
-    __syncthreads();
-    const uint64_t below = (1ull << lane) - 1ull;
-    const uint32_t base = blockIdx.x * kSortTile + threadIdx.x;
-    for (int k = 0; k < kSortRounds; ++k) {
-        const uint32_t i = base + (uint32_t)k * kSortThreads;
-        const bool in = i < n;
-        uint32_t key = 0, id = 0, d = 0;
-        if (in) {
-            key = keys_in[i];
-            id = ids_in ? ids_in[i] : i;
-            d = (key >> shift) & mask;
-        }
-        // the per-wave counts alternate between two buffers by round, so the
-        // next round's counts never meet this round's clearing
-        uint32_t (*wc)[kRadix] = wc_all[k & 1];
-        const uint64_t peers = digit_peers(in, d, bits);
-        if (in && (peers & below) == 0) wc[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (in) {
-            uint32_t pos = run[d] + (uint32_t)__popcll(peers & below);
-            for (int q = 0; q < w; ++q) pos += wc[q][d];
-            keys_out[pos] = key;
-            ids_out[pos] = id;
-        }
-        __syncthreads();
-        for (int dd = (int)threadIdx.x; dd < kRadix; dd += kSortThreads) {
-            run[dd] += wc[0][dd] + wc[1][dd] + wc[2][dd] + wc[3][dd];
-            wc[0][dd] = wc[1][dd] = wc[2][dd] = wc[3][dd] = 0u;
-        }
-    }
-}
-
 // Record total against the workspace capacity: n = min(total, cap), the
 // overflow flag, and the distinct-voxel counter cleared.
 __global__ void comp_check_kernel(uint32_t* misc, uint32_t cap) {
@@ -893,30 +709,17 @@ extern "C" int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, 
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-// Exclusive scan of n words (out may alias in) on `st`; partial holds
-// ceil(n / kScanChunk) words; *total (device, may be null) gets the sum.
-static int scan_u32(const uint32_t* in, uint32_t n, uint32_t* out, uint32_t* partial, uint32_t* total,
-                    hipStream_t st) {
-    const uint32_t nb = (n + kScanChunk - 1) / kScanChunk;
-    if (nb) hipLaunchKernelGGL(scan_reduce_kernel, dim3(nb), dim3(kScanThreads), 0, st, in, n, partial);
-    hipLaunchKernelGGL(scan_partials_kernel, dim3(1), dim3(kScanThreads), 0, st, partial, nb, total);
-    if (nb) hipLaunchKernelGGL(scan_apply_kernel, dim3(nb), dim3(kScanThreads), 0, st, in, n, partial, out);
-    YX_HIP(hipGetLastError());
-    return YX_OK;
-}
-
 // The backward's workspace for `cap` records, n_waves ray tiles and
 // num_slots brick slots (byte offsets, 256-aligned).
 struct CompLayout {
-    size_t misc, sums, offs, partial, keys, vals, kA, iA, kB, iB, hist, starts, bytes;
-    uint32_t nb;  // sort blocks
+    size_t misc, sums, offs, scan_ws, keys, vals, kA, iA, kB, iB, sort_ws, starts, bytes;
+    size_t scan_bytes, sort_bytes;
 };
 
 static CompLayout comp_layout(uint64_t cap, size_t n_waves, uint32_t num_slots) {
     CompLayout L;
-    L.nb = (uint32_t)std::max<uint64_t>(1, (cap + kSortTile - 1) / kSortTile);
-    const size_t hist = (size_t)kRadix * L.nb;
-    const size_t partial = std::max((n_waves + kScanChunk - 1) / kScanChunk, (hist + kScanChunk - 1) / kScanChunk) + 1;
+    L.scan_bytes = scan_workspace_bytes(n_waves);
+    L.sort_bytes = sort_workspace_bytes(cap);
     size_t o = 0;
     auto take = [&](size_t bytes) {
         const size_t at = o;
@@ -926,14 +729,14 @@ static CompLayout comp_layout(uint64_t cap, size_t n_waves, uint32_t num_slots) 
     L.misc = take(kMiscWords * 4);
     L.sums = take(n_waves * 4);
     L.offs = take(n_waves * 4);
-    L.partial = take(partial * 4);
+    L.scan_ws = take(L.scan_bytes);
     L.keys = take(cap * 4);
     L.vals = take(cap * 16);
     L.kA = take(cap * 4);
     L.iA = take(cap * 4);
     L.kB = take(cap * 4);
     L.iB = take(cap * 4);
-    L.hist = take(hist * 4);
+    L.sort_ws = take(L.sort_bytes);
     L.starts = take(((size_t)num_slots + 1) * 4);
     L.bytes = o;
     return L;
@@ -976,20 +779,20 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
                              (uint64_t)w, (uint64_t)h, (uint64_t)vol->grid, (uint64_t)vol->brick,
                              (uint64_t)vol->num_slots};
     if (!c->reserved && (c->cap == 0 || std::memcmp(sig, c->sig, sizeof(sig)) != 0)) {
-        const size_t pw = align256(n_waves * 4) / 4;
-        const size_t words = pw + (n_waves + kScanChunk - 1) / kScanChunk + 2;
+        const size_t sums_bytes = align256(n_waves * 4), red_bytes = scan_workspace_bytes(n_waves);
+        const size_t bytes = sums_bytes + red_bytes + 256;
         if (c->d_probe) YX_HIP(hipFree(c->d_probe));
         c->d_probe = nullptr;
-        hipError_t e = hipMalloc((void**)&c->d_probe, words * 4);
+        hipError_t e = hipMalloc((void**)&c->d_probe, bytes);
         if (e != hipSuccess) {
             c->d_probe = nullptr;
-            return fail(YX_ERR_OOM, "hipMalloc(%zu) for sizing: %s", words * 4, hipGetErrorString(e));
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) for sizing: %s", bytes, hipGetErrorString(e));
         }
         uint32_t* sums = c->d_probe;
-        uint32_t* part = c->d_probe + pw;
-        uint32_t* tot = part + (n_waves + kScanChunk - 1) / kScanChunk + 1;
+        char* red_ws = (char*)c->d_probe + sums_bytes;
+        uint32_t* tot = (uint32_t*)(red_ws + red_bytes);
         hipLaunchKernelGGL(comp_wave_sums_kernel, wgrid, wblock, 0, st, a, sums, (uint32_t)n_waves);
-        rc = scan_u32(sums, (uint32_t)n_waves, sums, part, tot, st);
+        rc = reduce_sum_u32(sums, n_waves, tot, red_ws, red_bytes, st);
         if (rc != YX_OK) return rc;
         uint32_t total = 0;
         YX_HIP(hipMemcpyAsync(&total, tot, 4, hipMemcpyDeviceToHost, st));
@@ -1026,12 +829,12 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     c->misc = misc;
     uint32_t* sums = (uint32_t*)(base + L.sums);
     uint32_t* offs = (uint32_t*)(base + L.offs);
-    uint32_t* partial = (uint32_t*)(base + L.partial);
+    void* scan_ws = base + L.scan_ws;
     uint32_t* keys = (uint32_t*)(base + L.keys);
     float4* vals = (float4*)(base + L.vals);
     uint32_t* kbuf[2] = {(uint32_t*)(base + L.kA), (uint32_t*)(base + L.kB)};
     uint32_t* ibuf[2] = {(uint32_t*)(base + L.iA), (uint32_t*)(base + L.iB)};
-    uint32_t* hist = (uint32_t*)(base + L.hist);
+    void* sort_ws = base + L.sort_ws;
     uint32_t* starts = (uint32_t*)(base + L.starts);
     uint32_t* n_dev = misc + kMiscN;
 
@@ -1039,7 +842,7 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     //    record offset; the total against the capacity
     hipLaunchKernelGGL(comp_wave_sums_kernel, wgrid, wblock, 0, st, a, sums, (uint32_t)n_waves);
     YX_HIP(hipGetLastError());
-    rc = scan_u32(sums, (uint32_t)n_waves, offs, partial, misc + kMiscTotal, st);
+    rc = scan_excl_u32(sums, offs, n_waves, misc + kMiscTotal, scan_ws, L.scan_bytes, st);
     if (rc != YX_OK) return rc;
     hipLaunchKernelGGL(comp_check_kernel, dim3(1), dim3(64), 0, st, misc, cap);
     YX_HIP(hipGetLastError());
@@ -1074,34 +877,12 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
 
     // 3. record ids sorted by the slot bits (or the whole key), stable: LSD
     //    passes of 8-bit digits, each a digit histogram per block, a scan of
-    //    the histograms and a scatter.
-    auto radix_sort = [&](const uint32_t* keys_in, const uint32_t* n_ptr, int b0, int b1, const uint32_t** kout,
-                          const uint32_t** iout) -> int {
-        const uint32_t* kin = keys_in;
-        const uint32_t* iin = nullptr;  // the first pass's ids are the positions
-        int pass = 0;
-        for (int sh = b0; sh < b1; sh += kRadixBits, ++pass) {
-            const int bits = std::min(kRadixBits, b1 - sh);
-            hipLaunchKernelGGL(rs_upsweep_kernel, dim3(L.nb), dim3(kSortThreads), 0, st, kin, n_ptr, sh, bits, hist,
-                               L.nb);
-            YX_HIP(hipGetLastError());
-            const int e = scan_u32(hist, (uint32_t)((size_t)kRadix * L.nb), hist, partial, nullptr, st);
-            if (e != YX_OK) return e;
-            hipLaunchKernelGGL(rs_downsweep_kernel, dim3(L.nb), dim3(kSortThreads), 0, st, kin, iin,
-                               kbuf[pass & 1], ibuf[pass & 1], n_ptr, sh, bits, hist, L.nb);
-            YX_HIP(hipGetLastError());
-            kin = kbuf[pass & 1];
-            iin = ibuf[pass & 1];
-        }
-        *kout = kin;
-        *iout = iin;
-        return YX_OK;
-    };
-
+    //    the histograms and a scatter; the first pass's ids are the positions.
     // 4. sum each voxel's records in emission order
     const uint32_t* keys_sorted = nullptr;
     const uint32_t* ids_sorted = nullptr;
-    rc = radix_sort(keys, n_dev, begin_bit, end_bit, &keys_sorted, &ids_sorted);
+    rc = radix_sort<uint32_t>(keys, nullptr, kbuf, ibuf, cap, n_dev, begin_bit, end_bit, &keys_sorted, &ids_sorted,
+                              sort_ws, L.sort_bytes, st);
     if (rc != YX_OK) return rc;
     if (by_slot) {
         const uint32_t ns = vol->num_slots;

@@ -22,7 +22,6 @@
 // (3D DDA) and composited front to back through a 256-entry RGBA8 transfer
 // table (volvis.py:145-152), stopping once alpha > 0.99.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstring>
@@ -30,6 +29,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "prims.h"
 #include "trace_device.h"
 #include "trace_persist.h"
 
@@ -190,10 +190,6 @@ __global__ __launch_bounds__(256) void nt_pack_kernel(NtSource s, int L, int R, 
         for (int z = 0; z < 8; ++z) b[lane + 64 * z] = (uint8_t)nt_voxel(s, P, x, y, bz * 8 + z);
     }
 }
-
-struct ToU32 {
-    __host__ __device__ uint32_t operator()(uint8_t v) const { return v; }
-};
 
 // ---- ray cast -----------------------------------------------------------------
 
@@ -624,14 +620,11 @@ struct DevBufs {
 
 static int nt_scan(const uint8_t* flags, uint32_t* out, uint64_t n, DevBufs& tmp, hipStream_t st) {
     if (n > 0xFFFFFFFFull) return fail(YX_ERR_RANGE, "more than 2^32 units on one level");
-    hipcub::TransformInputIterator<uint32_t, ToU32, const uint8_t*> in(flags, ToU32());
-    size_t bytes = 0;
-    YX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, st));
+    const size_t bytes = scan_workspace_bytes(n);
     void* ws = nullptr;
     int rc = tmp.alloc((uint8_t**)&ws, bytes);
     if (rc != YX_OK) return rc;
-    YX_HIP(hipcub::DeviceScan::ExclusiveSum(ws, bytes, in, out, (int)n, st));
-    return YX_OK;
+    return scan_excl_u8(flags, out, n, nullptr, ws, bytes, st);
 }
 
 // count of set flags = scan[n-1] + flags[n-1]

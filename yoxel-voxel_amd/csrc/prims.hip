@@ -1,0 +1,317 @@
+// Device-wide scans, a stable LSD radix sort and select-distinct (gfx950).
+//
+// Scans are reduce-then-scan in three launches: per-block sums of 4096 items
+// (256 threads, 16 coalesced rounds), one 1024-thread block that scans the
+// block sums (a strip per thread), and a pass that rescans each block's items
+// with its carry. Items come through a load functor and results leave
+// through a store functor, so flag scans, widening scans and the
+// select-distinct scatter need no intermediate arrays.
+//
+// The radix sort takes 8-bit digits from the least significant end: per pass
+// a digit histogram per block of 4096 keys (stored digit-major, so one
+// exclusive scan of it gives every (digit, block) pair its first output
+// position) and a scatter that ranks equal digits in key order with wave
+// ballots, so every pass is stable.
+#include "prims.h"
+
+#include <algorithm>
+
+#include "internal.h"
+
+namespace yx {
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanRounds = (int)(kPrimScanChunk / kScanThreads);
+constexpr int kSortThreads = 256;
+constexpr int kSortRounds = (int)(kPrimSortTile / kSortThreads);
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// ---- scan ----
+
+template <class Src, class Acc>
+__global__ __launch_bounds__(kScanThreads) void prim_scan_reduce(Src src, uint64_t n, Acc* partial) {
+    __shared__ Acc shw[kScanThreads / 64];
+    const uint64_t base = (uint64_t)blockIdx.x * kPrimScanChunk + threadIdx.x;
+    Acc s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanRounds; ++k) {
+        const uint64_t i = base + (uint64_t)k * kScanThreads;
+        s += i < n ? (Acc)src(i) : (Acc)0;
+    }
+    Acc tot;
+    (void)block_excl_scan<Acc, kScanThreads>(s, shw, tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// In-place exclusive scan of the np block sums; *total gets their sum.
+template <class Acc, class TotalT>
+__global__ __launch_bounds__(1024) void prim_scan_partials(Acc* partial, uint32_t np, TotalT* total) {
+    __shared__ Acc shw[1024 / 64];
+    const uint32_t per = (np + 1023u) / 1024u;
+    const uint32_t b = std::min(threadIdx.x * per, np), e = std::min(b + per, np);
+    Acc s = 0;
+    for (uint32_t i = b; i < e; ++i) s += partial[i];
+    Acc tot;
+    Acc run = block_excl_scan<Acc, 1024>(s, shw, tot);
+    for (uint32_t i = b; i < e; ++i) {
+        const Acc v = partial[i];
+        partial[i] = run;
+        run += v;
+    }
+    if (threadIdx.x == 0 && total) *total = (TotalT)tot;
+}
+
+// dst(i, exclusive prefix, item) for every item of the block's chunk. A
+// thread reads its item before the block scan and stores after it, so the
+// store may overwrite the loaded array in place.
+template <class Src, class Dst, class Acc>
+__global__ __launch_bounds__(kScanThreads) void prim_scan_apply(Src src, Dst dst, uint64_t n, const Acc* partial) {
+    __shared__ Acc shw[kScanThreads / 64];
+    Acc carry = partial[blockIdx.x];
+    const uint64_t base = (uint64_t)blockIdx.x * kPrimScanChunk + threadIdx.x;
+    for (int k = 0; k < kScanRounds; ++k) {
+        const uint64_t i = base + (uint64_t)k * kScanThreads;
+        const Acc v = i < n ? (Acc)src(i) : (Acc)0;
+        Acc tot;
+        const Acc ex = block_excl_scan<Acc, kScanThreads>(v, shw, tot);
+        if (i < n) dst(i, carry + ex, v);
+        carry += tot;
+    }
+}
+
+template <class T>
+struct Load {
+    const T* p;
+    __device__ T operator()(uint64_t i) const { return p[i]; }
+};
+
+template <class T>
+struct StorePrefix {
+    T* p;
+    __device__ void operator()(uint64_t i, T prefix, T) const { p[i] = prefix; }
+};
+
+struct NoStore {
+    template <class T>
+    __device__ void operator()(uint64_t, T, T) const {}
+};
+
+// 1 where the shifted value differs from its predecessor's (and at 0).
+struct FirstOfRun {
+    const uint64_t* p;
+    int shift;
+    __device__ uint32_t operator()(uint64_t i) const {
+        return (i == 0 || (p[i] >> shift) != (p[i - 1] >> shift)) ? 1u : 0u;
+    }
+};
+
+struct StoreFirst {
+    const uint64_t* p;
+    uint64_t* out;
+    int shift;
+    __device__ void operator()(uint64_t i, uint32_t prefix, uint32_t first) const {
+        if (first) out[prefix] = p[i] >> shift;
+    }
+};
+
+template <class Acc, class Src, class Dst, class TotalT>
+int scan_run(Src src, Dst dst, bool apply, uint64_t n, TotalT* total, void* ws, size_t ws_bytes, hipStream_t st) {
+    if (n >= (1ull << 32)) return fail(YX_ERR_RANGE, "scan of %llu items (at most 2^32 - 1)", (unsigned long long)n);
+    if (!ws || ws_bytes < scan_workspace_bytes(n)) return fail(YX_ERR_INVALID_ARG, "scan workspace too small");
+    const uint32_t np = (uint32_t)((n + kPrimScanChunk - 1) / kPrimScanChunk);
+    Acc* partial = (Acc*)ws;
+    if (np) hipLaunchKernelGGL((prim_scan_reduce<Src, Acc>), dim3(np), dim3(kScanThreads), 0, st, src, n, partial);
+    hipLaunchKernelGGL((prim_scan_partials<Acc, TotalT>), dim3(1), dim3(1024), 0, st, partial, np, total);
+    if (apply && np)
+        hipLaunchKernelGGL((prim_scan_apply<Src, Dst, Acc>), dim3(np), dim3(kScanThreads), 0, st, src, dst, n,
+                           (const Acc*)partial);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(YX_ERR_HIP, "scan launch: %s", hipGetErrorString(e));
+    return YX_OK;
+}
+
+// ---- radix sort ----
+
+// Digit histogram of block b's keys [b * 4096, ...) below n, stored
+// digit-major (hist[d * nb + b]); blocks past the end store zeros. Each run
+// of equal digits among a wave's lanes is counted with one LDS add by its
+// first lane (keys that arrive grouped, such as records emitted tile by tile,
+// share digits with their neighbours; per-lane adds to one counter would
+// serialise).
+template <class K>
+__global__ __launch_bounds__(kSortThreads) void prim_rs_upsweep(const K* keys, uint32_t n_host, const uint32_t* n_dev,
+                                                                int shift, int bits, uint32_t* hist, uint32_t nb) {
+    __shared__ uint32_t h[kPrimRadix];
+    for (int d = (int)threadIdx.x; d < kPrimRadix; d += kSortThreads) h[d] = 0u;
+    __syncthreads();
+    const uint32_t n = n_dev ? std::min(*n_dev, n_host) : n_host;
+    const uint32_t mask = (1u << bits) - 1u;
+    const uint32_t base = blockIdx.x * kPrimSortTile + threadIdx.x;
+    const int lane = (int)(threadIdx.x & 63);
+    const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
+    if (blockIdx.x * kPrimSortTile < n) {
+        for (int k = 0; k < kSortRounds; ++k) {
+            const uint32_t i = base + (uint32_t)k * kSortThreads;
+            const bool in = i < n;  // the lanes in range are a prefix of the wave
+            const uint32_t d = in ? (uint32_t)(keys[i] >> shift) & mask : 0u;
+            const uint32_t prev = __shfl_up(d, 1u);
+            const bool head = in && (lane == 0 || prev != d);
+            const uint64_t heads = __ballot(head);
+            const int n_in = __popcll(__ballot(in));
+            if (head) {
+                const uint64_t nxt = heads & above;
+                const int end = nxt ? __ffsll((unsigned long long)nxt) - 1 : n_in;
+                atomicAdd(&h[d], (uint32_t)(end - lane));
+            }
+        }
+    }
+    __syncthreads();
+    for (int d = (int)threadIdx.x; d < kPrimRadix; d += kSortThreads) hist[(size_t)d * nb + blockIdx.x] = h[d];
+}
+
+// Stable scatter of block b's keys (and ids) by digit; `offs` is the scanned
+// histogram. Rounds take 256 consecutive keys; a key's place among the
+// block's keys of its digit is the count from earlier rounds, from earlier
+// waves of its round and from earlier lanes of its wave (ballot rank).
+// IDS with ids_in null: the ids are the positions.
+template <class K, bool IDS>
+__global__ __launch_bounds__(kSortThreads) void prim_rs_downsweep(const K* keys_in, const uint32_t* ids_in,
+                                                                  K* keys_out, uint32_t* ids_out, uint32_t n_host,
+                                                                  const uint32_t* n_dev, int shift, int bits,
+                                                                  const uint32_t* offs, uint32_t nb) {
+    __shared__ uint32_t run[kPrimRadix];
+    __shared__ uint32_t wc_all[2][kSortThreads / 64][kPrimRadix];  // per round parity, per wave
+    const uint32_t n = n_dev ? std::min(*n_dev, n_host) : n_host;
+    if (blockIdx.x * kPrimSortTile >= n) return;  // whole block
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    const uint32_t mask = (1u << bits) - 1u;
+    for (int d = (int)threadIdx.x; d < kPrimRadix; d += kSortThreads) {
+        run[d] = offs[(size_t)d * nb + blockIdx.x];
+        for (int q = 0; q < kSortThreads / 64; ++q) wc_all[0][q][d] = wc_all[1][q][d] = 0u;
+    }
+    __syncthreads();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t base = blockIdx.x * kPrimSortTile + threadIdx.x;
+    for (int k = 0; k < kSortRounds; ++k) {
+        const uint32_t i = base + (uint32_t)k * kSortThreads;
+        const bool in = i < n;
+        K key = 0;
+        uint32_t id = 0, d = 0;
+        if (in) {
+            key = keys_in[i];
+            if (IDS) id = ids_in ? ids_in[i] : i;
+            d = (uint32_t)(key >> shift) & mask;
+        }
+        // the per-wave counts alternate between two buffers by round, so the
+        // next round's counts never meet this round's clearing
+        uint32_t(*wc)[kPrimRadix] = wc_all[k & 1];
+        const uint64_t peers = digit_peers(in, d, bits);
+        if (in && (peers & below) == 0) wc[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        if (in) {
+            uint32_t pos = run[d] + (uint32_t)__popcll(peers & below);
+            for (int q = 0; q < w; ++q) pos += wc[q][d];
+            keys_out[pos] = key;
+            if (IDS) ids_out[pos] = id;
+        }
+        __syncthreads();
+        for (int dd = (int)threadIdx.x; dd < kPrimRadix; dd += kSortThreads) {
+            uint32_t s = 0;
+#pragma unroll
+            for (int q = 0; q < kSortThreads / 64; ++q) {
+                s += wc[q][dd];
+                wc[q][dd] = 0u;
+            }
+            run[dd] += s;
+        }
+    }
+}
+
+uint32_t sort_blocks(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + kPrimSortTile - 1) / kPrimSortTile); }
+
+}  // namespace
+
+size_t scan_workspace_bytes(uint64_t n) { return align256(((n + kPrimScanChunk - 1) / kPrimScanChunk + 1) * 8); }
+
+int scan_excl_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total, void* ws, size_t ws_bytes,
+                  hipStream_t st) {
+    return scan_run<uint32_t>(Load<uint32_t>{in}, StorePrefix<uint32_t>{out}, true, n, total, ws, ws_bytes, st);
+}
+
+int scan_excl_u8(const uint8_t* in, uint32_t* out, uint64_t n, uint32_t* total, void* ws, size_t ws_bytes,
+                 hipStream_t st) {
+    return scan_run<uint32_t>(Load<uint8_t>{in}, StorePrefix<uint32_t>{out}, true, n, total, ws, ws_bytes, st);
+}
+
+int scan_excl_u32_u64(const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total, void* ws, size_t ws_bytes,
+                      hipStream_t st) {
+    return scan_run<uint64_t>(Load<uint32_t>{in}, StorePrefix<uint64_t>{out}, true, n, total, ws, ws_bytes, st);
+}
+
+int reduce_sum_u32(const uint32_t* in, uint64_t n, uint32_t* total, void* ws, size_t ws_bytes, hipStream_t st) {
+    return scan_run<uint32_t>(Load<uint32_t>{in}, NoStore{}, false, n, total, ws, ws_bytes, st);
+}
+
+size_t sort_workspace_bytes(uint64_t n_host) {
+    const uint64_t hist = (uint64_t)kPrimRadix * sort_blocks(n_host);
+    return align256(hist * 4) + scan_workspace_bytes(hist);
+}
+
+template <class K>
+int radix_sort(const K* keys_in, const uint32_t* ids_in, K* const kbuf[2], uint32_t* const ibuf[2], uint64_t n_host,
+               const uint32_t* n_dev, int b0, int b1, const K** kout, const uint32_t** iout, void* ws,
+               size_t ws_bytes, hipStream_t st) {
+    if (n_host >= 0x80000000ull) return fail(YX_ERR_RANGE, "sort of %llu keys (at most 2^31 - 1)",
+                                             (unsigned long long)n_host);
+    if (!ws || ws_bytes < sort_workspace_bytes(n_host)) return fail(YX_ERR_INVALID_ARG, "sort workspace too small");
+    if (b0 < 0 || b1 > (int)(8 * sizeof(K))) return fail(YX_ERR_INVALID_ARG, "sort bits [%d, %d)", b0, b1);
+    const uint32_t nb = sort_blocks(n_host);
+    const size_t nhist = (size_t)kPrimRadix * nb;
+    uint32_t* hist = (uint32_t*)ws;
+    char* scan_ws = (char*)ws + align256(nhist * 4);
+    const size_t scan_bytes = ws_bytes - align256(nhist * 4);
+    const bool ids = ibuf != nullptr;
+    const K* kin = keys_in;
+    const uint32_t* iin = ids_in;
+    int pass = 0;
+    for (int sh = b0; sh < b1; sh += kPrimRadixBits, ++pass) {
+        const int bits = std::min(kPrimRadixBits, b1 - sh);
+        hipLaunchKernelGGL(prim_rs_upsweep<K>, dim3(nb), dim3(kSortThreads), 0, st, kin, (uint32_t)n_host, n_dev, sh,
+                           bits, hist, nb);
+        const int rc = scan_excl_u32(hist, hist, nhist, nullptr, scan_ws, scan_bytes, st);
+        if (rc != YX_OK) return rc;
+        if (ids)
+            hipLaunchKernelGGL((prim_rs_downsweep<K, true>), dim3(nb), dim3(kSortThreads), 0, st, kin, iin,
+                               kbuf[pass & 1], ibuf[pass & 1], (uint32_t)n_host, n_dev, sh, bits,
+                               (const uint32_t*)hist, nb);
+        else
+            hipLaunchKernelGGL((prim_rs_downsweep<K, false>), dim3(nb), dim3(kSortThreads), 0, st, kin,
+                               (const uint32_t*)nullptr, kbuf[pass & 1], (uint32_t*)nullptr, (uint32_t)n_host, n_dev,
+                               sh, bits, (const uint32_t*)hist, nb);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(YX_ERR_HIP, "radix sort launch: %s", hipGetErrorString(e));
+        kin = kbuf[pass & 1];
+        iin = ids ? ibuf[pass & 1] : nullptr;
+    }
+    *kout = kin;
+    if (iout) *iout = iin;
+    return YX_OK;
+}
+
+template int radix_sort<uint32_t>(const uint32_t*, const uint32_t*, uint32_t* const[2], uint32_t* const[2], uint64_t,
+                                  const uint32_t*, int, int, const uint32_t**, const uint32_t**, void*, size_t,
+                                  hipStream_t);
+template int radix_sort<uint64_t>(const uint64_t*, const uint32_t*, uint64_t* const[2], uint32_t* const[2], uint64_t,
+                                  const uint32_t*, int, int, const uint64_t**, const uint32_t**, void*, size_t,
+                                  hipStream_t);
+
+size_t distinct_workspace_bytes(uint64_t n) { return scan_workspace_bytes(n); }
+
+int distinct_shifted_u64(const uint64_t* in, uint64_t* out, uint64_t n, int shift, uint64_t* count, void* ws,
+                         size_t ws_bytes, hipStream_t st) {
+    return scan_run<uint32_t>(FirstOfRun{in, shift}, StoreFirst{in, out, shift}, true, n, count, ws, ws_bytes, st);
+}
+
+}  // namespace yx

@@ -14,13 +14,13 @@
 // bottom-up sweep sums leaf colours into the per-record mean colour. Integer
 // work throughout: the result equals the host re-layout bit for bit.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <new>
 #include <vector>
 
 #include "internal.h"
+#include "prims.h"
 
 #define RL_HIP(call)                                                                           \
     do {                                                                                       \
@@ -254,9 +254,9 @@ int gpu_relayout(const uint32_t* d_vox, uint32_t nn, uint32_t root, void* stream
             }
         }
         w->cap = nn;
-        size_t bytes = 0;
-        RL_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, w->nrec, w->roff, (int)nn, st));
-        if ((rc = rl_grow((uint8_t**)&w->scan_tmp, &w->scan_bytes, bytes, "scan scratch")) != YX_OK) return rc;
+        if ((rc = rl_grow((uint8_t**)&w->scan_tmp, &w->scan_bytes, scan_workspace_bytes(nn), "scan scratch")) !=
+            YX_OK)
+            return rc;
     }
     if (!w->flags) {
         size_t c = 0;
@@ -289,8 +289,9 @@ int gpu_relayout(const uint32_t* d_vox, uint32_t nn, uint32_t root, void* stream
         hipLaunchKernelGGL(rl_count_kernel, rl_grid(n), dim3(256), 0, st, d_vox, nn, lv[l].ids, n, w->nrec, w->nnode,
                            w->seen, w->flags);
         RL_HIP(hipGetLastError());
-        RL_HIP(hipcub::DeviceScan::ExclusiveSum(w->scan_tmp, w->scan_bytes, w->nrec, lv[l].roff, (int)n, st));
-        RL_HIP(hipcub::DeviceScan::ExclusiveSum(w->scan_tmp, w->scan_bytes, w->nnode, lv[l].noff, (int)n, st));
+        if ((rc = scan_excl_u32(w->nrec, lv[l].roff, n, nullptr, w->scan_tmp, w->scan_bytes, st)) != YX_OK ||
+            (rc = scan_excl_u32(w->nnode, lv[l].noff, n, nullptr, w->scan_tmp, w->scan_bytes, st)) != YX_OK)
+            return rc;
         hipLaunchKernelGGL(rl_tail_kernel, dim3(1), dim3(64), 0, st, lv[l].roff, w->nrec, lv[l].noff, w->nnode, n,
                            w->flags, w->tail);
         RL_HIP(hipGetLastError());

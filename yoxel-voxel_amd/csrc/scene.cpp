@@ -1944,6 +1944,68 @@ extern "C" int yx_scene_nodes(const yx_scene* s, const uint32_t** nodes, uint32_
     return YX_OK;
 }
 
+// Leaf payload decoding of a dumped frame (the host half of
+// SVORenderer::DumpTraceData, demo/SVORenderer.cpp:158-192: UnpackVoxData /
+// UnpackColor / UnpackNormal, whose bodies the reference does not ship; the
+// colour16 + normal16 layout is the one the kernels shade with).
+static uint32_t host_rgba_from_565(uint32_t c) {
+    const uint32_t r5 = (c >> 11) & 31u, g6 = (c >> 5) & 63u, b5 = c & 31u;
+    const uint32_t r = (r5 << 3) | (r5 >> 2), g = (g6 << 2) | (g6 >> 4), b = (b5 << 3) | (b5 >> 2);
+    return r | (g << 8) | (b << 16) | 0xff000000u;
+}
+
+static void host_normal_from_16(uint32_t q, float* n) {
+    int u = (int)((q >> 7) & 63u), v = (int)(q & 127u);
+    if (u + v > 126) {
+        u = 127 - u;
+        v = 127 - v;
+    }
+    const float x = (float)u, y = (float)v, z = (float)(126 - u - v);
+    const float len = std::sqrt(x * x + y * y + z * z);
+    n[0] = (q & 0x8000u) ? -(x / len) : x / len;
+    n[1] = (q & 0x4000u) ? -(y / len) : y / len;
+    n[2] = (q & 0x2000u) ? -(z / len) : z / len;
+}
+
+extern "C" int yx_scene_decode_hits(const yx_scene* s, const uint32_t* hit_id, uint64_t n, int32_t payload,
+                                    uint32_t* color, float* normal) {
+    clear_error();
+    if (!s || (!hit_id && n)) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (payload != YX_PAYLOAD_RGBA8 && payload != YX_PAYLOAD_C16N16)
+        return fail(YX_ERR_INVALID_ARG, "unknown payload format %d", payload);
+    const uint64_t num_nodes = s->nodes.size() / 10;
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint32_t h = hit_id[i];
+        uint32_t c = 0;
+        float nn[3] = {0.0f, 0.0f, 0.0f};
+        if (h != YX_MISS) {
+            const uint64_t node = h >> 3;
+            const uint32_t slot = h & 7u;
+            if (node >= num_nodes)
+                return fail(YX_ERR_FORMAT, "pixel %llu: node %llu beyond the %llu nodes", (unsigned long long)i,
+                            (unsigned long long)node, (unsigned long long)num_nodes);
+            const uint32_t* rec = s->nodes.data() + node * 10;
+            if (!((rec[0] >> slot) & 1u))
+                return fail(YX_ERR_FORMAT, "pixel %llu: child %u of node %llu is not a leaf", (unsigned long long)i,
+                            slot, (unsigned long long)node);
+            const uint32_t leaf = rec[2 + slot];
+            if (payload == YX_PAYLOAD_C16N16) {
+                c = host_rgba_from_565(leaf & 0xffffu);
+                host_normal_from_16(leaf >> 16, nn);
+            } else {
+                c = leaf;
+            }
+        }
+        if (color) color[i] = c;
+        if (normal) {
+            normal[3 * i] = nn[0];
+            normal[3 * i + 1] = nn[1];
+            normal[3 * i + 2] = nn[2];
+        }
+    }
+    return YX_OK;
+}
+
 extern "C" void yx_scene_free(yx_scene* s) { delete s; }
 
 extern "C" const char* yx_last_error(void) { return g_error.c_str(); }

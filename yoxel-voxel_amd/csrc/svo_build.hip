@@ -18,7 +18,6 @@
 //   4. one thread per node: its children are a contiguous run of the level
 //      below (binary search), written as child ids or leaf colours.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -27,6 +26,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "prims.h"
 
 namespace yx {
 
@@ -408,13 +408,6 @@ __device__ __forceinline__ uint32_t sb_edited_color(uint64_t code, uint64_t seed
     return sb_leaf_color(code, seed);
 }
 
-struct Shr3 {
-    __host__ __device__ uint64_t operator()(uint64_t v) const { return v >> 3; }
-};
-struct ToU64 {
-    __host__ __device__ uint64_t operator()(uint32_t v) const { return v; }
-};
-
 // 4. node k of a level: children are the run of `below` whose parent code
 // (>> 3) equals this node's code; child ids (internal levels) or leaf colours
 // (the last node level, below = leaf codes).
@@ -579,24 +572,18 @@ static int gpu_build(const yx_scene_desc* d, const EditSet& edits, yx_scene* s, 
                        (const uint64_t*)nullptr, d_counts, (uint64_t*)nullptr);
     YX_HIP(hipGetLastError());
     {
-        hipcub::TransformInputIterator<uint64_t, ToU64, const uint32_t*> in(d_counts, ToU64());
-        size_t bytes = 0;
-        YX_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, d_offs, (int)(ncol + 1), st));
+        const size_t bytes = scan_workspace_bytes(ncol);
         void* tmp = nullptr;
         if ((rc = B.alloc((uint8_t**)&tmp, bytes)) != YX_OK) return rc;
-        // counts[ncol] is read past the end by the +1 scan: scan ncol items,
-        // then the total is offs[ncol-1] + counts[ncol-1]
-        YX_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, d_offs, (int)ncol, st));
+        // d_offs[ncol] receives the total
+        if ((rc = scan_excl_u32_u64(d_counts, d_offs, ncol, d_offs + ncol, tmp, bytes, st)) != YX_OK) return rc;
     }
     int over = 0;
-    uint64_t last_off = 0;
-    uint32_t last_cnt = 0;
+    uint64_t total = 0;
     YX_HIP(hipMemcpyAsync(&over, d_over, sizeof(int), hipMemcpyDeviceToHost, st));
-    YX_HIP(hipMemcpyAsync(&last_off, d_offs + ncol - 1, 8, hipMemcpyDeviceToHost, st));
-    YX_HIP(hipMemcpyAsync(&last_cnt, d_counts + ncol - 1, 4, hipMemcpyDeviceToHost, st));
+    YX_HIP(hipMemcpyAsync(&total, d_offs + ncol, 8, hipMemcpyDeviceToHost, st));
     YX_HIP(hipStreamSynchronize(st));
     if (over) return fail(YX_ERR_RANGE, "a column holds more than %d inside intervals", kColMax);
-    const uint64_t total = last_off + last_cnt;
 
     s->num_leaves = total;
     if (total == 0) {
@@ -609,42 +596,39 @@ static int gpu_build(const yx_scene_desc* d, const EditSet& edits, yx_scene* s, 
     // level holds at most `total` codes: 2^31 - 1 bounds them all
     if (total >= 0x7FFFFFFFull)
         return fail(YX_ERR_RANGE, "%llu surface voxels (at most 2^31 - 2)", (unsigned long long)total);
-    uint64_t *d_codes = nullptr, *d_sorted = nullptr;
-    if ((rc = B.alloc(&d_codes, total)) != YX_OK || (rc = B.alloc(&d_sorted, total)) != YX_OK) return rc;
+    uint64_t *d_codes = nullptr, *d_sorted = nullptr, *d_swap = nullptr;
+    if ((rc = B.alloc(&d_codes, total)) != YX_OK || (rc = B.alloc(&d_sorted, total)) != YX_OK ||
+        (rc = B.alloc(&d_swap, total)) != YX_OK)
+        return rc;
     hipLaunchKernelGGL(sb_surface_kernel, dim3((unsigned)((ncol + 127) / 128)), dim3(128), 0, st, a, d_offs,
                        d_counts, d_codes);
     if (hipGetLastError() != hipSuccess) return fail(YX_ERR_HIP, "surface kernel launch");
+    const uint64_t* leaf_codes = nullptr;
     {
-        size_t bytes = 0;
-        if (hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, d_codes, d_sorted, (int)total, 0, 3 * D, st) !=
-            hipSuccess)
-            return fail(YX_ERR_HIP, "radix sort sizing");
+        const size_t bytes = sort_workspace_bytes(total);
         void* tmp = nullptr;
         if ((rc = B.alloc((uint8_t**)&tmp, bytes)) != YX_OK) return rc;
-        if (hipcub::DeviceRadixSort::SortKeys(tmp, bytes, d_codes, d_sorted, (int)total, 0, 3 * D, st) !=
-            hipSuccess)
-            return fail(YX_ERR_HIP, "radix sort");
+        uint64_t* const kbuf[2] = {d_sorted, d_swap};
+        if ((rc = radix_sort<uint64_t>(d_codes, nullptr, kbuf, nullptr, total, nullptr, 0, 3 * D, &leaf_codes, nullptr,
+                                       tmp, bytes, st)) != YX_OK)
+            return rc;
     }
 
     // 3. levels: lv[L] = distinct (lv[L + 1] >> 3), lv[D] = leaf codes
     std::vector<uint64_t*> lv((size_t)D + 1, nullptr);
     std::vector<uint64_t> cnt((size_t)D + 1, 0);
-    lv[(size_t)D] = d_sorted;
+    lv[(size_t)D] = const_cast<uint64_t*>(leaf_codes);
     cnt[(size_t)D] = total;
     uint64_t* d_nsel = nullptr;
     if ((rc = B.alloc(&d_nsel, 1)) != YX_OK) return rc;
+    void* uniq_ws = nullptr;
+    const size_t uniq_bytes = distinct_workspace_bytes(total);
+    if ((rc = B.alloc((uint8_t**)&uniq_ws, uniq_bytes)) != YX_OK) return rc;
     for (int L = D - 1; L >= 0; --L) {
         if ((rc = B.alloc(&lv[(size_t)L], cnt[(size_t)L + 1])) != YX_OK) return rc;
-        hipcub::TransformInputIterator<uint64_t, Shr3, const uint64_t*> in(lv[(size_t)L + 1], Shr3());
-        size_t bytes = 0;
-        if (hipcub::DeviceSelect::Unique(nullptr, bytes, in, lv[(size_t)L], d_nsel, (int)cnt[(size_t)L + 1], st) !=
-            hipSuccess)
-            return fail(YX_ERR_HIP, "unique sizing");
-        void* tmp = nullptr;
-        if ((rc = B.alloc((uint8_t**)&tmp, bytes)) != YX_OK) return rc;
-        if (hipcub::DeviceSelect::Unique(tmp, bytes, in, lv[(size_t)L], d_nsel, (int)cnt[(size_t)L + 1], st) !=
-            hipSuccess)
-            return fail(YX_ERR_HIP, "unique");
+        if ((rc = distinct_shifted_u64(lv[(size_t)L + 1], lv[(size_t)L], cnt[(size_t)L + 1], 3, d_nsel, uniq_ws,
+                                       uniq_bytes, st)) != YX_OK)
+            return rc;
         uint64_t m = 0;
         if (hipMemcpyAsync(&m, d_nsel, 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)

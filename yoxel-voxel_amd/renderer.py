@@ -150,6 +150,18 @@ class SVOData:
         st = ctypes.c_void_p(_stream(stream) if how == _lib.YX_RANGE_REBUILD_GPU else 0)
         check(lib().yx_scene_build_range_volume(self._h, ctypes.byref(src), int(how), st))
 
+    def decode_hits(self, hit_id, payload=_lib.YX_PAYLOAD_RGBA8):
+        """Colour (uint32 RGBA8) and normal (float32 x 3) of the leaf each
+        hit_id names, zeros for misses (the decoding of
+        SVORenderer::DumpTraceData, demo/SVORenderer.cpp:158-192)."""
+        ids = np.ascontiguousarray(np.asarray(hit_id).reshape(-1), dtype=np.uint32)
+        color = np.zeros(ids.shape[0], dtype=np.uint32)
+        normal = np.zeros((ids.shape[0], 3), dtype=np.float32)
+        check(lib().yx_scene_decode_hits(self._h, ids.ctypes.data_as(ctypes.c_void_p), ids.shape[0], int(payload),
+                                         color.ctypes.data_as(ctypes.c_void_p),
+                                         normal.ctypes.data_as(ctypes.c_void_p)))
+        return color, normal
+
     def info(self):
         i = _lib.SceneInfo()
         check(lib().yx_scene_get_info(self._h, ctypes.byref(i)))
@@ -194,10 +206,12 @@ class SVORenderer:
     def SetScene(self, svo):
         """Upload `svo` and lay it out on the GPU (waits for it)."""
         check(lib().yx_renderer_set_scene(self._h, svo.handle))
+        self._svo = svo
 
     def SetSceneHost(self, svo):
         """The same layout computed on the host (bit-identical, slower)."""
         check(lib().yx_renderer_set_scene_host(self._h, svo.handle))
+        self._svo = svo
 
     def SetViewPos(self, pos):
         check(lib().yx_renderer_set_view_pos(self._h, f3(pos)))
@@ -229,6 +243,29 @@ class SVORenderer:
         """YX_PAYLOAD_RGBA8 (default) or YX_PAYLOAD_C16N16 (the reference's
         colour16 + normal16 VoxData, layout in include/yoxel.h)."""
         check(lib().yx_renderer_set_payload(self._h, int(fmt)))
+        self._payload = int(fmt)
+
+    def DumpTraceData(self, fnbase):
+        """SVORenderer::DumpTraceData (demo/SVORenderer.cpp:158-192): trace
+        the current view and write `{fnbase}_{w}x{h}.dist` (float32 per pixel,
+        zeros as in the reference), `.color` (RGBA8 per pixel) and `.normal`
+        (3 float32 per pixel) of the hit leaves, zeros for misses. The frame is
+        the one the current camera renders (rendering is deterministic), so
+        it equals the last frame when the camera has not moved since."""
+        svo = getattr(self, "_svo", None)
+        if svo is None:
+            raise YoxelError(_lib.YX_ERR_NO_SCENE, "DumpTraceData needs a scene set with SetScene")
+        import torch
+        hit = self.render_tensors(("hit_id",))["hit_id"]
+        torch.cuda.current_stream().synchronize()
+        ids = hit.cpu().numpy().view(np.uint32)
+        color, normal = svo.decode_hits(ids, getattr(self, "_payload", _lib.YX_PAYLOAD_RGBA8))
+        w, h = self.GetResolution()
+        base = f"{fnbase}_{w}x{h}"
+        np.zeros(w * h, dtype=np.float32).tofile(base + ".dist")
+        color.tofile(base + ".color")
+        normal.tofile(base + ".normal")
+        return base
 
     def get_tuning(self):
         t = _lib.Tuning()
@@ -287,6 +324,7 @@ class SVORenderer:
         """Upload the pages of `svo` edited since the last sync (all of them for
         another scene) and rebuild the traversal layout on the GPU."""
         check(lib().yx_renderer_update_scene(self._h, svo.handle, ctypes.c_void_p(_stream(stream))))
+        self._svo = svo
 
     def CountChangedPages(self, svo):
         """(pages, bytes) the next UpdateScene would upload
