@@ -481,8 +481,10 @@ typedef struct yx_volume_dev {
     uint32_t num_slots;
 } yx_volume_dev;
 
-/* Holds the backward's workspace: calls on one compositor must not overlap
- * (use one stream, or one compositor per stream). */
+/* Holds the backward's workspace and a summary of the brick table (one byte
+ * per 4^3 bricks, rebuilt by every forward and backward; allocated on the
+ * first call and whenever the grid grows): calls on one compositor must not
+ * overlap (use one stream, or one compositor per stream). */
 typedef struct yx_compositor yx_compositor;
 int yx_compositor_create(yx_compositor** out);
 void yx_compositor_destroy(yx_compositor* c);

@@ -50,9 +50,11 @@ struct CompArgs {
     const int32_t* table;
     const float4* params;
     const float* views;
+    const uint8_t* super;  // per super-brick of 2^ls bricks per axis: 1 when any of its bricks is allocated
     uint32_t num_slots;
     int n, lb, g;  // grid, log2 brick, bricks per axis
     int lg;        // log2 of g
+    int ls;        // log2 of the super-brick edge in bricks (min(2, lg))
     int n_views, w, h;
     float* rgb;
     float* trans;
@@ -132,6 +134,8 @@ struct SampleIter {
 
     BrickId cb;  // brick of the last table lookup (all ones / -1: none)
     int cslot;   // its slot
+    uint32_t csb;  // super-brick of the last summary lookup (all ones: none)
+    uint32_t cs;   // its summary byte (0: every brick in it is empty)
 
     // Advance by one step without loading the sample's parameters: returns
     // true with the parameter index of a sample, false after a skip or at the
@@ -156,24 +160,37 @@ struct SampleIter {
             bl = ((int64_t)bz * a.g + by) * a.g + bx;
         if (bl != cb) {  // consecutive samples mostly share a brick
             cb = bl;
-            cslot = a.table[bl];
+            // the super-brick summary first (a small array that stays in the
+            // cache): inside an empty super-brick the table is not read
+            const int gs_l = a.lg - a.ls;
+            const uint32_t sb = ((uint32_t)(bz >> a.ls) << (2 * gs_l)) | ((uint32_t)(by >> a.ls) << gs_l) |
+                                (uint32_t)(bx >> a.ls);
+            if (sb != csb) {
+                csb = sb;
+                cs = a.super[sb];
+            }
+            cslot = cs ? a.table[bl] : -1;
         }
         const int slot = cslot;
         if ((uint32_t)slot >= a.num_slots) {  // -1 (and anything out of range) is empty
-            // jump to the last sample still in this brick (exact: the candidate
-            // is checked and moved back while it left the brick)
-            const float bsz = (float)(1 << a.lb) * inv_n;
-            const float ex = ((r.dx > 0.0f ? (float)(bx + 1) : (float)bx) * bsz - r.ox) * r.rx;
-            const float ey = ((r.dy > 0.0f ? (float)(by + 1) : (float)by) * bsz - r.oy) * r.ry;
-            const float ez = ((r.dz > 0.0f ? (float)(bz + 1) : (float)bz) * bsz - r.oz) * r.rz;
+            // jump to the last sample still in this brick, or in its whole
+            // super-brick when that is empty (exact: the candidate is checked
+            // and moved back while it left the box; the samples inside a box
+            // are consecutive, since each voxel coordinate is monotonic in k)
+            const int sh = cs ? a.lb : a.lb + a.ls;  // log2 of the box edge in voxels
+            const int cx = vx >> sh, cy = vy >> sh, cz = vz >> sh;
+            const float bsz = (float)(1 << sh) * inv_n;
+            const float ex = ((r.dx > 0.0f ? (float)(cx + 1) : (float)cx) * bsz - r.ox) * r.rx;
+            const float ey = ((r.dy > 0.0f ? (float)(cy + 1) : (float)cy) * bsz - r.oy) * r.ry;
+            const float ez = ((r.dz > 0.0f ? (float)(cz + 1) : (float)cz) * bsz - r.oz) * r.rz;
             const float tb = fminf(ex, fminf(ey, ez));
             int kk = (int)ceilf(tb * nf) - 1;
             kk = min(max(kk, k), r.kend - 1);
             while (kk > k) {
                 const float t2 = (float)kk * inv_n;
-                const int wx = vox(r.ox, r.dx, t2, nf, a.n) >> a.lb, wy = vox(r.oy, r.dy, t2, nf, a.n) >> a.lb,
-                          wz = vox(r.oz, r.dz, t2, nf, a.n) >> a.lb;
-                if (wx == bx && wy == by && wz == bz) break;
+                const int wx = vox(r.ox, r.dx, t2, nf, a.n) >> sh, wy = vox(r.oy, r.dy, t2, nf, a.n) >> sh,
+                          wz = vox(r.oz, r.dz, t2, nf, a.n) >> sh;
+                if (wx == cx && wy == cy && wz == cz) break;
                 --kk;
             }
             k = kk + 1;
@@ -190,6 +207,8 @@ struct SampleIter {
         k = r.k;
         cb = (BrickId)-1;
         cslot = -1;
+        csb = ~0u;
+        cs = 1;
     }
 };
 
@@ -200,11 +219,17 @@ constexpr int kCompBatch = 4;
 // alpha = 1 - exp(-sigma) = sigmoid(raw) and exp(-sigma) = sigmoid(-raw).
 // With n = exp(-|raw|) in (0, 1] and r = 1 / (1 + n) = sigmoid(|raw|), one
 // is r and the other n * r, so a sample costs one exponential and one
-// division (no log1p / expm1, no cancellation for either sign of raw).
+// reciprocal (no log1p / expm1, no cancellation for either sign of raw).
 // alpha is also the chain factor dsigma/draw = sigmoid(raw) of the adjoint.
+// Both are the hardware instructions (v_exp_f32 on -|raw| log2 e, v_rcp_f32
+// on 1 + n in [1, 2], 1 ulp each): n carries a relative error below
+// |raw| * 1e-7 from the rounded product, far inside the compositor's error
+// budget against the FP64 model, and the forward and the adjoint share this
+// function, so the adjoint's recomputed transmittance is the forward's to
+// the bit.
 __device__ __forceinline__ void opacity(float raw, float& alpha, float& keep) {
-    const float n = expf(-fabsf(raw));
-    const float r = 1.0f / (1.0f + n);
+    const float n = __builtin_amdgcn_exp2f(-fabsf(raw) * 1.44269504f);
+    const float r = __builtin_amdgcn_rcpf(1.0f + n);
     const float nr = n * r;
     alpha = raw >= 0.0f ? r : nr;
     keep = raw >= 0.0f ? nr : r;
@@ -227,6 +252,39 @@ __device__ __forceinline__ bool ray_of(const CompArgs& a, size_t t, int& view, i
     if (x >= a.w || y >= a.h) return false;
     ray = ((size_t)view * a.h + y) * a.w + x;
     return true;
+}
+
+// Summary byte per super-brick (2^ls bricks per axis): 1 when any of its
+// bricks holds a slot below num_slots. With 4^3 bricks per super-brick
+// (ls = 2) a wave covers one super-brick, a lane per brick, and votes;
+// smaller grids loop over the bricks in one thread.
+__global__ __launch_bounds__(256) void comp_super_kernel(const int32_t* table, uint32_t num_slots, int lg, int ls,
+                                                         uint8_t* super) {
+    const int gs_l = lg - ls;
+    const uint32_t nsb = 1u << (3 * gs_l);
+    const uint64_t t = (uint64_t)blockIdx.x * 256u + threadIdx.x;
+    const uint32_t gmask = (1u << gs_l) - 1u;
+    if (ls == 2) {
+        if ((t >> 6) >= nsb) return;  // whole waves
+        const uint32_t i = (uint32_t)(t >> 6), l = (uint32_t)(t & 63u);  // super-brick, brick within it
+        const uint32_t sx = i & gmask, sy = (i >> gs_l) & gmask, sz = i >> (2 * gs_l);
+        const uint64_t b = ((((uint64_t)(sz * 4 + (l >> 4))) << lg | (sy * 4 + ((l >> 2) & 3u))) << lg) |
+                           (sx * 4 + (l & 3u));
+        const uint64_t any = __ballot((uint32_t)table[b] < num_slots);
+        if (l == 0) super[i] = any ? 1 : 0;
+        return;
+    }
+    if (t >= nsb) return;
+    const uint32_t sx = (uint32_t)t & gmask, sy = ((uint32_t)t >> gs_l) & gmask, sz = (uint32_t)t >> (2 * gs_l);
+    const uint32_t e = 1u << ls;
+    uint32_t any = 0;
+    for (uint32_t z = 0; z < e; ++z)
+        for (uint32_t y = 0; y < e; ++y)
+            for (uint32_t x = 0; x < e; ++x) {
+                const uint64_t b = ((((uint64_t)(sz * e + z)) << lg | (sy * e + y)) << lg) | (sx * e + x);
+                any |= (uint32_t)table[b] < num_slots ? 1u : 0u;
+            }
+    super[t] = (uint8_t)any;
 }
 
 template <bool SMALL>
@@ -589,6 +647,8 @@ struct yx_compositor {
     const uint32_t* sorted_keys = nullptr;  // the last backward's sorted keys (workspace)
     bool counted = false;  // the last backward's slot reduction counted its distinct voxels
     bool ran = false;      // a backward has run since the workspace was laid out
+    uint8_t* super = nullptr;  // super-brick summary (rebuilt by every call)
+    size_t super_bytes = 0;
 };
 
 static int check_vol(const yx_volume_dev* v, int32_t n_views, int32_t w, int32_t h) {
@@ -615,6 +675,7 @@ static CompArgs make_args(const yx_volume_dev* v, const float* views, int32_t n_
     a.lb = __builtin_ctz((unsigned)v->brick);
     a.g = v->grid / v->brick;
     a.lg = __builtin_ctz((unsigned)a.g);
+    a.ls = a.lg < 2 ? a.lg : 2;
     a.n_views = n_views;
     a.w = w;
     a.h = h;
@@ -645,6 +706,7 @@ extern "C" void yx_compositor_destroy(yx_compositor* c) {
     if (c->ws) (void)hipFree(c->ws);
     if (c->h_misc) (void)hipHostFree(c->h_misc);
     if (c->d_probe) (void)hipFree(c->d_probe);
+    if (c->super) (void)hipFree(c->super);
     delete c;
 }
 
@@ -687,6 +749,29 @@ extern "C" int yx_compositor_last_stats(const yx_compositor* c, uint64_t* record
     return YX_OK;
 }
 
+// The super-brick summary of the volume's current table, rebuilt on `st`
+// before every walk (the table may have changed since the last call).
+static int build_super(yx_compositor* c, CompArgs& a, hipStream_t st) {
+    const size_t nsb = (size_t)1 << (3 * (a.lg - a.ls));
+    if (c->super_bytes < nsb) {
+        if (c->super) YX_HIP(hipFree(c->super));
+        c->super = nullptr;
+        c->super_bytes = 0;
+        const hipError_t e = hipMalloc((void**)&c->super, nsb);
+        if (e != hipSuccess) {
+            c->super = nullptr;
+            return fail(YX_ERR_OOM, "hipMalloc(%zu) for the super-brick summary: %s", nsb, hipGetErrorString(e));
+        }
+        c->super_bytes = nsb;
+    }
+    const size_t threads = a.ls == 2 ? nsb * 64 : nsb;
+    hipLaunchKernelGGL(comp_super_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, a.table,
+                       a.num_slots, a.lg, a.ls, c->super);
+    YX_HIP(hipGetLastError());
+    a.super = c->super;
+    return YX_OK;
+}
+
 extern "C" int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, const float* views, int32_t n_views,
                                     int32_t w, int32_t h, float* rgb, float* trans, uint32_t* nsamp, void* stream) {
     clear_error();
@@ -697,6 +782,8 @@ extern "C" int yx_composite_forward(yx_compositor* c, const yx_volume_dev* vol, 
     a.rgb = rgb;
     a.trans = trans;
     a.nsamp = nsamp;
+    rc = build_super(c, a, (hipStream_t)stream);
+    if (rc != YX_OK) return rc;
     const size_t waves = (size_t)n_views * ((w + 7) / 8) * ((h + 7) / 8);
     const dim3 grid((unsigned)((waves * 64 + 255) / 256)), block(256);
     if (a.g <= 1024)
@@ -863,6 +950,8 @@ extern "C" int yx_composite_backward(yx_compositor* c, const yx_volume_dev* vol,
     const int end_bit = std::max(key_bits, begin_bit + 1);
 
     // 2. emit records (one wave = one 8x8 ray tile) at their offsets
+    rc = build_super(c, a, st);
+    if (rc != YX_OK) return rc;
     a.rgb_in = rgb;
     a.grad_rgb = grad_rgb;
     a.offsets = offs;

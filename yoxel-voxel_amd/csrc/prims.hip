@@ -81,6 +81,31 @@ __global__ __launch_bounds__(kScanThreads) void prim_scan_apply(Src src, Dst dst
     }
 }
 
+// Scans of at most kScanSingleMax items run as one 1024-thread block (a strip
+// of at most 8 items per thread: sums, one block scan, rescans with the carry)
+// instead of three launches: small scans are bound by launch and barrier
+// latency, not bandwidth.
+constexpr uint32_t kScanSingleMax = 8u * 1024u;
+
+template <class Src, class Dst, class Acc, class TotalT>
+__global__ __launch_bounds__(1024) void prim_scan_single(Src src, Dst dst, uint32_t n, bool apply, TotalT* total) {
+    __shared__ Acc shw[1024 / 64];
+    const uint32_t per = (n + 1023u) / 1024u;
+    const uint32_t b = std::min(threadIdx.x * per, n), e = std::min(b + per, n);
+    Acc s = 0;
+    for (uint32_t i = b; i < e; ++i) s += (Acc)src(i);
+    Acc tot;
+    Acc run = block_excl_scan<Acc, 1024>(s, shw, tot);
+    if (apply) {
+        for (uint32_t i = b; i < e; ++i) {
+            const Acc v = (Acc)src(i);
+            dst(i, run, v);
+            run += v;
+        }
+    }
+    if (threadIdx.x == 0 && total) *total = (TotalT)tot;
+}
+
 template <class T>
 struct Load {
     const T* p;
@@ -120,6 +145,13 @@ template <class Acc, class Src, class Dst, class TotalT>
 int scan_run(Src src, Dst dst, bool apply, uint64_t n, TotalT* total, void* ws, size_t ws_bytes, hipStream_t st) {
     if (n >= (1ull << 32)) return fail(YX_ERR_RANGE, "scan of %llu items (at most 2^32 - 1)", (unsigned long long)n);
     if (!ws || ws_bytes < scan_workspace_bytes(n)) return fail(YX_ERR_INVALID_ARG, "scan workspace too small");
+    if (n <= kScanSingleMax) {
+        hipLaunchKernelGGL((prim_scan_single<Src, Dst, Acc, TotalT>), dim3(1), dim3(1024), 0, st, src, dst,
+                           (uint32_t)n, apply, total);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return fail(YX_ERR_HIP, "scan launch: %s", hipGetErrorString(e));
+        return YX_OK;
+    }
     const uint32_t np = (uint32_t)((n + kPrimScanChunk - 1) / kPrimScanChunk);
     Acc* partial = (Acc*)ws;
     if (np) hipLaunchKernelGGL((prim_scan_reduce<Src, Acc>), dim3(np), dim3(kScanThreads), 0, st, src, n, partial);
