@@ -235,7 +235,8 @@ int yx_renderer_set_light(yx_renderer* r, const float dir[3], int32_t mode);
  *     u + v <= 126 (points with u >= 64 folded to 127 - u, 127 - v); normal =
  *     normalize(u, v, 126 - u - v) with the signs. Lambert over that normal;
  *     shadow rays as for RGBA8.
- * The shading pipeline (yx_render_pipeline) and batches read RGBA8. */
+ * Batches take the format from yx_batch_set_payload; the shading pipeline
+ * (yx_render_pipeline) reads RGBA8. */
 #define YX_PAYLOAD_RGBA8 0
 #define YX_PAYLOAD_C16N16 1
 int yx_renderer_set_payload(yx_renderer* r, int32_t format);
@@ -432,6 +433,10 @@ int yx_render_batch(yx_batch* b, const yx_outputs* out, void* stream);
 int yx_batch_autotune(yx_batch* b, const yx_outputs* out, int32_t reps, yx_tune_trial* trials, int32_t cap,
                       int32_t* n_trials, void* stream);
 int yx_batch_scene_bytes(const yx_batch* b, uint64_t* bytes);
+/* How the leaf payloads of every scene of the batch are read (YX_PAYLOAD_*,
+ * as yx_renderer_set_payload; default RGBA8). No counterpart in the
+ * reference, whose renderer handles one scene at a time. */
+int yx_batch_set_payload(yx_batch* b, int32_t format);
 
 /* ---- differentiable front-to-back compositing over brick voxels ----------
  * Forward model: reaction/trace/trace.cu:42-139 (castRay / walkBrick: brick

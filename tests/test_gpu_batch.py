@@ -236,3 +236,40 @@ def test_renderer_autotune(yx, orc, cuda):
     o = orc.render(s.nodes(), s.GetRoot(), pos, d, fov=60.0, w=512, h=512, light_mode=2, light=yx.workloads.SUN,
                    threads=8)
     assert_same(to_np(out), o, what=f"autotuned {best}")
+
+
+def test_batch_c16n16_payloads(yx, orc, cuda):
+    """yx_batch_set_payload: scenes whose leaves hold colour16 + normal16 render
+    in one batch as each job does alone with the oracle in that format, in
+    every light mode; switching the format back re-captures the graph and
+    reads the same bytes as RGBA8."""
+    from test_payload_c16n16 import payload
+    scenes, jobs = small_batch(yx)
+    conv = []
+    for k, s in enumerate(scenes):
+        nodes = s.nodes().copy()
+        leaf = ((nodes[:, 0:1] >> np.arange(8, dtype=np.uint32)) & 1).astype(bool)
+        if leaf.any():
+            nrm = np.random.default_rng(40 + k).normal(size=(int(leaf.sum()), 3))
+            nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+            sub = nodes[:, 2:].copy()
+            sub[leaf] = payload(sub[leaf], nrm)
+            nodes[:, 2:] = sub
+        conv.append(yx.SVOData.from_nodes(nodes, s.GetRoot()))
+    b = yx.BatchRenderer()
+    b.set_scenes(conv)
+    b.set_jobs(jobs)
+    for fmt in (yx.YX_PAYLOAD_C16N16, yx.YX_PAYLOAD_RGBA8, yx.YX_PAYLOAD_C16N16):
+        b.set_payload(fmt)
+        print(f"batch payload {fmt}", flush=True)
+        out = b.render_tensors(KEYS)
+        cuda.cuda.synchronize()
+        got = to_np(out)
+        for i, ((off, w, h), j) in enumerate(zip(b.job_slices(), jobs)):
+            s = conv[j["scene"]]
+            o = orc.render(s.nodes(), s.GetRoot(), j["pos"], j["dir"], fov=j["fov"], w=w, h=h,
+                           light_mode=j["light_mode"], light=j["light"], payload=fmt, threads=8)
+            g = {k: job_view(got[k], k, off, w, h) for k in KEYS}
+            assert_same(g, o, KEYS, what=f"payload {fmt} job {i} ({w}x{h})")
+    with pytest.raises(yx.YoxelError):
+        b.set_payload(5)

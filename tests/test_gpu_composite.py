@@ -169,3 +169,47 @@ def test_backward_capacity(yx, cuda, brick):
     torch.cuda.synchronize()
     assert torch.equal(g, g_ref)
     assert small.last_stats()[0] == total
+
+
+def test_table_edited_between_calls(yx, oracle_mod, cuda):
+    """Both walks skip whole empty super-bricks (4^3 bricks) from a summary of
+    the brick table that every call rebuilds: after the table is edited in
+    place (bricks released, others allocated into their slots), a compositor
+    that already ran on the old table gives the oracle's sample counts and
+    the same bits as a fresh compositor, forward and backward."""
+    torch = cuda
+    grid, brick, w, h = 128, 8, 48, 40
+    v, views, table, params, vd = setup_case(yx, torch, grid, brick, 0.15, 21, 3, w, h)
+    comp = yx.composite.Compositor()
+    rgb, _, ns = comp.forward(table, params, grid, brick, vd, w, h)
+    # room for the edited table's records too (its total may be larger)
+    comp.reserve(4 * int(ns.to(torch.int64).sum()) + 4096)
+    comp.backward(table, params, grid, brick, vd, w, h, rgb, ns, torch.ones_like(rgb))
+    torch.cuda.synchronize()
+    # release every allocated brick of one half of the volume and give their
+    # slots to empty bricks of the other half, in place
+    g = grid // brick
+    tab = v.table.copy().reshape(g, g, g)
+    alloc = np.argwhere(tab >= 0)
+    free = np.argwhere(tab < 0)
+    gone = alloc[alloc[:, 2] < g // 2]
+    new = free[free[:, 2] >= g // 2][: len(gone)]
+    slots = tab[gone[:, 0], gone[:, 1], gone[:, 2]].copy()
+    tab[gone[:, 0], gone[:, 1], gone[:, 2]] = -1
+    tab[new[:, 0], new[:, 1], new[:, 2]] = slots[: len(new)]
+    assert len(gone) > 10 and len(new) == len(gone)
+    table.copy_(torch.from_numpy(tab.reshape(-1)))
+    rgb1, tr1, ns1 = comp.forward(table, params, grid, brick, vd, w, h)
+    fresh = yx.composite.Compositor()
+    rgb2, tr2, ns2 = fresh.forward(table, params, grid, brick, vd, w, h)
+    torch.cuda.synchronize()
+    _, _, o_ns = oracle_mod.CompositeOracle().forward(tab.reshape(-1), v.params, grid, brick, views, w, h,
+                                                      threads=8)
+    assert np.array_equal(ns1.cpu().numpy().view(np.uint32), o_ns)
+    assert not np.array_equal(ns1.cpu().numpy(), ns.cpu().numpy()), "the edit must change the samples"
+    assert torch.equal(rgb1, rgb2) and torch.equal(tr1, tr2) and torch.equal(ns1, ns2)
+    gr = torch.sin(rgb1 * 5.0).contiguous()
+    g1 = comp.backward(table, params, grid, brick, vd, w, h, rgb1, ns1, gr)
+    g2 = fresh.backward(table, params, grid, brick, vd, w, h, rgb2, ns2, gr)
+    torch.cuda.synchronize()
+    assert torch.equal(g1, g2)

@@ -296,6 +296,7 @@ SIGNATURES = {
     "yx_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "yx_batch_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(Tuning)]),
     "yx_batch_set_graph": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "yx_batch_set_payload": (ctypes.c_int, [_P, ctypes.c_int32]),
     "yx_render_batch": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), _P]),
     "yx_batch_autotune": (ctypes.c_int, [_P, ctypes.POINTER(Outputs), ctypes.c_int32, ctypes.POINTER(TuneTrial),
                                          ctypes.c_int32, ctypes.POINTER(ctypes.c_int32), _P]),

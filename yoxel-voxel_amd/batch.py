@@ -91,6 +91,10 @@ class BatchRenderer:
     def set_graph(self, use_graph):
         check(lib().yx_batch_set_graph(self._h, 1 if use_graph else 0))
 
+    def set_payload(self, fmt):
+        """YX_PAYLOAD_RGBA8 (default) or YX_PAYLOAD_C16N16 for every scene."""
+        check(lib().yx_batch_set_payload(self._h, int(fmt)))
+
     @staticmethod
     def _outputs(out):
         return _lib.Outputs(*[_addr(out.get(k)) for k in ("hit_id", "voxel", "t", "rgba", "shadow", "visits")])

@@ -46,7 +46,8 @@ struct yx_batch {
     uint32_t total_tiles = 0;
     int table_tile_w = 0, table_tile_h = 0;
     int levels = 1;  // stack rows: deepest node level of any job's scene
-    uint64_t layout_gen = 1;  // changes with scenes, job scenes / resolutions, tile shape
+    uint64_t layout_gen = 1;  // changes with scenes, job scenes / resolutions, tile shape, payload format
+    int payload = YX_PAYLOAD_RGBA8;  // how every scene's leaf payloads are read
     // device
     uint32_t* d_counters = nullptr;
     int device = -1, num_cus = 0;
@@ -250,6 +251,18 @@ extern "C" int yx_batch_set_graph(yx_batch* b, int32_t use_graph) {
     return YX_OK;
 }
 
+extern "C" int yx_batch_set_payload(yx_batch* b, int32_t format) {
+    clear_error();
+    if (!b) return fail(YX_ERR_INVALID_ARG, "null argument");
+    if (format != YX_PAYLOAD_RGBA8 && format != YX_PAYLOAD_C16N16)
+        return fail(YX_ERR_INVALID_ARG, "unknown payload format %d", format);
+    if (format != b->payload) {
+        b->payload = format;
+        ++b->layout_gen;  // the captured launch holds the old format
+    }
+    return YX_OK;
+}
+
 extern "C" int yx_batch_scene_bytes(const yx_batch* b, uint64_t* bytes) {
     clear_error();
     if (!b || !bytes) return fail(YX_ERR_INVALID_ARG, "null argument");
@@ -344,6 +357,7 @@ static int plan_launch(yx_batch* b, const yx_outputs& out, BatchLaunch& L) {
     a.per_q = queue_rays(b->total_tiles, tile, t.queues);
     a.refill_min = t.refill_min;
     a.levels = b->levels;
+    a.payload = b->payload;
     a.jobs = b->d_jobs;
     a.job_tiles = b->d_job_tiles;
     a.n_jobs = (int)b->jobs.size();

@@ -53,7 +53,7 @@ struct PersistArgs {
     float basis[9];
     float light[3];
     int light_mode;
-    int payload;  // YX_PAYLOAD_*: how leaf payloads are shaded (single scene)
+    int payload;  // YX_PAYLOAD_*: how leaf payloads are shaded (a batch's scenes share it)
     int w, h;
     yx_outputs out;
     uint32_t* counters;  // [0, nq): queue heads; [nq]: retired-wave count

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     lam = dmax((de < 0) ? le : -le, 0.0f);
                 }
                 color = leaf.x;
-                if (!BATCH && a.payload == YX_PAYLOAD_C16N16) {  // the leaf's own colour and normal
+                if (a.payload == YX_PAYLOAD_C16N16) {  // the leaf's own colour and normal
                     lam = light_mode == YX_LIGHT_HEADLIGHT ? lambert_c16n16(leaf.x, -dx, -dy, -dz)
                                                            : lambert_c16n16(leaf.x, light[0], light[1], light[2]);
                     color = rgba_from_565(leaf.x & 0xffffu);
