@@ -184,10 +184,19 @@ __global__ __launch_bounds__(kSortThreads) void prim_rs_upsweep(const K* keys, u
     const int lane = (int)(threadIdx.x & 63);
     const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
     if (blockIdx.x * kPrimSortTile < n) {
+        // every round's digit first: the block's 16 loads per thread are in
+        // flight together instead of one round's at a time
+        uint32_t dr[kSortRounds];
+#pragma unroll
+        for (int k = 0; k < kSortRounds; ++k) {
+            const uint32_t i = base + (uint32_t)k * kSortThreads;
+            dr[k] = i < n ? (uint32_t)(keys[i] >> shift) & mask : 0u;
+        }
+#pragma unroll
         for (int k = 0; k < kSortRounds; ++k) {
             const uint32_t i = base + (uint32_t)k * kSortThreads;
             const bool in = i < n;  // the lanes in range are a prefix of the wave
-            const uint32_t d = in ? (uint32_t)(keys[i] >> shift) & mask : 0u;
+            const uint32_t d = dr[k];
             const uint32_t prev = __shfl_up(d, 1u);
             const bool head = in && (lane == 0 || prev != d);
             const uint64_t heads = __ballot(head);
@@ -226,16 +235,22 @@ __global__ __launch_bounds__(kSortThreads) void prim_rs_downsweep(const K* keys_
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t base = blockIdx.x * kPrimSortTile + threadIdx.x;
+    // every round's key and id first, so the loads overlap
+    K kr[kSortRounds];
+    uint32_t ir[IDS ? kSortRounds : 1];
+#pragma unroll
+    for (int k = 0; k < kSortRounds; ++k) {
+        const uint32_t i = base + (uint32_t)k * kSortThreads;
+        kr[k] = i < n ? keys_in[i] : (K)0;
+        if (IDS) ir[k] = i < n ? (ids_in ? ids_in[i] : i) : 0u;
+    }
+#pragma unroll
     for (int k = 0; k < kSortRounds; ++k) {
         const uint32_t i = base + (uint32_t)k * kSortThreads;
         const bool in = i < n;
-        K key = 0;
-        uint32_t id = 0, d = 0;
-        if (in) {
-            key = keys_in[i];
-            if (IDS) id = ids_in ? ids_in[i] : i;
-            d = (uint32_t)(key >> shift) & mask;
-        }
+        const K key = kr[k];
+        const uint32_t id = IDS ? ir[k] : 0u;
+        const uint32_t d = in ? (uint32_t)(key >> shift) & mask : 0u;
         // the per-wave counts alternate between two buffers by round, so the
         // next round's counts never meet this round's clearing
         uint32_t(*wc)[kPrimRadix] = wc_all[k & 1];

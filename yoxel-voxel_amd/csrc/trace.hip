@@ -139,9 +139,10 @@ struct yx_renderer {
     int num_cus = 0;
     size_t max_lds = 0;  // dynamic LDS bytes a workgroup may use
     // measured best on MI355X for the 1080p depth-11 frame (stack kernel, one
-    // wave per workgroup, no register window, refill at 64 idle lanes: a wave
-    // takes a whole 8x8 tile at once and traces its primary rays, then its
-    // shadow rays, in phase)
+    // wave per workgroup, no register window, refill at 64 waiting lanes: a
+    // wave takes rays only when none of its lanes traces, so the lanes that
+    // start together - new primary rays and the previous packet's shadow
+    // rays - trace in phase)
     yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 64, 1, 0, 0, 0};
     // shading pipeline (SVORenderer::Render): per-record mean leaf colours,
     // settings with SVORenderer's defaults (demo/SVORenderer.cpp:6-16), and the
