@@ -55,7 +55,8 @@ __device__ __forceinline__ uint64_t digit_peers(bool in, uint32_t d, int bits) {
 
 // Items per scan block: 256 threads x 16 coalesced rounds.
 constexpr uint32_t kPrimScanChunk = 4096;
-// Keys per sort block: 256 threads x 16 rounds.
+// Keys per sort tile (256 threads x 16 rounds); a sort block takes a run of
+// consecutive tiles.
 constexpr uint32_t kPrimSortTile = 4096;
 constexpr int kPrimRadixBits = 8;
 constexpr int kPrimRadix = 1 << kPrimRadixBits;

@@ -166,12 +166,24 @@ int scan_run(Src src, Dst dst, bool apply, uint64_t n, TotalT* total, void* ws, 
 
 // ---- radix sort ----
 
-// Digit histogram of block b's keys [b * 4096, ...) below n, stored
-// digit-major (hist[d * nb + b]); blocks past the end store zeros. Each run
-// of equal digits among a wave's lanes is counted with one LDS add by its
-// first lane (keys that arrive grouped, such as records emitted tile by tile,
-// share digits with their neighbours; per-lane adds to one counter would
-// serialise).
+// The n keys are cut into tiles of 4096; block b of nb takes the consecutive
+// tiles [b * per, (b + 1) * per), per = ceil(tiles / nb), computed from the
+// count the kernels read (so a count known only on the device still spreads
+// evenly over the blocks). nb stays near the number of resident blocks: the
+// digit-major histogram has 256 * nb entries, and each block writes and reads
+// its 256 entries scattered over it.
+__device__ __forceinline__ void sort_block_tiles(uint32_t n, uint32_t nb, uint32_t& first, uint32_t& last) {
+    const uint32_t tiles = (n + kPrimSortTile - 1) / kPrimSortTile;
+    const uint32_t per = (tiles + nb - 1) / nb;
+    first = std::min(blockIdx.x * per, tiles);
+    last = std::min(first + per, tiles);
+}
+
+// Digit histogram of block b's keys, stored digit-major (hist[d * nb + b]);
+// blocks without keys store zeros. Each run of equal digits among a wave's
+// lanes is counted with one LDS add by its first lane (keys that arrive
+// grouped, such as records emitted tile by tile, share digits with their
+// neighbours; per-lane adds to one counter would serialise).
 template <class K>
 __global__ __launch_bounds__(kSortThreads) void prim_rs_upsweep(const K* keys, uint32_t n_host, const uint32_t* n_dev,
                                                                 int shift, int bits, uint32_t* hist, uint32_t nb) {
@@ -180,10 +192,12 @@ __global__ __launch_bounds__(kSortThreads) void prim_rs_upsweep(const K* keys, u
     __syncthreads();
     const uint32_t n = n_dev ? std::min(*n_dev, n_host) : n_host;
     const uint32_t mask = (1u << bits) - 1u;
-    const uint32_t base = blockIdx.x * kPrimSortTile + threadIdx.x;
     const int lane = (int)(threadIdx.x & 63);
     const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
-    if (blockIdx.x * kPrimSortTile < n) {
+    uint32_t t0, t1;
+    sort_block_tiles(n, nb, t0, t1);
+    for (uint32_t tile = t0; tile < t1; ++tile) {
+        const uint32_t base = tile * kPrimSortTile + threadIdx.x;
         // every round's digit first: the block's 16 loads per thread are in
         // flight together instead of one round's at a time
         uint32_t dr[kSortRounds];
@@ -213,9 +227,10 @@ __global__ __launch_bounds__(kSortThreads) void prim_rs_upsweep(const K* keys, u
 }
 
 // Stable scatter of block b's keys (and ids) by digit; `offs` is the scanned
-// histogram. Rounds take 256 consecutive keys; a key's place among the
-// block's keys of its digit is the count from earlier rounds, from earlier
-// waves of its round and from earlier lanes of its wave (ballot rank).
+// histogram. Rounds take 256 consecutive keys, tile after tile; a key's place
+// among the block's keys of its digit is the count from earlier rounds, from
+// earlier waves of its round and from earlier lanes of its wave (ballot
+// rank).
 // IDS with ids_in null: the ids are the positions.
 template <class K, bool IDS>
 __global__ __launch_bounds__(kSortThreads) void prim_rs_downsweep(const K* keys_in, const uint32_t* ids_in,
@@ -225,7 +240,9 @@ __global__ __launch_bounds__(kSortThreads) void prim_rs_downsweep(const K* keys_
     __shared__ uint32_t run[kPrimRadix];
     __shared__ uint32_t wc_all[2][kSortThreads / 64][kPrimRadix];  // per round parity, per wave
     const uint32_t n = n_dev ? std::min(*n_dev, n_host) : n_host;
-    if (blockIdx.x * kPrimSortTile >= n) return;  // whole block
+    uint32_t t0, t1;
+    sort_block_tiles(n, nb, t0, t1);
+    if (t0 >= t1) return;  // whole block
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
     const uint32_t mask = (1u << bits) - 1u;
     for (int d = (int)threadIdx.x; d < kPrimRadix; d += kSortThreads) {
@@ -234,49 +251,71 @@ __global__ __launch_bounds__(kSortThreads) void prim_rs_downsweep(const K* keys_
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
-    const uint32_t base = blockIdx.x * kPrimSortTile + threadIdx.x;
-    // every round's key and id first, so the loads overlap
-    K kr[kSortRounds];
-    uint32_t ir[IDS ? kSortRounds : 1];
+    for (uint32_t tile = t0; tile < t1; ++tile) {
+        const uint32_t base = tile * kPrimSortTile + threadIdx.x;
+        // every round's key and id first, so the loads overlap
+        K kr[kSortRounds];
+        uint32_t ir[IDS ? kSortRounds : 1];
 #pragma unroll
-    for (int k = 0; k < kSortRounds; ++k) {
-        const uint32_t i = base + (uint32_t)k * kSortThreads;
-        kr[k] = i < n ? keys_in[i] : (K)0;
-        if (IDS) ir[k] = i < n ? (ids_in ? ids_in[i] : i) : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < kSortRounds; ++k) {
-        const uint32_t i = base + (uint32_t)k * kSortThreads;
-        const bool in = i < n;
-        const K key = kr[k];
-        const uint32_t id = IDS ? ir[k] : 0u;
-        const uint32_t d = in ? (uint32_t)(key >> shift) & mask : 0u;
-        // the per-wave counts alternate between two buffers by round, so the
-        // next round's counts never meet this round's clearing
-        uint32_t(*wc)[kPrimRadix] = wc_all[k & 1];
-        const uint64_t peers = digit_peers(in, d, bits);
-        if (in && (peers & below) == 0) wc[w][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (in) {
-            uint32_t pos = run[d] + (uint32_t)__popcll(peers & below);
-            for (int q = 0; q < w; ++q) pos += wc[q][d];
-            keys_out[pos] = key;
-            if (IDS) ids_out[pos] = id;
+        for (int k = 0; k < kSortRounds; ++k) {
+            const uint32_t i = base + (uint32_t)k * kSortThreads;
+            kr[k] = i < n ? keys_in[i] : (K)0;
+            if (IDS) ir[k] = i < n ? (ids_in ? ids_in[i] : i) : 0u;
         }
-        __syncthreads();
-        for (int dd = (int)threadIdx.x; dd < kPrimRadix; dd += kSortThreads) {
-            uint32_t s = 0;
 #pragma unroll
-            for (int q = 0; q < kSortThreads / 64; ++q) {
-                s += wc[q][dd];
-                wc[q][dd] = 0u;
+        for (int k = 0; k < kSortRounds; ++k) {
+            const uint32_t i = base + (uint32_t)k * kSortThreads;
+            const bool in = i < n;
+            const K key = kr[k];
+            const uint32_t id = IDS ? ir[k] : 0u;
+            const uint32_t d = in ? (uint32_t)(key >> shift) & mask : 0u;
+            // the per-wave counts alternate between two buffers by round, so the
+            // next round's counts never meet this round's clearing
+            uint32_t(*wc)[kPrimRadix] = wc_all[k & 1];
+            const uint64_t peers = digit_peers(in, d, bits);
+            if (in && (peers & below) == 0) wc[w][d] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if (in) {
+                uint32_t pos = run[d] + (uint32_t)__popcll(peers & below);
+                for (int q = 0; q < w; ++q) pos += wc[q][d];
+                keys_out[pos] = key;
+                if (IDS) ids_out[pos] = id;
             }
-            run[dd] += s;
+            __syncthreads();
+            for (int dd = (int)threadIdx.x; dd < kPrimRadix; dd += kSortThreads) {
+                uint32_t s = 0;
+#pragma unroll
+                for (int q = 0; q < kSortThreads / 64; ++q) {
+                    s += wc[q][dd];
+                    wc[q][dd] = 0u;
+                }
+                run[dd] += s;
+            }
         }
     }
 }
 
-uint32_t sort_blocks(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + kPrimSortTile - 1) / kPrimSortTile); }
+// Blocks of a sort: one per tile, at most kSortBlocksPerCU per compute unit
+// of the current device, and never more than kSortMaxBlocks, which sizes the
+// workspace. Fewer blocks mean fewer scattered histogram entries (the
+// upsweep's 256 stores and the downsweep's 256 loads per block), more mean
+// better balance; on composite_512's 82 M records 16 per CU (about 5 tiles a
+// block) measured best of 4, 6, 8, 12, 16, 24 and 32.
+constexpr uint32_t kSortBlocksPerCU = 16;
+constexpr uint32_t kSortMaxBlocks = 8192;
+uint32_t resident_sort_blocks() {
+    static int cus_of[64] = {};  // compute units per device, filled on first use (same value on a race)
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+    if (cus_of[dev] <= 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        cus_of[dev] = cus;
+    }
+    return std::min<uint32_t>(kSortMaxBlocks, (uint32_t)cus_of[dev] * kSortBlocksPerCU);
+}
+uint32_t sort_tiles(uint64_t n) { return (uint32_t)std::max<uint64_t>(1, (n + kPrimSortTile - 1) / kPrimSortTile); }
 
 }  // namespace
 
@@ -302,7 +341,7 @@ int reduce_sum_u32(const uint32_t* in, uint64_t n, uint32_t* total, void* ws, si
 }
 
 size_t sort_workspace_bytes(uint64_t n_host) {
-    const uint64_t hist = (uint64_t)kPrimRadix * sort_blocks(n_host);
+    const uint64_t hist = (uint64_t)kPrimRadix * std::min<uint64_t>(sort_tiles(n_host), kSortMaxBlocks);
     return align256(hist * 4) + scan_workspace_bytes(hist);
 }
 
@@ -314,7 +353,7 @@ int radix_sort(const K* keys_in, const uint32_t* ids_in, K* const kbuf[2], uint3
                                              (unsigned long long)n_host);
     if (!ws || ws_bytes < sort_workspace_bytes(n_host)) return fail(YX_ERR_INVALID_ARG, "sort workspace too small");
     if (b0 < 0 || b1 > (int)(8 * sizeof(K))) return fail(YX_ERR_INVALID_ARG, "sort bits [%d, %d)", b0, b1);
-    const uint32_t nb = sort_blocks(n_host);
+    const uint32_t nb = std::min(sort_tiles(n_host), resident_sort_blocks());
     const size_t nhist = (size_t)kPrimRadix * nb;
     uint32_t* hist = (uint32_t*)ws;
     char* scan_ws = (char*)ws + align256(nhist * 4);
