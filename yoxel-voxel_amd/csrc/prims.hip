@@ -169,7 +169,7 @@ int scan_run(Src src, Dst dst, bool apply, uint64_t n, TotalT* total, void* ws, 
 // The n keys are cut into tiles of 4096; block b of nb takes the consecutive
 // tiles [b * per, (b + 1) * per), per = ceil(tiles / nb), computed from the
 // count the kernels read (so a count known only on the device still spreads
-// evenly over the blocks). nb stays near the number of resident blocks: the
+// evenly over the blocks). nb is bounded (see resident_sort_blocks): the
 // digit-major histogram has 256 * nb entries, and each block writes and reads
 // its 256 entries scattered over it.
 __device__ __forceinline__ void sort_block_tiles(uint32_t n, uint32_t nb, uint32_t& first, uint32_t& last) {
