@@ -71,6 +71,8 @@ struct CompArgs {
     float4* vals;
 };
 
+typedef float V4f __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint32_t lanes_rank(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
@@ -402,8 +404,12 @@ __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
                 const float dsig = gr * (tn * q[j].y - sr) + gg * (tn * q[j].z - sg) + gb * (tn * q[j].w - sb);
                 const uint32_t pos = cursor + lanes_rank(mask);
                 if (pos < a.total_records) {
-                    a.keys[pos] = ix[j];
-                    a.vals[pos] = make_float4(dsig * al, wgt * gr, wgt * gg, wgt * gb);
+                    // streaming stores: the records are read back only by the
+                    // sort, after this kernel, so they should not take the L2
+                    // lines of the parameters and table this walk reads
+                    const V4f v = {dsig * al, wgt * gr, wgt * gg, wgt * gb};
+                    __builtin_nontemporal_store(ix[j], a.keys + pos);
+                    __builtin_nontemporal_store(v, reinterpret_cast<V4f*>(a.vals) + pos);
                 }
                 T = tn;
             }
