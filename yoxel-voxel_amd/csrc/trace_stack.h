@@ -376,7 +376,12 @@ __global__ __launch_bounds__(BLOCK) void stack_kernel(PersistArgs a) {
                     write_final(a.out, pix, color, 0.1f + 0.9f * lam, 0, visits);
                 status = sIdle;
             }
-            const uint64_t need = __ballot(status == sIdle);
+            // shadow rays and primary rays go in separate packets: while this
+            // block starts shadow rays, the idle lanes wait instead of taking
+            // the next primary rays, so a packet is either the shadow rays of
+            // one packet's lit hits or a whole run of 64 primary rays (a full
+            // 8x8 tile); mixed packets ran 4 % longer on the 1080p frame
+            const uint64_t need = __ballot(status == sStartS) != 0 ? 0ull : __ballot(status == sIdle);
             if (need != 0) {
                 if (qc.left == 0) {
                     if (status == sIdle) status = sRetired;
