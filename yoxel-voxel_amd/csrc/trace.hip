@@ -141,8 +141,8 @@ struct yx_renderer {
     // measured best on MI355X for the 1080p depth-11 frame (stack kernel, one
     // wave per workgroup, no register window, refill at 64 waiting lanes: a
     // wave takes rays only when none of its lanes traces, so the lanes that
-    // start together - new primary rays and the previous packet's shadow
-    // rays - trace in phase)
+    // start together - a whole tile of primary rays, or the shadow rays of
+    // the last tile's lit hits - trace in phase)
     yx_tuning tune = {YX_KERNEL_PERSISTENT, 64, 8, 8, 64, 1, 0, 0, 0};
     // shading pipeline (SVORenderer::Render): per-record mean leaf colours,
     // settings with SVORenderer's defaults (demo/SVORenderer.cpp:6-16), and the
