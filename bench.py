@@ -541,7 +541,7 @@ def update_bench(yx, torch, scene, reps=3, with_edit=True):
             "speedup_edit_vs_set_scene_host": round(host_ms / min(edit_ms), 1), "sphere_edits": edit}
 
 
-def edit_bench(yx, torch, wall, set_scene_ms, reps=3):
+def edit_bench(yx, torch, wall, set_scene_ms, reps=5):
     """Demo::DoEdit on large_1080p (demo/Demo.cpp:82-114): 100 sphere edits of
     radius 4 at level 11, centred on voxels that primary rays of an orbit view
     hit, applied with BuildRange in place on the host, then UpdateScene
