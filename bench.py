@@ -93,6 +93,8 @@ def main():
                                  not args.no_cpu_baseline)
         else:
             leg = brick_bench(yx, torch, args.brick_frames, not args.no_cpu_baseline)
+        if args.copy_calibration:
+            copy_calibration(torch, dev)
         print(json.dumps(leg), flush=True)
         return
     if args.workload == "large_1080p":
@@ -198,12 +200,7 @@ def main():
                   "value": round(float(np.sum(rays_f)) / (all_kernel_s * nframes) / 1e6, 3)}
 
     if args.copy_calibration:
-        src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
+        copy_calibration(torch, dev)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -259,6 +256,17 @@ def main():
         "voxtree": voxtree,
     }
     print(json.dumps(line), flush=True)
+
+
+def copy_calibration(torch, dev):
+    """Three 256 MiB device-to-device copies: a kernel of known traffic for
+    scaling rocprofv3's FETCH_SIZE / WRITE_SIZE."""
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
 
 
 def composite_bench(yx, torch, dev, steps, with_cpu=False):
