@@ -648,6 +648,33 @@ int yx_ntree_render(const yx_ntree* t, const float view[12], int32_t width, int3
  * reference counterpart (launch shape). */
 int yx_ntree_set_launch(yx_ntree* t, int32_t persistent, int32_t refill_min);
 
+/* ---- device primitives ----
+ * The scans, stable radix sort and select-distinct that the scene builder,
+ * the re-layout, the n-tree builder, the brick pool and the compositor's
+ * adjoint run, exposed so that they can be checked against a host sort or
+ * scan. No reference counterpart: the reference's GPU paths would call a
+ * library for these. Every pointer is device memory; each call allocates its
+ * workspace, runs on `stream` and waits for it to finish. */
+/* Stable LSD sort of n keys by bits [begin_bit, end_bit) (8-bit digits) into
+ * keys_out, carrying ids (NULL: the positions 0..n-1) into ids_out (NULL:
+ * keys only). n_dev (NULL: n) is a device count read by the kernels; only the
+ * first min(*n_dev, n) entries are sorted and defined. n < 2^31. */
+int yx_prim_sort_u32(const uint32_t* keys, const uint32_t* ids, uint64_t n, const uint32_t* n_dev,
+                     int32_t begin_bit, int32_t end_bit, uint32_t* keys_out, uint32_t* ids_out, void* stream);
+int yx_prim_sort_u64(const uint64_t* keys, const uint32_t* ids, uint64_t n, const uint32_t* n_dev,
+                     int32_t begin_bit, int32_t end_bit, uint64_t* keys_out, uint32_t* ids_out, void* stream);
+/* Exclusive prefix sums out[i] = in[0] + ... + in[i-1] (out may alias in);
+ * *total (may be NULL) receives the sum of all n. u32 sums wrap. n < 2^32. */
+int yx_prim_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* total, void* stream);
+int yx_prim_scan_u8(const uint8_t* in, uint32_t* out, uint64_t n, uint32_t* total, void* stream);
+int yx_prim_scan_u32_u64(const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* total, void* stream);
+/* *total = sum of the n items (u32 wrap). */
+int yx_prim_reduce_u32(const uint32_t* in, uint64_t n, uint32_t* total, void* stream);
+/* The distinct values of in[i] >> shift of a sorted array, in order, into
+ * out; *count receives their number. */
+int yx_prim_distinct_u64(const uint64_t* in, uint64_t* out, uint64_t n, int32_t shift, uint64_t* count,
+                         void* stream);
+
 const char* yx_last_error(void);
 const char* yx_version(void);
 

@@ -321,6 +321,13 @@ SIGNATURES = {
     "yx_ntree_download": (ctypes.c_int, [_P, _P, _P, _P]),
     "yx_ntree_free": (None, [_P]),
     "yx_ntree_set_launch": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "yx_prim_sort_u32": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
+    "yx_prim_sort_u64": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
+    "yx_prim_scan_u32": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, _P]),
+    "yx_prim_scan_u8": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, _P]),
+    "yx_prim_scan_u32_u64": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P, _P]),
+    "yx_prim_reduce_u32": (ctypes.c_int, [_P, ctypes.c_uint64, _P, _P]),
+    "yx_prim_distinct_u64": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.c_int32, _P, _P]),
     "yx_ntree_render": (ctypes.c_int, [_P, _F3, ctypes.c_int32, ctypes.c_int32, _P, ctypes.POINTER(NTreeOutputs),
                                        _P]),
     "yx_last_error": (ctypes.c_char_p, []),
