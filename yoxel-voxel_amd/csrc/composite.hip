@@ -293,7 +293,7 @@ template <bool SMALL>
 __global__ __launch_bounds__(256) void comp_forward_kernel(CompArgs a) {
     int view, x, y;
     size_t ray;
-    if (!ray_of(a, (size_t)blockIdx.x * 256 + threadIdx.x, view, x, y, ray)) return;
+    if (!ray_of(a, (size_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x, view, x, y, ray)) return;
     float cr = 0.0f, cg = 0.0f, cb = 0.0f, T = 1.0f;
     uint32_t n = 0;
     SampleIter<SMALL> it;
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void comp_wave_sums_kernel(CompArgs a, uint32_
 // stores coalesce and the record order is fixed by the inputs alone.
 template <bool SMALL>
 __global__ __launch_bounds__(256) void comp_backward_kernel(CompArgs a) {
-    const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const size_t t = (size_t)xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     const size_t wave = t >> 6;
     if (wave >= a.n_waves) return;  // whole wave past the end
     int view = 0, x = 0, y = 0;

@@ -279,4 +279,18 @@ __device__ bool trace_ray(const uint2* __restrict__ recs, const float p[3], cons
     }
 }
 
+// Workgroup order for one-tile-per-wave launches. The hardware hands
+// workgroup b to XCD b mod 8, so consecutive workgroups land on different
+// XCDs, each with its own L2. Workgroup b instead takes block b / 8 of XCD
+// (b mod 8)'s share, one contiguous eighth of the grid, so each XCD walks one
+// run of neighbouring tiles and its L2 holds what those rays share. A
+// bijection on the first 8 * floor(nb / 8) blocks; the rest keep their index.
+// Callers whose outputs depend only on the tile get the same results in any
+// order.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t full = nb & ~7u;
+    if (b >= full) return b;
+    return (b & 7u) * (full >> 3) + (b >> 3);
+}
+
 }  // namespace yx
