@@ -227,69 +227,82 @@ __global__ __launch_bounds__(kSortThreads) void prim_rs_upsweep(const K* keys, u
 }
 
 // Stable scatter of block b's keys (and ids) by digit; `offs` is the scanned
-// histogram. Rounds take 256 consecutive keys, tile after tile; a key's place
-// among the block's keys of its digit is the count from earlier rounds, from
-// earlier waves of its round and from earlier lanes of its wave (ballot
-// rank).
+// histogram. Each wave of the block takes one contiguous quarter of a tile
+// (16 coalesced rounds of 64 keys) and ranks its keys among its own keys of
+// the same digit: earlier rounds through a per-wave count in LDS, the same
+// round by ballot rank. One barrier later, a thread per digit turns the four
+// waves' counts into their bases (the block's keys of that digit from earlier
+// tiles, then the earlier waves' of this tile), and after a second barrier
+// every lane writes its 16 keys. Two barriers per tile of 4096 keys.
 // IDS with ids_in null: the ids are the positions.
 template <class K, bool IDS>
 __global__ __launch_bounds__(kSortThreads) void prim_rs_downsweep(const K* keys_in, const uint32_t* ids_in,
                                                                   K* keys_out, uint32_t* ids_out, uint32_t n_host,
                                                                   const uint32_t* n_dev, int shift, int bits,
                                                                   const uint32_t* offs, uint32_t nb) {
-    __shared__ uint32_t run[kPrimRadix];
-    __shared__ uint32_t wc_all[2][kSortThreads / 64][kPrimRadix];  // per round parity, per wave
+    constexpr int kWaves = kSortThreads / 64;
+    static_assert(kSortThreads == kPrimRadix, "one thread per digit in the base step");
+    __shared__ uint32_t run[kPrimRadix];             // the block's keys per digit so far (with the offset)
+    __shared__ uint32_t wcnt[kWaves][kPrimRadix];    // this tile's keys per wave and digit
+    __shared__ uint32_t wbase[kWaves][kPrimRadix];   // where each wave's keys of a digit start
     const uint32_t n = n_dev ? std::min(*n_dev, n_host) : n_host;
     uint32_t t0, t1;
     sort_block_tiles(n, nb, t0, t1);
     if (t0 >= t1) return;  // whole block
     const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
     const uint32_t mask = (1u << bits) - 1u;
-    for (int d = (int)threadIdx.x; d < kPrimRadix; d += kSortThreads) {
+    {
+        const int d = (int)threadIdx.x;
         run[d] = offs[(size_t)d * nb + blockIdx.x];
-        for (int q = 0; q < kSortThreads / 64; ++q) wc_all[0][q][d] = wc_all[1][q][d] = 0u;
+        for (int q = 0; q < kWaves; ++q) wcnt[q][d] = 0u;
     }
     __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
     for (uint32_t tile = t0; tile < t1; ++tile) {
-        const uint32_t base = tile * kPrimSortTile + threadIdx.x;
+        const uint32_t base = tile * kPrimSortTile + (uint32_t)w * (kSortRounds * 64) + (uint32_t)lane;
         // every round's key and id first, so the loads overlap
         K kr[kSortRounds];
         uint32_t ir[IDS ? kSortRounds : 1];
 #pragma unroll
         for (int k = 0; k < kSortRounds; ++k) {
-            const uint32_t i = base + (uint32_t)k * kSortThreads;
+            const uint32_t i = base + (uint32_t)k * 64u;
             kr[k] = i < n ? keys_in[i] : (K)0;
             if (IDS) ir[k] = i < n ? (ids_in ? ids_in[i] : i) : 0u;
         }
+        uint32_t rk[kSortRounds];
 #pragma unroll
         for (int k = 0; k < kSortRounds; ++k) {
-            const uint32_t i = base + (uint32_t)k * kSortThreads;
+            const uint32_t i = base + (uint32_t)k * 64u;
             const bool in = i < n;
-            const K key = kr[k];
-            const uint32_t id = IDS ? ir[k] : 0u;
-            const uint32_t d = in ? (uint32_t)(key >> shift) & mask : 0u;
-            // the per-wave counts alternate between two buffers by round, so the
-            // next round's counts never meet this round's clearing
-            uint32_t(*wc)[kPrimRadix] = wc_all[k & 1];
+            const uint32_t d = in ? (uint32_t)(kr[k] >> shift) & mask : 0u;
             const uint64_t peers = digit_peers(in, d, bits);
-            if (in && (peers & below) == 0) wc[w][d] = (uint32_t)__popcll(peers);
-            __syncthreads();
-            if (in) {
-                uint32_t pos = run[d] + (uint32_t)__popcll(peers & below);
-                for (int q = 0; q < w; ++q) pos += wc[q][d];
-                keys_out[pos] = key;
-                if (IDS) ids_out[pos] = id;
-            }
-            __syncthreads();
-            for (int dd = (int)threadIdx.x; dd < kPrimRadix; dd += kSortThreads) {
-                uint32_t s = 0;
+            const uint32_t before = wcnt[w][d];
+            rk[k] = before + (uint32_t)__popcll(peers & below);
+            // the lowest lane of each digit group advances the wave's count;
+            // the next round's read sees it (a wave's LDS accesses are in order)
+            if (in && (peers & below) == 0) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+        }
+        __syncthreads();
+        {
+            const int d = (int)threadIdx.x;
+            uint32_t s = run[d];
 #pragma unroll
-                for (int q = 0; q < kSortThreads / 64; ++q) {
-                    s += wc[q][dd];
-                    wc[q][dd] = 0u;
-                }
-                run[dd] += s;
+            for (int q = 0; q < kWaves; ++q) {
+                wbase[q][d] = s;
+                s += wcnt[q][d];
+                wcnt[q][d] = 0u;  // for the next tile (its counts start after the next barrier)
+            }
+            run[d] = s;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kSortRounds; ++k) {
+            const uint32_t i = base + (uint32_t)k * 64u;
+            if (i < n) {
+                const uint32_t d = (uint32_t)(kr[k] >> shift) & mask;
+                const uint32_t pos = wbase[w][d] + rk[k];
+                keys_out[pos] = kr[k];
+                if (IDS) ids_out[pos] = ir[k];
             }
         }
     }
