@@ -184,7 +184,9 @@ __device__ __forceinline__ void sort_block_tiles(uint32_t n, uint32_t nb, uint32
 // lanes is counted with one LDS add by its first lane (keys that arrive
 // grouped, such as records emitted tile by tile, share digits with their
 // neighbours; per-lane adds to one counter would serialise).
-template <class K>
+// VEC (keys 16-byte aligned): each thread loads 16 consecutive bytes of keys
+// per round and adds each run of equal digits among them with one LDS add.
+template <class K, bool VEC>
 __global__ __launch_bounds__(kSortThreads) void prim_rs_upsweep(const K* keys, uint32_t n_host, const uint32_t* n_dev,
                                                                 int shift, int bits, uint32_t* hist, uint32_t nb) {
     __shared__ uint32_t h[kPrimRadix];
@@ -196,6 +198,45 @@ __global__ __launch_bounds__(kSortThreads) void prim_rs_upsweep(const K* keys, u
     const uint64_t above = lane == 63 ? 0ull : (~0ull << (lane + 1));
     uint32_t t0, t1;
     sort_block_tiles(n, nb, t0, t1);
+    if constexpr (VEC) {
+        constexpr int E = 16 / (int)sizeof(K);                             // keys per load
+        constexpr int R = (int)(kPrimSortTile / (kSortThreads * E));       // loads per tile
+        struct alignas(16) Vec {
+            K k[E];
+        };
+        constexpr uint32_t kNone = 0xFFFFFFFFu;  // no key (beyond n)
+        for (uint32_t tile = t0; tile < t1; ++tile) {
+            uint32_t dr[R][E];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const uint32_t i0 = tile * kPrimSortTile + ((uint32_t)r * kSortThreads + threadIdx.x) * (uint32_t)E;
+                if (i0 + (uint32_t)E <= n) {
+                    const Vec v = *reinterpret_cast<const Vec*>(keys + i0);
+#pragma unroll
+                    for (int j = 0; j < E; ++j) dr[r][j] = (uint32_t)(v.k[j] >> shift) & mask;
+                } else {
+#pragma unroll
+                    for (int j = 0; j < E; ++j)
+                        dr[r][j] = i0 + (uint32_t)j < n ? (uint32_t)(keys[i0 + j] >> shift) & mask : kNone;
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                uint32_t cur = dr[r][0], cnt = 1u;
+#pragma unroll
+                for (int j = 1; j < E; ++j) {
+                    if (dr[r][j] == cur) {
+                        ++cnt;
+                    } else {
+                        if (cur != kNone) atomicAdd(&h[cur], cnt);
+                        cur = dr[r][j];
+                        cnt = 1u;
+                    }
+                }
+                if (cur != kNone) atomicAdd(&h[cur], cnt);
+            }
+        }
+    } else
     for (uint32_t tile = t0; tile < t1; ++tile) {
         const uint32_t base = tile * kPrimSortTile + threadIdx.x;
         // every round's digit first: the block's 16 loads per thread are in
@@ -377,8 +418,12 @@ int radix_sort(const K* keys_in, const uint32_t* ids_in, K* const kbuf[2], uint3
     int pass = 0;
     for (int sh = b0; sh < b1; sh += kPrimRadixBits, ++pass) {
         const int bits = std::min(kPrimRadixBits, b1 - sh);
-        hipLaunchKernelGGL(prim_rs_upsweep<K>, dim3(nb), dim3(kSortThreads), 0, st, kin, (uint32_t)n_host, n_dev, sh,
-                           bits, hist, nb);
+        if (((uintptr_t)kin & 15u) == 0)
+            hipLaunchKernelGGL((prim_rs_upsweep<K, true>), dim3(nb), dim3(kSortThreads), 0, st, kin, (uint32_t)n_host,
+                               n_dev, sh, bits, hist, nb);
+        else
+            hipLaunchKernelGGL((prim_rs_upsweep<K, false>), dim3(nb), dim3(kSortThreads), 0, st, kin, (uint32_t)n_host,
+                               n_dev, sh, bits, hist, nb);
         const int rc = scan_excl_u32(hist, hist, nhist, nullptr, scan_ws, scan_bytes, st);
         if (rc != YX_OK) return rc;
         if (ids)
