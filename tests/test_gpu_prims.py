@@ -36,8 +36,8 @@ def _keys(rng, kind, n, dtype):
     if kind == "few":  # many equal keys: only stability orders them
         return (rng.integers(0, 7, size=n, dtype=dtype) * dtype(0x01010101 if bits == 32 else 0x0101010101010101))
     if kind == "runs":  # keys that arrive in runs, as records emitted tile by tile do
-        vals = rng.integers(0, hi, size=max(1, n // 5 + 1), dtype=dtype, endpoint=True)
-        return np.repeat(vals, rng.integers(1, 10, size=vals.size))[:n] if n else vals[:0]
+        vals = rng.integers(0, hi, size=max(1, n), dtype=dtype, endpoint=True)
+        return np.repeat(vals, rng.integers(1, 10, size=vals.size))[:n]  # every repeat >= 1: n keys
     if kind == "sorted":
         return np.sort(rng.integers(0, hi, size=n, dtype=dtype, endpoint=True))
     if kind == "reversed":
@@ -126,6 +126,28 @@ def test_sort_device_count(yx, cuda, n, n_dev):
     m = min(n_dev, n)
     ek, ei = _expect(keys[:m], 9, 25)
     assert np.array_equal(k[:m], ek) and np.array_equal(i[:m], ei)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.uint32, np.uint64])
+@pytest.mark.parametrize("n", [5, 4099, 100_003])
+def test_sort_unaligned_keys(yx, cuda, dtype, n):
+    """Keys that start one element past a 16-byte boundary take the upsweep's
+    one-key-per-lane path (aligned keys are read 16 bytes at a time)."""
+    lib = yx.lib()
+    rng = np.random.default_rng(n)
+    keys = _keys(rng, "runs", n + 1, dtype)
+    assert keys.size == n + 1
+    dk = _dev(cuda, keys)
+    assert dk.data_ptr() % 16 == 0
+    ko = cuda.empty(n, dtype=dk.dtype, device="cuda")
+    io = cuda.empty(n, dtype=cuda.int32, device="cuda")
+    fn = lib.yx_prim_sort_u32 if dtype == np.uint32 else lib.yx_prim_sort_u64
+    bits = (3, 27) if dtype == np.uint32 else (5, 45)
+    yx._lib.check(fn(dk.data_ptr() + keys.itemsize, None, n, None, bits[0], bits[1], ko.data_ptr(),
+                     io.data_ptr(), _stream(cuda)))
+    ek, ei = _expect(keys[1:], *bits)
+    assert np.array_equal(_host(ko, dtype), ek) and np.array_equal(_host(io, np.uint32), ei)
 
 
 @pytest.mark.gpu
