@@ -8,10 +8,10 @@
 // select-distinct scatter need no intermediate arrays.
 //
 // The radix sort takes 8-bit digits from the least significant end: per pass
-// a digit histogram per block of 4096 keys (stored digit-major, so one
-// exclusive scan of it gives every (digit, block) pair its first output
-// position) and a scatter that ranks equal digits in key order with wave
-// ballots, so every pass is stable.
+// a digit histogram per block (a block takes a run of tiles of 4096 keys; the
+// histogram is stored digit-major, so one exclusive scan of it gives every
+// (digit, block) pair its first output position) and a scatter that ranks
+// equal digits in key order with wave ballots, so every pass is stable.
 #include "prims.h"
 
 #include <algorithm>
