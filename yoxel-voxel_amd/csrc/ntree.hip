@@ -244,7 +244,9 @@ struct NtRay {
     uint32_t brick;  // index of the brick the ray is in (its 512 voxels at bricks + 512 * brick)
     // per level the node last read: its cell (kx | ky << 10 | kz << 20; the
     // coordinates stay below 2^9 for sizes up to 16384, so all ones means
-    // none), child masks and child base
+    // none), child masks and child base. The deepest level keeps only its
+    // cell (to count reads of a new node): its node is read again at every
+    // lookup, which leaves the cast at 64 VGPRs and 8 waves per SIMD
     uint32_t nck[L > 0 ? L : 1];
     uint32_t ncm_lo[L > 0 ? L : 1], ncm_hi[L > 0 ? L : 1], ncb[L > 0 ? L : 1];
 };
@@ -321,18 +323,32 @@ __device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_ro
                 const int sn = 3 + 2 * (L - l + 1);
                 const uint32_t key = (uint32_t)(r.vx >> sn) | ((uint32_t)(r.vy >> sn) << 10) |
                                      ((uint32_t)(r.vz >> sn) << 20);
-                if (r.nck[l - 1] != key) {
+                uint32_t mlo, mhi, nbase;
+                if (l == L) {  // deepest level: read again each time, no registers held
                     const uint4 nd = a.nodes[idx];
-                    ++r.nn;
-                    r.ncm_lo[l - 1] = nd.x;
-                    r.ncm_hi[l - 1] = nd.y;
-                    r.ncb[l - 1] = nd.z;
-                    r.nck[l - 1] = key;
+                    if (r.nck[l - 1] != key) {
+                        ++r.nn;
+                        r.nck[l - 1] = key;
+                    }
+                    mlo = nd.x;
+                    mhi = nd.y;
+                    nbase = nd.z;
+                } else {
+                    if (r.nck[l - 1] != key) {
+                        const uint4 nd = a.nodes[idx];
+                        ++r.nn;
+                        r.ncm_lo[l - 1] = nd.x;
+                        r.ncm_hi[l - 1] = nd.y;
+                        r.ncb[l - 1] = nd.z;
+                        r.nck[l - 1] = key;
+                    }
+                    mlo = r.ncm_lo[l - 1];
+                    mhi = r.ncm_hi[l - 1];
+                    nbase = r.ncb[l - 1];
                 }
                 lS -= 2;
                 const int s = sn - 2;
                 const int c = ((((r.vz >> s) & 3) * 4) + ((r.vy >> s) & 3)) * 4 + ((r.vx >> s) & 3);
-                const uint32_t mlo = r.ncm_lo[l - 1], mhi = r.ncm_hi[l - 1];
                 const bool set = c < 32 ? ((mlo >> c) & 1u) : ((mhi >> (c - 32)) & 1u);
                 if (!set) {
                     idx = kNtEmpty;
@@ -340,7 +356,7 @@ __device__ __forceinline__ bool nt_iterate(const NtArgs& a, const uint32_t* s_ro
                     const uint32_t below = c < 32 ? __builtin_popcount(mlo & ((1u << c) - 1u))
                                                   : __builtin_popcount(mlo) +
                                                         __builtin_popcount(mhi & ((1u << (c - 32)) - 1u));
-                    idx = r.ncb[l - 1] + below;
+                    idx = nbase + below;
                 }
             }
         }
